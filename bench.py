@@ -1,0 +1,197 @@
+"""Benchmark: one GPU's share at the reference README's scale.
+
+Workload: 2^30 16-byte {uint64 key uniform over [0, 2^64), uint64 val =
+index} pairs (16 GiB in A + 16 GiB ping-pong B), keys from pcg64 over 256
+virtual ranks exactly like the reference keygen (mpi/mpi_lsbsort.cpp:
+676-704), sorted by the full 64-bit key, stable.  A step is one lsb_sort of
+that array: the input is restored from a pristine device copy before every
+step (untimed), then the sort is timed between two device synchronisations.
+Metric: Melem/s sorted = n / t_sort / 1e6 (mpi/mpi_lsbsort.cpp:741).
+
+roofline: the dominant kernel is the onesweep scatter pass.  Its
+algorithmic bytes are 32 B per element per executed pass (16 B read + 16 B
+written); its duration is read from HIP events recorded on the sort's stream
+between launches (lsb_profile_read), averaged over the timed steps.  The
+whole sort is also reported against the model n * (8 + 32 * P_exec) bytes
+(one 8-byte key read for the all-digit histogram, 32 B per executed pass).
+
+cpu_baseline: the reference's own mpi_lsbsort (oracle/_ref, built from its
+sources) under mpirun on this host's cores at a bounded sample size, before
+the GPU is touched; if it cannot run, the oracle's single-thread C LSD sort
+(a port) is timed instead.
+"""
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "distributed-lsb_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Melem/s sorted, 2^30 16B pairs, % HBM roofline; served p50/p99 vs plain"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sample_n, max_cores, reps):
+    """Reference mpi_lsbsort on the host (kind=reference), else the oracle port."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "mpi_lsbsort")
+    mpirun = shutil.which("mpirun") or "/opt/conda/bin/mpirun"
+    cores = max(1, min(max_cores, len(os.sched_getaffinity(0))))
+    if os.path.exists(ref) and os.path.exists(mpirun):
+        rates = []
+        try:
+            for _ in range(reps):
+                p = subprocess.run([mpirun, "-n", str(cores), ref, "--n", str(sample_n),
+                                    "--no-verify"], capture_output=True, text=True, timeout=300)
+                m = re.search(r"That's ([0-9.eE+]+) M elements sorted / s", p.stdout)
+                if p.returncode != 0 or not m:
+                    raise RuntimeError(p.stdout[-500:] + p.stderr[-500:])
+                rates.append(float(m.group(1)))
+            rates.sort()
+            return {"value": rates[len(rates) // 2], "unit": "Melem/s", "cores": cores,
+                    "kind": "reference",
+                    "sample": f"mpi_lsbsort --n {sample_n} --no-verify, mpirun -n {cores}, "
+                              f"median of {reps} (reference keygen pcg64(rank), 64-bit keys)"}
+        except Exception as e:  # fall through to the port
+            log("reference CPU baseline failed:", e)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    n = min(sample_n, 1 << 24)
+    a = O.gen_uniform(n, ranks=4)
+    t = time.perf_counter()
+    O.lsd_sort(a, 64, 16)
+    dt = time.perf_counter() - t
+    return {"value": n / dt / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
+            "sample": f"oracle C LSD sort (16-bit digits, one thread), n={n}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log2n", type=int, default=30)
+    ap.add_argument("--digit-bits", type=int, default=0)
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-log2n", type=int, default=26)
+    ap.add_argument("--no-verify", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the sort is one rank's share on one MI355X "
+              "(the multi-GPU exchange is design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    base = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        base = cpu_baseline(1 << args.cpu_sample_log2n, 16, 3)
+        log("cpu baseline:", base)
+
+    import torch
+    import lsb_amd
+
+    n = 1 << args.log2n
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    log(f"generating n={n} on {torch.cuda.get_device_name(0)} ...")
+    pristine = torch.empty((n, 2), dtype=torch.int64, device=dev)
+    lsb_amd.generate(pristine, kind=lsb_amd.GEN_UNIFORM, ranks=256)
+    A = torch.empty_like(pristine)
+    B = torch.empty_like(pristine)
+    d, tile, P = lsb_amd.plan(n, digit_bits=args.digit_bits, tile=args.tile)
+    torch.cuda.synchronize()
+
+    def step(profile):
+        A.copy_(pristine)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lsb_amd.sort(A, B, digit_bits=args.digit_bits, tile=args.tile, profile=profile)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    for i in range(args.warmup):
+        step(False)
+    log(f"warmup done ({args.warmup}); timing {args.steps} steps ...")
+    wall, profs = [], []
+    for i in range(args.steps):
+        wall.append(step(True))
+        profs.append(lsb_amd.profile_read())
+    ms_wall = sum(wall) / len(wall) * 1e3
+    ms_event = sum(p["total_ms"] for p in profs) / len(profs)
+
+    executed = profs[-1]["executed"]
+    p_exec = sum(executed)
+    pass_ms = [t for p in profs for t, e in zip(p["pass_ms"], p["executed"]) if e]
+    avg_pass_ms = sum(pass_ms) / len(pass_ms) if pass_ms else float("nan")
+    hist_ms = sum(p["histogram_ms"] for p in profs) / len(profs)
+    pass_bytes = 32 * n
+    achieved = pass_bytes / (avg_pass_ms * 1e-3) / 1e9
+    model_bytes = n * (8 + 32 * p_exec)
+    sort_gbs = model_bytes / (ms_wall * 1e-3) / 1e9
+
+    verified = None
+    if not args.no_verify:
+        keys = torch.empty(n, dtype=torch.int64, device=dev)
+        lsb_amd.extract_keys(pristine, keys)
+        bad, first = lsb_amd.verify(A, keys)
+        verified = bad == 0
+        del keys
+        if not verified:
+            log(f"VERIFY FAILED: {bad} bad positions, first {first}")
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "onesweep_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            tj = json.load(open(tpath))
+            if tj.get("log2n") == args.log2n and tj.get("digit_bits") == d and tj.get("tile") == tile:
+                traffic = tj.get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    value = n / (ms_wall * 1e-3) / 1e6
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Melem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_wall, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (pcg64 keys over 256 virtual ranks, reference keygen)",
+        "config": {"workload": f"plain LSD sort, 2^{args.log2n} 16-byte pairs, uniform 64-bit "
+                               "keys, val = index, stable, HBM-resident",
+                   "n": n, "digit_bits": d, "tile": tile, "passes": P,
+                   "passes_executed": p_exec},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": f"k_onesweep<{d}> scatter pass, 32 B/elem x 2^{args.log2n}",
+                     "avg_launch_ms": round(avg_pass_ms, 4)},
+        "sort_roofline": {"model_bytes": model_bytes, "achieved": round(sort_gbs, 1),
+                          "frac": round(sort_gbs / HBM_PEAK_GBS, 4),
+                          "ms_event": round(ms_event, 4), "histogram_ms": round(hist_ms, 4)},
+        "verified_bit_exact": verified,
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

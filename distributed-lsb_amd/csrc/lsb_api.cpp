@@ -1,0 +1,442 @@
+// C ABI of the sort (include/lsb.h): argument checks, pass planning,
+// workspace management and the launch sequence that replaces mySort
+// (mpi/mpi_lsbsort.cpp:591-597).
+//
+// One sort is, all on the caller's stream and without host synchronisation:
+//   k_zero      counters
+//   k_histogram every digit's counts in one read of the keys
+//   k_scan_plan bucket starts per digit, skip mask, pass plan
+//   k_onesweep  once per digit (a skipped digit's launch exits at once)
+//   k_copy_back result B -> A if an odd number of passes ran
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string.h>
+#include <utility>
+#include <vector>
+#include <math.h>
+
+#include "../../include/lsb.h"
+#include "lsb_device.hpp"
+#include "lsb_internal.hpp"
+
+namespace {
+
+thread_local int g_last_hip = 0;
+
+#define LSB_HIP(x)                      \
+  do {                                  \
+    hipError_t e_ = (x);                \
+    if (e_ != hipSuccess) {             \
+      g_last_hip = (int)e_;             \
+      (void)hipGetLastError();          \
+      return LSB_EHIP;                  \
+    }                                   \
+  } while (0)
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+struct Plan {
+  uint32_t key_bits, d, tile, passes, nb, ntiles;
+  const lsb::SortVariant* v;
+  // workspace offsets
+  size_t counts, starts, plan, tickets, status, total;
+};
+
+lsb_config defaults() {
+  lsb_config c;
+  memset(&c, 0, sizeof c);
+  c.key_bits = 64;
+  return c;
+}
+
+// Auto choice of digit width and tile.  d = 8 keeps the per-bucket runs of
+// a tile long (coalesced writes); d = 11 saves passes on wide keys.
+void auto_choice(size_t n, uint32_t key_bits, uint32_t* d, uint32_t* tile) {
+  (void)key_bits;
+  *d = 8;
+  *tile = n >= ((size_t)1 << 22) ? 256 * 16 : 256 * 16;
+}
+
+int make_plan(size_t n, const lsb_config& c, Plan* p) {
+  p->key_bits = c.key_bits ? c.key_bits : 64;
+  if (p->key_bits > 64) return LSB_EINVAL;
+  uint32_t d = c.digit_bits, tile = c.tile;
+  if (d == 0 && tile == 0) {
+    auto_choice(n, p->key_bits, &d, &tile);
+  } else if (d == 0) {
+    d = 8;
+  }
+  int nv = 0;
+  const lsb::SortVariant* vs = lsb::sort_variants(&nv);
+  const lsb::SortVariant* pick = nullptr;
+  for (int i = 0; i < nv; ++i) {
+    if (vs[i].digit_bits != d) continue;
+    if (tile == 0 || vs[i].tile == tile) {
+      pick = &vs[i];
+      break;
+    }
+  }
+  if (!pick) return LSB_EINVAL;
+  p->v = pick;
+  p->d = d;
+  p->tile = pick->tile;
+  p->passes = (p->key_bits + d - 1) / d;
+  p->nb = 1u << d;
+  p->ntiles = (uint32_t)((n + p->tile - 1) / p->tile);
+  size_t off = 0;
+  p->counts = off;  off += align_up((size_t)p->passes * p->nb * 4);
+  p->starts = off;  off += align_up((size_t)p->passes * p->nb * 4);
+  p->plan = off;    off += align_up((size_t)(4 * p->passes + 4) * 4);
+  p->tickets = off; off += align_up((size_t)p->passes * 4);
+  p->status = off;  off += align_up(2 * (size_t)p->ntiles * p->nb * 4);
+  p->total = off;
+  return LSB_OK;
+}
+
+bool overlaps(const void* a, const void* b, size_t bytes) {
+  const char* x = static_cast<const char*>(a);
+  const char* y = static_cast<const char*>(b);
+  return x < y + bytes && y < x + bytes;
+}
+
+// Host checks that need no HIP call.
+int check_host_args(const lsb_elem* A, const lsb_elem* B, size_t n) {
+  if (n > LSB_MAX_N) return LSB_ERANGE;
+  if (n <= 1) return LSB_OK;
+  if (!A || !B) return LSB_EINVAL;
+  if (((uintptr_t)A | (uintptr_t)B) & 15u) return LSB_EINVAL;
+  if (overlaps(A, B, n * sizeof(lsb_elem))) return LSB_EINVAL;
+  return LSB_OK;
+}
+
+// The pointer must be device memory holding `bytes` from ptr on.
+int check_device_range(const void* ptr, size_t bytes) {
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, ptr) != hipSuccess) {
+    (void)hipGetLastError();
+    return LSB_EINVAL;
+  }
+  if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged) return LSB_EINVAL;
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)ptr) == hipSuccess && base) {
+    const char* b = static_cast<const char*>(base);
+    const char* p = static_cast<const char*>(ptr);
+    if (p < b || (size_t)(p - b) + bytes > size) return LSB_EINVAL;
+  } else {
+    (void)hipGetLastError();
+  }
+  return LSB_OK;
+}
+
+// Library-managed workspaces, one per (device, stream), grown on demand.
+struct Workspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mu;
+std::map<std::pair<int, void*>, Workspace> g_ws;
+
+int get_workspace(void* stream, size_t need, void** out) {
+  int dev = 0;
+  LSB_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  Workspace& w = g_ws[std::make_pair(dev, stream)];
+  if (w.bytes < need) {
+    if (w.ptr) {
+      // Earlier sorts on this stream may still use the old buffer.
+      LSB_HIP(hipStreamSynchronize((hipStream_t)stream));
+      LSB_HIP(hipFree(w.ptr));
+      w.ptr = nullptr;
+      w.bytes = 0;
+    }
+    size_t bytes = need + need / 4;
+    if (hipMalloc(&w.ptr, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      w.ptr = nullptr;
+      return LSB_ENOMEM;
+    }
+    w.bytes = bytes;
+  }
+  *out = w.ptr;
+  return LSB_OK;
+}
+
+// Events of the last LSB_PROFILE call on this thread: [0] before the
+// histogram, [1] after it, [2] after the scan, [3 + q] after pass q, then
+// one after the copy-back.
+struct ProfileState {
+  std::vector<hipEvent_t> ev;
+  uint32_t passes = 0, d = 0, tile = 0;
+  const uint32_t* plan = nullptr;
+  hipStream_t stream = nullptr;
+  bool valid = false;
+};
+thread_local ProfileState g_prof;
+
+int prof_mark(bool on, size_t i, hipStream_t s) {
+  if (!on) return LSB_OK;
+  while (g_prof.ev.size() <= i) {
+    hipEvent_t e;
+    LSB_HIP(hipEventCreate(&e));
+    g_prof.ev.push_back(e);
+  }
+  LSB_HIP(hipEventRecord(g_prof.ev[i], s));
+  return LSB_OK;
+}
+
+int enqueue_sort(lsb_elem* A, lsb_elem* B, uint32_t n, const Plan& p, char* ws,
+                 uint32_t flags, hipStream_t s) {
+  const bool prof = (flags & LSB_PROFILE) != 0;
+  int rc;
+  uint32_t* counts = reinterpret_cast<uint32_t*>(ws + p.counts);
+  uint32_t* starts = reinterpret_cast<uint32_t*>(ws + p.starts);
+  uint32_t* plan = reinterpret_cast<uint32_t*>(ws + p.plan);
+  uint32_t* tickets = reinterpret_cast<uint32_t*>(ws + p.tickets);
+  uint32_t* status = reinterpret_cast<uint32_t*>(ws + p.status);
+  const size_t slice = (size_t)p.ntiles * p.nb;
+
+  if (prof) {
+    g_prof.valid = false;
+    if (p.passes > 16) return LSB_EINVAL;
+  }
+  if ((rc = prof_mark(prof, 0, s))) return rc;
+  lsb::launch_zero(counts, (size_t)p.passes * p.nb, s);
+  LSB_HIP(lsb::launch_histogram(p.d, A, n, p.key_bits, p.passes, counts, status, slice, s));
+  if ((rc = prof_mark(prof, 1, s))) return rc;
+  LSB_HIP(lsb::launch_scan_plan(p.d, counts, starts, plan, tickets, n, p.passes, flags, s));
+  if ((rc = prof_mark(prof, 2, s))) return rc;
+  for (uint32_t q = 0; q < p.passes; ++q) {
+    lsb::PassArgs a;
+    a.A = A;
+    a.B = B;
+    a.n = n;
+    a.pass = q;
+    a.shift = q * p.d;
+    const uint32_t w = p.key_bits - a.shift < p.d ? p.key_bits - a.shift : p.d;
+    a.mask = (w >= 32) ? 0xFFFFFFFFu : ((1u << w) - 1u);
+    a.plan = plan;
+    a.starts = starts + (size_t)q * p.nb;
+    a.tickets = tickets;
+    a.status = status;
+    a.ntiles = p.ntiles;
+    p.v->scatter(a, s);
+    if ((rc = prof_mark(prof, 3 + q, s))) return rc;
+  }
+  lsb::launch_copy_back(A, B, n, plan + 4 * p.passes, s);
+  if ((rc = prof_mark(prof, 3 + p.passes, s))) return rc;
+  LSB_HIP(hipGetLastError());
+  if (prof) {
+    g_prof.passes = p.passes;
+    g_prof.d = p.d;
+    g_prof.tile = p.tile;
+    g_prof.plan = plan;
+    g_prof.stream = s;
+    g_prof.valid = true;
+  }
+  return LSB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void lsb_config_init(lsb_config* cfg) {
+  if (cfg) *cfg = defaults();
+}
+
+const char* lsb_version(void) { return "lsb-mi355x 0.1.0 (gfx950)"; }
+
+const char* lsb_strerror(int code) {
+  switch (code) {
+    case LSB_OK: return "success";
+    case LSB_EINVAL: return "invalid argument";
+    case LSB_ENOMEM: return "out of device memory or workspace too small";
+    case LSB_EHIP: return "HIP runtime error";
+    case LSB_ERANGE: return "n exceeds LSB_MAX_N";
+    case LSB_EBUSY: return "queue or memory budget full";
+    case LSB_ENOTFOUND: return "unknown job";
+    default: return "unknown error";
+  }
+}
+
+int lsb_last_hip_error(void) { return g_last_hip; }
+
+int lsb_variants(uint32_t* digit_bits, uint32_t* tiles, int cap) {
+  int nv = 0;
+  const lsb::SortVariant* vs = lsb::sort_variants(&nv);
+  for (int i = 0; i < nv && i < cap; ++i) {
+    if (digit_bits) digit_bits[i] = vs[i].digit_bits;
+    if (tiles) tiles[i] = vs[i].tile;
+  }
+  return nv;
+}
+
+int lsb_plan(size_t n, const lsb_config* cfg, uint32_t* digit_bits, uint32_t* tile,
+             uint32_t* passes) {
+  const lsb_config c = cfg ? *cfg : defaults();
+  if (n > LSB_MAX_N) return LSB_ERANGE;
+  Plan p;
+  const int rc = make_plan(n, c, &p);
+  if (rc) return rc;
+  if (digit_bits) *digit_bits = p.d;
+  if (tile) *tile = p.tile;
+  if (passes) *passes = p.passes;
+  return LSB_OK;
+}
+
+size_t lsb_workspace_size(size_t n, const lsb_config* cfg) {
+  const lsb_config c = cfg ? *cfg : defaults();
+  if (n <= 1 || n > LSB_MAX_N) return 0;
+  Plan p;
+  if (make_plan(n, c, &p)) return 0;
+  return p.total;
+}
+
+int lsb_sort(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg) {
+  const lsb_config c = cfg ? *cfg : defaults();
+  int rc = check_host_args(A, B, n);
+  if (rc) return rc;
+  Plan p;
+  rc = make_plan(n > 1 ? n : 2, c, &p);
+  if (rc) return rc;
+  if (n <= 1) return LSB_OK;
+  const size_t bytes = n * sizeof(lsb_elem);
+  if ((rc = check_device_range(A, bytes)) || (rc = check_device_range(B, bytes))) return rc;
+  void* ws = c.workspace;
+  if (ws) {
+    if (c.workspace_bytes < p.total || ((uintptr_t)ws & 255u)) return LSB_ENOMEM;
+  } else if ((rc = get_workspace(c.stream, p.total, &ws))) {
+    return rc;
+  }
+  hipStream_t s = (hipStream_t)c.stream;
+  rc = enqueue_sort(A, B, (uint32_t)n, p, static_cast<char*>(ws), c.flags, s);
+  if (rc) return rc;
+  if (c.flags & LSB_SYNC) LSB_HIP(hipStreamSynchronize(s));
+  return LSB_OK;
+}
+
+int lsb_profile_read(lsb_profile* out) {
+  if (!out) return LSB_EINVAL;
+  memset(out, 0, sizeof *out);
+  if (!g_prof.valid) return LSB_EINVAL;
+  const uint32_t P = g_prof.passes;
+  std::vector<hipEvent_t>& ev = g_prof.ev;
+  LSB_HIP(hipEventSynchronize(ev[3 + P]));
+  uint32_t plan[4 * 16 + 4];
+  LSB_HIP(hipMemcpy(plan, g_prof.plan, (4 * P + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  out->passes = P;
+  out->digit_bits = g_prof.d;
+  out->tile = g_prof.tile;
+  for (uint32_t q = 0; q < P; ++q)
+    if (plan[4 * q]) out->executed_mask |= 1u << q;
+  LSB_HIP(hipEventElapsedTime(&out->histogram_ms, ev[0], ev[1]));
+  LSB_HIP(hipEventElapsedTime(&out->scan_ms, ev[1], ev[2]));
+  for (uint32_t q = 0; q < P; ++q)
+    LSB_HIP(hipEventElapsedTime(&out->pass_ms[q], ev[2 + q], ev[3 + q]));
+  LSB_HIP(hipEventElapsedTime(&out->copy_ms, ev[2 + P], ev[3 + P]));
+  LSB_HIP(hipEventElapsedTime(&out->total_ms, ev[0], ev[3 + P]));
+  return LSB_OK;
+}
+
+int lsb_debug_histogram(const lsb_elem* A, size_t n, const lsb_config* cfg, uint32_t* counts) {
+  const lsb_config c = cfg ? *cfg : defaults();
+  if (n > LSB_MAX_N) return LSB_ERANGE;
+  if (!counts || (n && !A)) return LSB_EINVAL;
+  Plan p;
+  int rc = make_plan(n > 1 ? n : 2, c, &p);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)c.stream;
+  lsb::launch_zero(counts, (size_t)p.passes * p.nb, s);
+  if (n) LSB_HIP(lsb::launch_histogram(p.d, A, (uint32_t)n, p.key_bits, p.passes, counts,
+                                       nullptr, 0, s));
+  LSB_HIP(hipGetLastError());
+  if (c.flags & LSB_SYNC) LSB_HIP(hipStreamSynchronize(s));
+  return LSB_OK;
+}
+
+// ------------------------------------------------------------- workloads
+
+int lsb_generate(lsb_elem* A, size_t n, const lsb_gen_spec* spec, void* stream) {
+  if (!spec) return LSB_EINVAL;
+  if (n == 0) return LSB_OK;
+  if (!A || ((uintptr_t)A & 15u)) return LSB_EINVAL;
+  if (spec->kind > LSB_GEN_ZIPF || spec->key_bits > 64) return LSB_EINVAL;
+  if (spec->kind == LSB_GEN_ZIPF && (spec->zipf_m == 0 || !(spec->zipf_s > 0.0)))
+    return LSB_EINVAL;
+  const uint32_t ranks = spec->ranks ? spec->ranks : 1;
+  const uint64_t per = (n + ranks - 1) / ranks;
+  const uint32_t kb = spec->key_bits ? spec->key_bits : 64;
+  const uint64_t mask = kb >= 64 ? ~0ull : ((1ull << kb) - 1);
+  hipStream_t s = (hipStream_t)stream;
+  double* d_cdf = nullptr;
+  uint64_t* d_table = nullptr;
+  if (spec->kind == LSB_GEN_ZIPF) {
+    const uint32_t m = spec->zipf_m;
+    std::vector<double> cdf(m);
+    std::vector<uint64_t> table(m);
+    double acc = 0.0;
+    for (uint32_t k = 0; k < m; ++k) {
+      acc += pow((double)(k + 1), -spec->zipf_s);
+      cdf[k] = acc;
+    }
+    for (uint32_t k = 0; k < m; ++k) cdf[k] /= acc;
+    cdf[m - 1] = 1.0;
+    lsb::u128 st = lsb::pcg_seed_state(spec->zipf_table_seed);
+    for (uint32_t k = 0; k < m; ++k) {
+      st = st * lsb::pcg_mul() + lsb::pcg_inc();
+      table[k] = lsb::pcg_output(st);
+    }
+    LSB_HIP(hipMalloc(&d_cdf, m * sizeof(double)));
+    LSB_HIP(hipMalloc(&d_table, m * sizeof(uint64_t)));
+    LSB_HIP(hipMemcpyAsync(d_cdf, cdf.data(), m * sizeof(double), hipMemcpyHostToDevice, s));
+    LSB_HIP(hipMemcpyAsync(d_table, table.data(), m * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  }
+  hipError_t e = lsb::launch_generate(A, n, spec->kind, per, spec->seed_base, mask,
+                                      spec->val_base, d_cdf, d_table, spec->zipf_m, s);
+  if (d_cdf) {
+    LSB_HIP(hipStreamSynchronize(s));
+    LSB_HIP(hipFree(d_cdf));
+    LSB_HIP(hipFree(d_table));
+  }
+  LSB_HIP(e);
+  return LSB_OK;
+}
+
+int lsb_verify(const lsb_elem* out, const uint64_t* in_keys, size_t n, uint64_t val_base,
+               uint32_t key_bits, uint64_t* bad_count, int64_t* first_bad, void* stream) {
+  if (!bad_count || !first_bad) return LSB_EINVAL;
+  *bad_count = 0;
+  *first_bad = -1;
+  if (n == 0) return LSB_OK;
+  if (!out || !in_keys || key_bits > 64) return LSB_EINVAL;
+  const uint32_t kb = key_bits ? key_bits : 64;
+  const uint64_t mask = kb >= 64 ? ~0ull : ((1ull << kb) - 1);
+  hipStream_t s = (hipStream_t)stream;
+  const size_t words = (n + 31) / 32;
+  char* buf = nullptr;
+  LSB_HIP(hipMalloc(&buf, align_up(words * 4) + 16));
+  uint32_t* bitmap = reinterpret_cast<uint32_t*>(buf);
+  unsigned long long* res = reinterpret_cast<unsigned long long*>(buf + align_up(words * 4));
+  hipError_t e = lsb::launch_verify(out, in_keys, n, val_base, mask, bitmap, res, s);
+  unsigned long long host[2] = {0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(host, res, sizeof host, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(buf);
+  LSB_HIP(e);
+  *bad_count = host[0];
+  *first_bad = host[0] ? (int64_t)host[1] : -1;
+  return LSB_OK;
+}
+
+int lsb_extract_keys(const lsb_elem* A, size_t n, uint64_t* keys, void* stream) {
+  if (n == 0) return LSB_OK;
+  if (!A || !keys) return LSB_EINVAL;
+  LSB_HIP(lsb::launch_extract_keys(A, n, keys, (hipStream_t)stream));
+  return LSB_OK;
+}
+
+}  // extern "C"

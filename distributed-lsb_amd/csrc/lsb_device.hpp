@@ -1,0 +1,127 @@
+// Device-side helpers shared by the sort, segmented-sort and workload kernels.
+// CDNA4 (gfx950) only: 64-lane wavefronts, LDS, agent-scope atomics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lsb {
+
+// One element, 16 bytes, key at offset 0 (SortElement, mpi/mpi_lsbsort.cpp:30-33).
+struct __attribute__((aligned(16))) elem_t {
+  uint64_t key;
+  uint64_t val;
+};
+
+constexpr int kWave = 64;
+
+// Look-back status word: 0 = tile not counted yet; otherwise bit 31 says
+// the low 31 bits hold the INCLUSIVE prefix (all earlier tiles plus this
+// one) rather than this tile's own AGGREGATE, and the low 31 bits are the
+// count plus one (so a zero count still reads as published).
+constexpr uint32_t kFlagInclusive = 0x80000000u;
+constexpr uint32_t kCountMask = 0x7FFFFFFFu;
+
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// popcount(m & lanes below this one)
+__device__ __forceinline__ uint32_t count_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t ld_status(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, kWave);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  return x;
+}
+
+// In-place exclusive scan of a[0..N) in LDS by a BLOCK-thread workgroup.
+// Every thread of the block must call it; it begins and ends with a barrier
+// so callers may write a[] right before and read it right after.
+template <int N, int BLOCK>
+__device__ __forceinline__ void block_exclusive_scan(uint32_t* a, uint32_t* wave_sums) {
+  constexpr int PER = (N + BLOCK - 1) / BLOCK;
+  constexpr int W = BLOCK / kWave;
+  const int t = threadIdx.x;
+  __syncthreads();
+  uint32_t v[PER];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = t * PER + k;
+    v[k] = (i < N) ? a[i] : 0u;
+    s += v[k];
+  }
+  const uint32_t inc = wave_inclusive_scan(s);
+  if (lane_id() == kWave - 1) wave_sums[t / kWave] = inc;
+  __syncthreads();
+  uint32_t run = inc - s;
+  const int w = t / kWave;
+#pragma unroll
+  for (int k = 0; k < W; ++k)
+    if (k < w) run += wave_sums[k];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = t * PER + k;
+    if (i < N) a[i] = run;
+    run += v[k];
+  }
+  __syncthreads();
+}
+
+// Digit p of a key: bits [shift, shift + width) where the last digit is cut
+// to the key width (Chapel's nBits, chpl/arkouda-radix-sort.chpl:69-78).
+__device__ __forceinline__ uint32_t digit_of(uint64_t key, uint32_t shift, uint32_t mask) {
+  return (uint32_t)(key >> shift) & mask;
+}
+
+// ------------------------------------------------------------- pcg64 (device)
+// PCG XSL-RR 128/64 with the pcg-cpp default multiplier and increment, the
+// generator behind the reference keygen (mpi/mpi_lsbsort.cpp:686-690).
+typedef unsigned __int128 u128;
+
+__device__ __host__ __forceinline__ u128 pcg_mul() {
+  return ((u128)2549297995355413924ULL << 64) | (u128)4865540595714422341ULL;
+}
+__device__ __host__ __forceinline__ u128 pcg_inc() {
+  return ((u128)6364136223846793005ULL << 64) | (u128)1442695040888963407ULL;
+}
+__device__ __host__ __forceinline__ uint64_t pcg_output(u128 s) {
+  const uint64_t x = (uint64_t)(s >> 64) ^ (uint64_t)s;
+  const unsigned r = (unsigned)(s >> 122);
+  return (x >> r) | (x << ((64 - r) & 63));
+}
+__device__ __host__ __forceinline__ u128 pcg_seed_state(uint64_t seed) {
+  return ((u128)seed + pcg_inc()) * pcg_mul() + pcg_inc();
+}
+// Jump an LCG state ahead by delta steps in O(log delta) (F. Brown,
+// "Random Number Generation with Arbitrary Strides", 1994).
+__device__ __host__ inline u128 pcg_advance(u128 s, uint64_t delta) {
+  u128 acc_mul = 1, acc_add = 0, cur_mul = pcg_mul(), cur_add = pcg_inc();
+  while (delta) {
+    if (delta & 1) {
+      acc_mul *= cur_mul;
+      acc_add = acc_add * cur_mul + cur_add;
+    }
+    cur_add = (cur_mul + 1) * cur_add;
+    cur_mul *= cur_mul;
+    delta >>= 1;
+  }
+  return acc_mul * s + acc_add;
+}
+
+}  // namespace lsb

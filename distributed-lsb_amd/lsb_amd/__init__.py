@@ -1,0 +1,232 @@
+"""Python binding of the C ABI in include/lsb.h (liblsb_amd.so).
+
+This is the host-side mirror of the reference's per-rank sort entry
+``mySort(A, B)`` (mpi/mpi_lsbsort.cpp:591-597): ``sort(A, B)`` sorts A stably
+by key using B as scratch.  A and B are torch CUDA (HIP) tensors of dtype
+int64 and shape [n, 2] (or any contiguous 16-byte-per-element buffer): column
+0 is the key, column 1 the value, bit-identical to the reference's
+``struct SortElement {uint64_t key; uint64_t val;}`` (30-33).
+
+PyTorch only provides device memory and streams here.  Every computation
+runs in the HIP kernels of liblsb_amd.so; if the library is missing, loading
+raises instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "liblsb_amd.so")
+
+LSB_OK, LSB_EINVAL, LSB_ENOMEM, LSB_EHIP, LSB_ERANGE, LSB_EBUSY, LSB_ENOTFOUND = (
+    0, -1, -2, -3, -4, -5, -6)
+LSB_SYNC, LSB_NO_SKIP, LSB_PROFILE = 1, 2, 4
+LSB_MAX_N = 0x7FFFFFFE
+GEN_UNIFORM, GEN_SORTED, GEN_REVERSE, GEN_ZIPF = 0, 1, 2, 3
+
+
+class LsbError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = strerror(code) if _lib is not None else str(code)
+        if code == LSB_EHIP and _lib is not None:
+            msg += f" (hipError {_lib.lsb_last_hip_error()})"
+        super().__init__(f"{what}: {msg} [{code}]" if what else f"{msg} [{code}]")
+
+
+class Config(ctypes.Structure):
+    """struct lsb_config."""
+    _fields_ = [("key_bits", ctypes.c_uint32), ("digit_bits", ctypes.c_uint32),
+                ("tile", ctypes.c_uint32), ("reserved0", ctypes.c_uint32),
+                ("stream", ctypes.c_void_p), ("workspace", ctypes.c_void_p),
+                ("workspace_bytes", ctypes.c_size_t), ("flags", ctypes.c_uint32),
+                ("reserved1", ctypes.c_uint32)]
+
+
+class GenSpec(ctypes.Structure):
+    """struct lsb_gen_spec."""
+    _fields_ = [("kind", ctypes.c_uint32), ("ranks", ctypes.c_uint32),
+                ("key_bits", ctypes.c_uint32), ("zipf_m", ctypes.c_uint32),
+                ("zipf_s", ctypes.c_double), ("zipf_table_seed", ctypes.c_uint64),
+                ("seed_base", ctypes.c_uint64), ("val_base", ctypes.c_uint64)]
+
+
+class Profile(ctypes.Structure):
+    """struct lsb_profile."""
+    _fields_ = [("passes", ctypes.c_uint32), ("executed_mask", ctypes.c_uint32),
+                ("digit_bits", ctypes.c_uint32), ("tile", ctypes.c_uint32),
+                ("histogram_ms", ctypes.c_float), ("scan_ms", ctypes.c_float),
+                ("copy_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
+                ("pass_ms", ctypes.c_float * 16)]
+
+
+_lib = None
+
+# Every function include/lsb.h declares, with its ctypes signature.
+_P, _SZ, _U32, _U64, _I = (ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32,
+                           ctypes.c_uint64, ctypes.c_int)
+SIGNATURES = {
+    "lsb_config_init": (None, [ctypes.POINTER(Config)]),
+    "lsb_sort": (_I, [_P, _P, _SZ, ctypes.POINTER(Config)]),
+    "lsb_workspace_size": (_SZ, [_SZ, ctypes.POINTER(Config)]),
+    "lsb_plan": (_I, [_SZ, ctypes.POINTER(Config), ctypes.POINTER(_U32),
+                      ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "lsb_variants": (_I, [_P, _P, _I]),
+    "lsb_strerror": (ctypes.c_char_p, [_I]),
+    "lsb_version": (ctypes.c_char_p, []),
+    "lsb_last_hip_error": (_I, []),
+    "lsb_profile_read": (_I, [ctypes.POINTER(Profile)]),
+    "lsb_debug_histogram": (_I, [_P, _SZ, ctypes.POINTER(Config), _P]),
+    "lsb_generate": (_I, [_P, _SZ, ctypes.POINTER(GenSpec), _P]),
+    "lsb_verify": (_I, [_P, _P, _SZ, _U64, _U32, ctypes.POINTER(_U64),
+                        ctypes.POINTER(ctypes.c_int64), _P]),
+    "lsb_extract_keys": (_I, [_P, _SZ, _P, _P]),
+}
+
+
+def lib():
+    """Loads liblsb_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"liblsb_amd.so not built at {LIB_PATH}: run `make -C distributed-lsb_amd` "
+                "or __graft_entry__.build(); there is no fallback path")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != LSB_OK:
+        raise LsbError(rc, what)
+
+
+def strerror(code):
+    return lib().lsb_strerror(code).decode()
+
+
+def version():
+    return lib().lsb_version().decode()
+
+
+def variants():
+    n = lib().lsb_variants(None, None, 0)
+    d = (ctypes.c_uint32 * n)()
+    t = (ctypes.c_uint32 * n)()
+    lib().lsb_variants(d, t, n)
+    return [(d[i], t[i]) for i in range(n)]
+
+
+def make_config(key_bits=64, digit_bits=0, tile=0, stream=None, flags=0,
+                workspace=None, workspace_bytes=0):
+    c = Config()
+    lib().lsb_config_init(ctypes.byref(c))
+    c.key_bits = key_bits
+    c.digit_bits = digit_bits
+    c.tile = tile
+    c.stream = _stream_ptr(stream)
+    c.flags = flags
+    if workspace is not None:
+        c.workspace = workspace if isinstance(workspace, int) else workspace.data_ptr()
+        c.workspace_bytes = workspace_bytes or (
+            workspace.numel() * workspace.element_size() if hasattr(workspace, "numel") else 0)
+    return c
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        try:
+            import torch
+            if torch.cuda.is_available():
+                return torch.cuda.current_stream().cuda_stream
+        except ImportError:
+            pass
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _nelem(x):
+    if hasattr(x, "numel"):
+        return x.numel() * x.element_size() // 16
+    raise TypeError("pass n explicitly for raw pointers")
+
+
+def plan(n, key_bits=64, digit_bits=0, tile=0):
+    c = make_config(key_bits, digit_bits, tile, stream=0)
+    d, t, p = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+    _check(lib().lsb_plan(n, ctypes.byref(c), ctypes.byref(d), ctypes.byref(t),
+                          ctypes.byref(p)), "lsb_plan")
+    return d.value, t.value, p.value
+
+
+def workspace_size(n, key_bits=64, digit_bits=0, tile=0):
+    c = make_config(key_bits, digit_bits, tile, stream=0)
+    return lib().lsb_workspace_size(n, ctypes.byref(c))
+
+
+def sort(A, B, n=None, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,
+         no_skip=False, workspace=None, workspace_bytes=0, profile=False):
+    """mySort(A, B): stable sort of A[0:n] by key, B is scratch, result in A."""
+    n = _nelem(A) if n is None else n
+    flags = ((LSB_SYNC if sync else 0) | (LSB_NO_SKIP if no_skip else 0)
+             | (LSB_PROFILE if profile else 0))
+    c = make_config(key_bits, digit_bits, tile, stream, flags, workspace, workspace_bytes)
+    _check(lib().lsb_sort(_ptr(A), _ptr(B), n, ctypes.byref(c)), "lsb_sort")
+
+
+def profile_read():
+    """Kernel timings (ms) of the last sort(profile=True) on this thread."""
+    p = Profile()
+    _check(lib().lsb_profile_read(ctypes.byref(p)), "lsb_profile_read")
+    P = p.passes
+    return {"passes": P, "executed": [bool(p.executed_mask >> q & 1) for q in range(P)],
+            "digit_bits": p.digit_bits, "tile": p.tile, "histogram_ms": p.histogram_ms,
+            "scan_ms": p.scan_ms, "copy_ms": p.copy_ms, "total_ms": p.total_ms,
+            "pass_ms": [p.pass_ms[q] for q in range(P)]}
+
+
+def debug_histogram(A, counts, n=None, key_bits=64, digit_bits=0, stream=None, sync=True):
+    n = _nelem(A) if n is None else n
+    c = make_config(key_bits, digit_bits, 0, stream, LSB_SYNC if sync else 0)
+    _check(lib().lsb_debug_histogram(_ptr(A), n, ctypes.byref(c), _ptr(counts)),
+           "lsb_debug_histogram")
+
+
+def generate(A, n=None, kind=GEN_UNIFORM, ranks=1, key_bits=64, seed_base=0, val_base=0,
+             zipf_m=1 << 20, zipf_s=1.1, zipf_table_seed=12345, stream=None):
+    n = _nelem(A) if n is None else n
+    s = GenSpec(kind=kind, ranks=ranks, key_bits=key_bits, zipf_m=zipf_m, zipf_s=zipf_s,
+                zipf_table_seed=zipf_table_seed, seed_base=seed_base, val_base=val_base)
+    _check(lib().lsb_generate(_ptr(A), n, ctypes.byref(s), _stream_ptr(stream)),
+           "lsb_generate")
+
+
+def verify(out, in_keys, n=None, val_base=0, key_bits=64, stream=None):
+    """Returns (bad_count, first_bad) -- (0, -1) proves out is the stable sort."""
+    n = _nelem(out) if n is None else n
+    bad, first = ctypes.c_uint64(), ctypes.c_int64()
+    _check(lib().lsb_verify(_ptr(out), _ptr(in_keys), n, val_base, key_bits,
+                            ctypes.byref(bad), ctypes.byref(first), _stream_ptr(stream)),
+           "lsb_verify")
+    return bad.value, first.value
+
+
+def extract_keys(A, keys, n=None, stream=None):
+    n = _nelem(A) if n is None else n
+    _check(lib().lsb_extract_keys(_ptr(A), n, _ptr(keys), _stream_ptr(stream)),
+           "lsb_extract_keys")

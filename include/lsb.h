@@ -1,0 +1,172 @@
+/*
+ * lsb.h -- C ABI of the MI355X-native stable LSD radix sort of 16-byte
+ * {uint64 key, uint64 val} elements.
+ *
+ * The reference (mppf/distributed-lsb) exposes no library API: its per-rank
+ * sort is the internal function
+ *     void mySort(DistributedArray<SortElement>& A,
+ *                 DistributedArray<SortElement>& B)      mpi/mpi_lsbsort.cpp:591-597
+ *                                                        shmem/shmem_lsbsort.cpp:469-482
+ * called once by main (mpi/mpi_lsbsort.cpp:734) between two barriers.  On
+ * entry A holds the elements and B is same-sized scratch; on exit A holds the
+ * stable sort by key and B is clobbered.  lsb_sort() below has exactly that
+ * contract for one rank's block resident in HBM.
+ *
+ * All pointers to elements are DEVICE pointers (HBM), 16-byte aligned.
+ * Everything is plain C: no C++ types, no exceptions cross this boundary.
+ * Calls are stream-ordered and asynchronous unless LSB_SYNC is set.
+ */
+#ifndef LSB_H
+#define LSB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* struct SortElement {uint64_t key; uint64_t val;}  mpi/mpi_lsbsort.cpp:30-33,
+ * shmem/shmem_lsbsort.cpp:25-28.  16 bytes, key at offset 0, little-endian.
+ * Ordering is by key only (shmem/shmem_lsbsort.cpp:49-51), stable. */
+typedef struct lsb_elem {
+  uint64_t key;
+  uint64_t val;
+} lsb_elem;
+
+/* Return codes.  The reference returns no errors (MPI errors abort,
+ * mpi/mpi_lsbsort.cpp:18-20; invariants are asserts); these replace its
+ * asserts and its one pre-flight guard (n/R <= INT_MAX, 647-655). */
+#define LSB_OK 0
+#define LSB_EINVAL (-1)  /* null / misaligned / overlapping / host pointer, bad key_bits, digit_bits or tile */
+#define LSB_ENOMEM (-2)  /* device allocation failed, or caller workspace too small */
+#define LSB_EHIP (-3)    /* a HIP call failed; see lsb_last_hip_error() */
+#define LSB_ERANGE (-4)  /* n above LSB_MAX_N */
+#define LSB_EBUSY (-5)   /* serving: job queue or memory budget full */
+#define LSB_ENOTFOUND (-6) /* serving: unknown job id */
+
+/* Largest n one lsb_sort call accepts.  Per-bucket counts and the
+ * look-back status words are 32-bit (31-bit count + 1 flag bit). */
+#define LSB_MAX_N ((size_t)0x7FFFFFFEu)
+
+/* lsb_config.flags */
+#define LSB_SYNC 1u     /* return only after the sort finished on the stream */
+#define LSB_NO_SKIP 2u  /* run every digit pass, even single-bucket ones */
+#define LSB_PROFILE 4u  /* record HIP events around each kernel of this call (lsb_profile_read) */
+
+typedef struct lsb_config {
+  uint32_t key_bits;          /* sort by the low key_bits bits of key (1..64); 0 = 64.
+                                 Chapel's nBits, chpl/arkouda-radix-sort.chpl:69 */
+  uint32_t digit_bits;        /* digit width d: 0 = auto, or 8 or 11 (mpi_lsbsort.cpp:22
+                                 uses 16; 2^16 counters per wave do not fit 160 KiB LDS) */
+  uint32_t tile;              /* elements per scatter tile: 0 = auto, else a compiled
+                                 tile for the digit width (see lsb_variants) */
+  uint32_t reserved0;
+  void* stream;               /* hipStream_t to run on; NULL = default stream */
+  void* workspace;            /* device scratch of >= lsb_workspace_size() bytes;
+                                 NULL = library-managed, cached per (device, stream) */
+  size_t workspace_bytes;
+  uint32_t flags;             /* LSB_SYNC | LSB_NO_SKIP */
+  uint32_t reserved1;
+} lsb_config;
+
+/* Fills *cfg with the defaults (64-bit keys, auto digit/tile, default
+ * stream, library workspace, asynchronous). */
+void lsb_config_init(lsb_config* cfg);
+
+/* Stable LSD sort of A[0..n) by key, B[0..n) is scratch; result in A.
+ * Replaces mySort (mpi/mpi_lsbsort.cpp:591-597; shmem/shmem_lsbsort.cpp:
+ * 469-482).  n = 0 and n = 1 succeed without touching memory.  cfg may be
+ * NULL.  Returns LSB_OK or a negative LSB_E* code. */
+int lsb_sort(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg);
+
+/* Bytes of device workspace lsb_sort(n, cfg) needs (0 for n <= 1). */
+size_t lsb_workspace_size(size_t n, const lsb_config* cfg);
+
+/* Digit width and tile that lsb_sort will use for (n, cfg); returns
+ * LSB_OK or LSB_EINVAL. */
+int lsb_plan(size_t n, const lsb_config* cfg, uint32_t* digit_bits, uint32_t* tile,
+             uint32_t* passes);
+
+/* Compiled (digit_bits, tile) pairs: writes up to cap pairs and returns how
+ * many exist. */
+int lsb_variants(uint32_t* digit_bits, uint32_t* tiles, int cap);
+
+const char* lsb_strerror(int code);
+const char* lsb_version(void);
+/* hipError_t of the last failed HIP call on this thread (0 if none). */
+int lsb_last_hip_error(void);
+
+/* Kernel timings of the last lsb_sort call made with LSB_PROFILE on this
+ * thread, from HIP events recorded on its stream between its launches.  No
+ * counterpart in the reference, which times only the whole sort with
+ * steady_clock between barriers (mpi/mpi_lsbsort.cpp:731-745). */
+typedef struct lsb_profile {
+  uint32_t passes;         /* digit passes planned */
+  uint32_t executed_mask;  /* bit p set when pass p ran (not single-bucket) */
+  uint32_t digit_bits;
+  uint32_t tile;
+  float histogram_ms;      /* counter reset + all-digit histogram */
+  float scan_ms;           /* scan + plan */
+  float copy_ms;           /* B -> A copy-back (near 0 when not needed) */
+  float total_ms;          /* first event to last */
+  float pass_ms[16];       /* scatter pass p */
+} lsb_profile;
+
+/* Waits for the profiled call's events and fills *out. */
+int lsb_profile_read(lsb_profile* out);
+
+/* Debug entry for the all-digit histogram kernel alone: writes
+ * counts[p * 2^d + b] (uint32, device memory, P = ceil(key_bits/d) rows) for
+ * the digits of A[0..n).  The reference's count loop is
+ * mpi/mpi_lsbsort.cpp:237-241, run once per digit. */
+int lsb_debug_histogram(const lsb_elem* A, size_t n, const lsb_config* cfg,
+                        uint32_t* counts);
+
+/* ------------------------------------------------------------ workloads */
+
+/* Synthetic inputs, generated on the device.  UNIFORM reproduces the
+ * reference's keygen (mpi/mpi_lsbsort.cpp:676-704): the array is block
+ * distributed over `ranks` virtual ranks of ceil(n/ranks) elements
+ * (divCeil, 146-171); element g takes output number (g mod per) of
+ * pcg64(seed_base + g/per) as its key (masked to key_bits) and
+ * val = val_base + g. */
+#define LSB_GEN_UNIFORM 0u
+#define LSB_GEN_SORTED 1u   /* key = i */
+#define LSB_GEN_REVERSE 2u  /* key = n-1-i */
+#define LSB_GEN_ZIPF 3u     /* key = table[rank], rank ~ Zipf(s) over m keys by inverse CDF of
+                               the element's pcg64 draw (53-bit uniform); table[k] = output k
+                               of pcg64(zipf_table_seed) */
+
+typedef struct lsb_gen_spec {
+  uint32_t kind;
+  uint32_t ranks;           /* virtual ranks (>= 1) */
+  uint32_t key_bits;        /* UNIFORM: keys masked to the low key_bits bits; 0 = 64 */
+  uint32_t zipf_m;          /* ZIPF: number of distinct keys */
+  double zipf_s;            /* ZIPF: exponent */
+  uint64_t zipf_table_seed; /* ZIPF: seed of the key table */
+  uint64_t seed_base;       /* rank r uses pcg64(seed_base + r) */
+  uint64_t val_base;        /* val = val_base + global index */
+} lsb_gen_spec;
+
+int lsb_generate(lsb_elem* A, size_t n, const lsb_gen_spec* spec, void* stream);
+
+/* O(n) on-device proof that out[0..n) is the stable sort by (masked) key of
+ * an input whose vals were val_base + i and whose key at index i is
+ * in_keys[i] (device): vals form a permutation, every key matches its source,
+ * keys are non-decreasing and equal keys keep index order.  Equivalent to the
+ * reference's full stable_sort comparison (mpi/mpi_lsbsort.cpp:755-784).
+ * Synchronous.  *bad_count = failing positions, *first_bad = lowest one
+ * (or -1). */
+int lsb_verify(const lsb_elem* out, const uint64_t* in_keys, size_t n,
+               uint64_t val_base, uint32_t key_bits, uint64_t* bad_count,
+               int64_t* first_bad, void* stream);
+
+/* Copies the keys of A[0..n) into keys[0..n) (device), stream-ordered. */
+int lsb_extract_keys(const lsb_elem* A, size_t n, uint64_t* keys, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSB_H */

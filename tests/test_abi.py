@@ -1,0 +1,90 @@
+"""The C ABI library loads, exports every function include/lsb.h declares,
+and rejects bad arguments before touching the GPU.  CPU only: nothing here
+launches a kernel."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import lsb_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = open(os.path.join(ROOT, "include", "lsb.h")).read()
+
+
+def declared_functions():
+    body = re.sub(r"/\*.*?\*/", "", HEADER, flags=re.S)
+    return sorted(set(re.findall(r"\b(lsb_[a-z_0-9]+)\s*\(", body)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = lsb_amd.lib()
+    names = declared_functions()
+    assert "lsb_sort" in names and len(names) >= 10
+    for name in names:
+        assert hasattr(L, name), name
+        assert name in lsb_amd.SIGNATURES, f"binding for {name} missing"
+
+
+def test_struct_layouts_match_header():
+    assert ctypes.sizeof(lsb_amd.Config) == 48
+    assert lsb_amd.Config.stream.offset == 16
+    assert ctypes.sizeof(lsb_amd.GenSpec) == 48
+
+
+def test_version_and_strerror():
+    assert "gfx950" in lsb_amd.version()
+    assert lsb_amd.strerror(0) == "success"
+    for code in (-1, -2, -3, -4, -5, -6):
+        assert lsb_amd.strerror(code) != "unknown error"
+    assert lsb_amd.strerror(-99) == "unknown error"
+
+
+def _sort_rc(A, B, n, **kw):
+    c = lsb_amd.make_config(stream=0, **kw)
+    return lsb_amd.lib().lsb_sort(A, B, n, ctypes.byref(c))
+
+
+def test_argument_errors_need_no_gpu():
+    a, b = 0x10000, 0x20000000  # never dereferenced: every case fails before HIP
+    assert _sort_rc(None, None, 0) == 0                 # n = 0 always succeeds
+    assert _sort_rc(a, b, 1) == 0                       # n = 1: nothing to do
+    assert _sort_rc(None, b, 5) == lsb_amd.LSB_EINVAL
+    assert _sort_rc(a, None, 5) == lsb_amd.LSB_EINVAL
+    assert _sort_rc(a + 8, b, 5) == lsb_amd.LSB_EINVAL  # misaligned
+    assert _sort_rc(a, a + 16 * 4, 5) == lsb_amd.LSB_EINVAL  # overlapping
+    assert _sort_rc(a, b, lsb_amd.LSB_MAX_N + 1) == lsb_amd.LSB_ERANGE
+    assert _sort_rc(a, b, 5, key_bits=65) == lsb_amd.LSB_EINVAL
+    assert _sort_rc(a, b, 5, digit_bits=16) == lsb_amd.LSB_EINVAL
+    assert _sort_rc(a, b, 5, digit_bits=8, tile=12345) == lsb_amd.LSB_EINVAL
+
+
+def test_plan_and_workspace():
+    assert lsb_amd.plan(1 << 24, digit_bits=8)[::2] == (8, 8)
+    assert lsb_amd.plan(1 << 24, digit_bits=11)[::2] == (11, 6)
+    assert lsb_amd.plan(1 << 24, key_bits=32, digit_bits=8)[2] == 4
+    assert lsb_amd.plan(1 << 24, key_bits=1, digit_bits=8)[2] == 1
+    assert lsb_amd.workspace_size(0) == 0 and lsb_amd.workspace_size(1) == 0
+    w1, w2 = lsb_amd.workspace_size(1 << 20), lsb_amd.workspace_size(1 << 24)
+    assert 0 < w1 < w2
+    # two look-back slices of tiles x 2^d words dominate
+    d, tile, _ = lsb_amd.plan(1 << 24)
+    assert w2 >= 2 * ((1 << 24) // tile) * (1 << d) * 4
+    vs = lsb_amd.variants()
+    assert (8, 4096) in vs and any(d == 11 for d, _ in vs)
+    for d, t in vs:
+        assert lsb_amd.plan(1 << 20, digit_bits=d, tile=t)[:2] == (d, t)
+
+
+def test_generate_and_verify_reject_bad_args():
+    L = lsb_amd.lib()
+    spec = lsb_amd.GenSpec(kind=7)
+    assert L.lsb_generate(0x10000, 10, ctypes.byref(spec), None) == lsb_amd.LSB_EINVAL
+    spec = lsb_amd.GenSpec(kind=lsb_amd.GEN_ZIPF, zipf_m=0, zipf_s=1.1)
+    assert L.lsb_generate(0x10000, 10, ctypes.byref(spec), None) == lsb_amd.LSB_EINVAL
+    bad, first = ctypes.c_uint64(), ctypes.c_int64()
+    assert L.lsb_verify(None, None, 0, 0, 64, ctypes.byref(bad), ctypes.byref(first), None) == 0
+    assert first.value == -1
+    assert L.lsb_verify(None, None, 5, 0, 64, ctypes.byref(bad), ctypes.byref(first),
+                        None) == lsb_amd.LSB_EINVAL

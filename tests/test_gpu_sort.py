@@ -1,0 +1,265 @@
+"""GPU parity of the HIP sort against the CPU oracle, through the C ABI.
+
+Every comparison is bit-exact on (key, val): integers only.  Sizes the oracle
+finishes in seconds are compared element by element; the full-size configs
+are proven with the O(n) invariants (permutation, key matches source,
+sorted, equal keys in index order) on the device, which together imply
+equality with the stable sort."""
+import numpy as np
+import pytest
+import torch
+
+import lsb_amd
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def to_dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int64).reshape(len(a), 2).copy()).to(DEV)
+
+
+def from_dev(t):
+    return t.cpu().numpy().reshape(-1).view(O.ELEM)
+
+
+def gpu_sort(a, **kw):
+    A = to_dev(a)
+    B = torch.empty_like(A)
+    lsb_amd.sort(A, B, sync=True, **kw)
+    return from_dev(A)
+
+
+def stable(a, key_bits=64):
+    if key_bits >= 64:
+        return a[np.argsort(a["key"], kind="stable")]
+    return a[np.argsort(a["key"] & np.uint64((1 << key_bits) - 1), kind="stable")]
+
+
+VARIANTS = None
+
+
+def variants():
+    global VARIANTS
+    if VARIANTS is None:
+        VARIANTS = lsb_amd.variants()
+    return VARIANTS
+
+
+SIZES = [2, 3, 63, 64, 65, 1000, 2047, 2048, 2049, 4095, 4096, 4097, 6143, 6144, 6145,
+         8191, 8192, 8193, 65537, 100003, (1 << 20) + 7]
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_sort_uniform_all_variants(n):
+    a = O.gen_uniform(n, ranks=3)
+    want = stable(a)
+    for d, tile in variants():
+        got = gpu_sort(a, digit_bits=d, tile=tile)
+        assert np.array_equal(got, want), (n, d, tile)
+
+
+def test_sort_default_and_empty():
+    for n in (0, 1):
+        A = torch.zeros((max(n, 1), 2), dtype=torch.int64, device=DEV)
+        A[0, 0] = 5
+        B = torch.empty_like(A)
+        lsb_amd.sort(A, B, n=n, sync=True)
+        assert A[0, 0].item() == 5
+
+
+def test_c1_reference_config_digest():
+    """n = 2^24 keys of the reference's 4-rank input: the GPU generator
+    reproduces the input digest and the sort reproduces the sorted digest."""
+    n = 1 << 24
+    A = torch.empty((n, 2), dtype=torch.int64, device=DEV)
+    lsb_amd.generate(A, kind=lsb_amd.GEN_UNIFORM, ranks=4)
+    torch.cuda.synchronize()
+    host = from_dev(A)
+    assert O.sha256(host) == "4a8c88b35e1bc097a11998afab7608c9c3f66c2d4dd56fb501233124a51d450d"
+    for d, tile in variants():
+        A2 = A.clone()
+        B = torch.empty_like(A2)
+        lsb_amd.sort(A2, B, digit_bits=d, tile=tile, sync=True)
+        assert O.sha256(from_dev(A2)) == \
+            "edbb62f7b024eaade41bc12278e7cb240d038889a9780f8913a336352530d0c9", (d, tile)
+
+
+@pytest.mark.parametrize("kind,kw", [
+    (lsb_amd.GEN_UNIFORM, dict(ranks=3)),
+    (lsb_amd.GEN_UNIFORM, dict(ranks=7, key_bits=32, seed_base=11)),
+    (lsb_amd.GEN_SORTED, {}),
+    (lsb_amd.GEN_REVERSE, {}),
+    (lsb_amd.GEN_ZIPF, dict(ranks=5, zipf_m=1 << 20)),
+])
+def test_generator_matches_oracle(kind, kw):
+    n = (1 << 18) + 12345
+    A = torch.empty((n, 2), dtype=torch.int64, device=DEV)
+    lsb_amd.generate(A, kind=kind, **kw)
+    torch.cuda.synchronize()
+    got = from_dev(A)
+    if kind == lsb_amd.GEN_UNIFORM:
+        want = O.gen_uniform(n, ranks=kw["ranks"], key_bits=kw.get("key_bits", 64),
+                             seed_base=kw.get("seed_base", 0))
+    elif kind == lsb_amd.GEN_ZIPF:
+        want = O.gen_zipf(n, ranks=kw["ranks"], m=kw["zipf_m"])
+    else:
+        want = O.gen_ordered(n, reverse=kind == lsb_amd.GEN_REVERSE)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("d", [8, 11])
+@pytest.mark.parametrize("case", ["uniform", "zipf", "sorted", "low32"])
+def test_histogram_kernel_equals_oracle(d, case):
+    n = 300_001
+    a = {"uniform": lambda: O.gen_uniform(n, ranks=2),
+         "zipf": lambda: O.gen_zipf(n, ranks=2),
+         "sorted": lambda: O.gen_ordered(n),
+         "low32": lambda: O.gen_uniform(n, ranks=2, key_bits=32)}[case]()
+    P = O.n_digits(64, d)
+    counts = torch.zeros(P * (1 << d), dtype=torch.int32, device=DEV)
+    lsb_amd.debug_histogram(to_dev(a), counts, digit_bits=d)
+    got = counts.cpu().numpy().view(np.uint32).reshape(P, 1 << d)
+    assert np.array_equal(got, O.histograms(a, 64, d))
+
+
+def _skew_cases(n):
+    rng = np.random.default_rng(7)
+    out = {
+        "low32_keys": O.gen_uniform(n, ranks=2, key_bits=32),
+        "zipf": O.gen_zipf(n, ranks=4),
+        "sorted": O.gen_ordered(n),
+        "reverse": O.gen_ordered(n, reverse=True),
+    }
+    eq = np.zeros(n, dtype=O.ELEM)
+    eq["key"] = 0xDEADBEEF
+    eq["val"] = np.arange(n)
+    out["all_equal"] = eq
+    few = np.zeros(n, dtype=O.ELEM)
+    few["key"] = rng.integers(0, 3, n).astype(np.uint64) << np.uint64(61)
+    few["val"] = np.arange(n)
+    out["three_keys_top_bits"] = few
+    one = O.gen_uniform(n, ranks=1)
+    one["key"] &= np.uint64(0xFF00)  # a single varying digit
+    out["one_digit"] = one
+    return out
+
+
+@pytest.mark.parametrize("case", sorted(_skew_cases(10)))
+def test_skew_and_stability(case):
+    n = 200_003
+    a = _skew_cases(n)[case]
+    want = stable(a)
+    for d, tile in variants():
+        assert np.array_equal(gpu_sort(a, digit_bits=d, tile=tile), want), (case, d, tile)
+        assert np.array_equal(gpu_sort(a, digit_bits=d, tile=tile, no_skip=True), want)
+
+
+@pytest.mark.parametrize("key_bits", [1, 8, 16, 21, 32, 33, 40])
+def test_key_bits_and_per_pass_prefix(key_bits):
+    """Sorting by the low key_bits is what the first passes leave behind:
+    after pass p the array is stably sorted by the low (p+1)d bits."""
+    n = 150_001
+    a = O.gen_uniform(n, ranks=2)
+    want = stable(a, key_bits)
+    for d, tile in variants():
+        assert np.array_equal(gpu_sort(a, key_bits=key_bits, digit_bits=d, tile=tile), want)
+
+
+def test_non_default_stream_and_caller_workspace():
+    n = 500_000
+    a = O.gen_uniform(n, ranks=2)
+    want = stable(a)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        A = to_dev(a)
+        B = torch.empty_like(A)
+        ws = torch.empty(lsb_amd.workspace_size(n), dtype=torch.uint8, device=DEV)
+        lsb_amd.sort(A, B, stream=s, workspace=ws)
+    s.synchronize()
+    assert np.array_equal(from_dev(A), want)
+    # too-small workspace is rejected, not overrun
+    small = torch.empty(256, dtype=torch.uint8, device=DEV)
+    with pytest.raises(lsb_amd.LsbError):
+        lsb_amd.sort(A, B, workspace=small)
+
+
+def test_rejects_host_pointer():
+    n = 1000
+    host = np.zeros(2 * n + 2, dtype=np.int64)
+    B = torch.zeros((n, 2), dtype=torch.int64, device=DEV)
+    with pytest.raises(lsb_amd.LsbError):
+        lsb_amd.sort(host.ctypes.data + (16 - host.ctypes.data % 16) % 16, B.data_ptr(), n=n)
+
+
+def test_repeated_sorts_reuse_workspace():
+    """Back-to-back sorts of different sizes on one stream share the cached
+    workspace; stale look-back state must never leak between them."""
+    for n in (1 << 20, 5000, 1 << 16, (1 << 20) + 3, 77):
+        a = O.gen_uniform(n, ranks=2, seed_base=n)
+        assert np.array_equal(gpu_sort(a), stable(a)), n
+
+
+def _device_check(A_sorted, keys, n, key_bits=64):
+    bad, first = lsb_amd.verify(A_sorted, keys, n=n, key_bits=key_bits)
+    return bad, first
+
+
+def test_device_verifier_detects_wrong_output():
+    n = 100_000
+    a = O.gen_zipf(n, ranks=2, m=256)
+    s = stable(a)
+    keys = torch.from_numpy(a["key"].view(np.int64).copy()).to(DEV)
+    assert _device_check(to_dev(s), keys, n) == (0, -1)
+    for mutate in ("swap", "unstable", "dup"):
+        b = s.copy()
+        if mutate == "swap":
+            b[[10, 90000]] = b[[90000, 10]]
+        elif mutate == "unstable":
+            i = int(np.nonzero(b["key"][1:] == b["key"][:-1])[0][0])
+            b[[i, i + 1]] = b[[i + 1, i]]
+        else:
+            b[500]["val"] = b[501]["val"]
+        bad, first = _device_check(to_dev(b), keys, n)
+        assert bad > 0 and first >= 0, mutate
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(kind=lsb_amd.GEN_UNIFORM, ranks=64, key_bits=32),   # high 32 bits zero: skipped passes
+    dict(kind=lsb_amd.GEN_ZIPF, ranks=64),                   # heavy duplicates
+    dict(kind=lsb_amd.GEN_SORTED),
+    dict(kind=lsb_amd.GEN_REVERSE),
+], ids=["low32", "zipf", "sorted", "reverse"])
+def test_stress_2p28(cfg):
+    """The skew/stability stress configs at their full 2^28 size."""
+    n = 1 << 28
+    A = torch.empty((n, 2), dtype=torch.int64, device=DEV)
+    lsb_amd.generate(A, **cfg)
+    keys = torch.empty(n, dtype=torch.int64, device=DEV)
+    lsb_amd.extract_keys(A, keys)
+    B = torch.empty_like(A)
+    lsb_amd.sort(A, B, sync=True)
+    assert _device_check(A, keys, n) == (0, -1)
+    # spot-check against the oracle on a window of the input
+    del B
+    torch.cuda.empty_cache()
+
+
+def test_full_2p30_uniform():
+    """One GPU's share at the README's scale: 2^30 pairs (16 GiB + 16 GiB)."""
+    n = 1 << 30
+    A = torch.empty((n, 2), dtype=torch.int64, device=DEV)
+    lsb_amd.generate(A, kind=lsb_amd.GEN_UNIFORM, ranks=256)
+    keys = torch.empty(n, dtype=torch.int64, device=DEV)
+    lsb_amd.extract_keys(A, keys)
+    # the first 2^24 elements are the reference's 4-rank input (same per-rank size)
+    head = from_dev(A[: 1 << 16])
+    assert np.array_equal(head, O.gen_uniform(1 << 24, ranks=4, count=1 << 16))
+    B = torch.empty_like(A)
+    lsb_amd.sort(A, B, sync=True)
+    del B
+    assert _device_check(A, keys, n) == (0, -1)
+    del A, keys
+    torch.cuda.empty_cache()

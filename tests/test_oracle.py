@@ -1,0 +1,194 @@
+"""Pins the CPU oracle (oracle/lsb_oracle.c) before anything is compared
+with it: known-answer values of pcg64, the reference's golden digests and
+printed rows (tests/golden/ref_runs.json, written from the reference's own
+program), numpy's stable argsort, numpy.bincount, hashlib, and planted
+mistakes that the O(n) checker must catch.  CPU only."""
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = json.load(open(os.path.join(HERE, "golden", "ref_runs.json")))
+
+# pcg64(s) first three outputs, from the pcg-cpp header (mpi_lsbsort.cpp:686)
+PCG_KAT = {
+    0: [0x01070196e695f8f1, 0x703ec840c59f4493, 0xe54954914b3a44fa],
+    1: [0xe175e32ed3507bfa, 0xc0bf922a0b283109, 0x140bfa21e68785bb],
+    2: [0xd41ba4d955e4d97f, 0xde364f9a213d60fe, 0xfa613022f290fbd4],
+    3: [0x3f72f405ec3e13c8, 0x44ca66ef335f7835, 0x982486d15a5da308],
+}
+C1_INPUT_SHA = "4a8c88b35e1bc097a11998afab7608c9c3f66c2d4dd56fb501233124a51d450d"
+C1_SORTED_SHA = "edbb62f7b024eaade41bc12278e7cb240d038889a9780f8913a336352530d0c9"
+
+
+@pytest.mark.parametrize("seed", sorted(PCG_KAT))
+def test_pcg64_known_answers(seed):
+    assert [int(x) for x in O.pcg64(seed, 3)] == PCG_KAT[seed]
+
+
+def test_pcg64_jump_ahead_matches_stepping():
+    full = O.pcg64(7, 5000)
+    for skip in (0, 1, 63, 64, 1000, 4999):
+        assert int(O.pcg64(7, 1, skip=skip)[0]) == int(full[skip])
+    # output 2^22 - 1 of pcg64(0): last key of rank 0 at n = 2^24, R = 4
+    assert int(O.pcg64(0, 1, skip=(1 << 22) - 1)[0]) == 0xa8c2fe6d573c40a5
+
+
+def test_pcg64_matches_numpy_pcg64():
+    """numpy's PCG64 is the same XSL-RR 128/64 generator; with the state set
+    to the pcg-cpp seeding, bump(seed + inc), the streams agree."""
+    mul = (2549297995355413924 << 64) | 4865540595714422341
+    inc = (6364136223846793005 << 64) | 1442695040888963407
+    for seed in (0, 5, 2**40 + 3):
+        bg = np.random.PCG64(0)
+        bg.state = {"bit_generator": "PCG64",
+                    "state": {"state": ((seed + inc) * mul + inc) % (1 << 128), "inc": inc},
+                    "has_uint32": 0, "uinteger": 0}
+        assert np.array_equal(bg.random_raw(1000).astype(np.uint64), O.pcg64(seed, 1000))
+
+
+def test_sha256_matches_hashlib():
+    rng = np.random.default_rng(1)
+    for n in (0, 1, 55, 56, 63, 64, 65, 119, 120, 1000, 4097):
+        b = rng.integers(0, 256, n, dtype=np.uint8)
+        assert O.sha256(b) == hashlib.sha256(b.tobytes()).hexdigest()
+
+
+def test_c1_digests_match_reference():
+    """n = 2^24, 4 ranks: the exact reference input and its sorted output."""
+    a = O.gen_uniform(1 << 24, ranks=4)
+    assert O.sha256(a) == C1_INPUT_SHA
+    s = O.lsd_sort(a, 64, 8)
+    assert O.sha256(s) == C1_SORTED_SHA
+    assert O.check(s, a["key"]) is None
+
+
+@pytest.mark.parametrize("run", GOLDEN["runs"], ids=lambda r: f"n{r['n']}_R{r['ranks']}")
+def test_reference_printed_rows(run):
+    """The rows the reference program printed before and after its sort
+    (first 10 per rank) equal the oracle's input and sorted arrays."""
+    assert run["reference_verify_ran_and_passed"]
+    n, R = run["n"], run["ranks"]
+    a = O.gen_uniform(n, ranks=R)
+    s = O.lsd_sort(a, 64, 16)
+    for idx, key, val in run["input_rows"]:
+        assert (int(a[idx]["key"]), int(a[idx]["val"])) == (int(key, 16), val)
+    for idx, key, val in run["sorted_rows"]:
+        assert (int(s[idx]["key"]), int(s[idx]["val"])) == (int(key, 16), val)
+
+
+def _cases():
+    rng = np.random.default_rng(3)
+    n = 5000
+    out = {
+        "uniform": O.gen_uniform(n, ranks=3),
+        "low32": O.gen_uniform(n, ranks=2, key_bits=32),
+        "zipf": O.gen_zipf(n, ranks=2, m=1 << 12),
+        "sorted": O.gen_ordered(n),
+        "reverse": O.gen_ordered(n, reverse=True),
+    }
+    eq = np.zeros(n, dtype=O.ELEM)
+    eq["key"] = 42
+    eq["val"] = np.arange(n)
+    out["all_equal"] = eq
+    few = np.zeros(n, dtype=O.ELEM)
+    few["key"] = rng.integers(0, 4, n).astype(np.uint64) << np.uint64(60)
+    few["val"] = np.arange(n)
+    out["four_keys_top_bits"] = few
+    return out
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("d", [8, 11, 16])
+def test_lsd_sort_equals_numpy_stable_argsort(name, d):
+    a = CASES[name]
+    want = a[np.argsort(a["key"], kind="stable")]
+    assert np.array_equal(O.lsd_sort(a, 64, d), want)
+    assert np.array_equal(O.stable_sort(a), want)
+
+
+@pytest.mark.parametrize("key_bits", [1, 7, 8, 20, 32, 33, 63])
+def test_key_bits_sorts_by_low_bits_only(key_bits):
+    a = CASES["uniform"]
+    m = np.uint64((1 << key_bits) - 1)
+    want = a[np.argsort(a["key"] & m, kind="stable")]
+    for d in (8, 11):
+        assert np.array_equal(O.lsd_sort(a, key_bits, d), want)
+    assert np.array_equal(O.stable_sort(a, key_bits), want)
+
+
+@pytest.mark.parametrize("d", [8, 11])
+def test_histograms_equal_bincount(d):
+    for name in ("uniform", "zipf", "sorted", "low32"):
+        a = CASES[name]
+        h = O.histograms(a, 64, d)
+        for p in range(h.shape[0]):
+            dig = (a["key"] >> np.uint64(p * d)) & np.uint64((1 << d) - 1)
+            assert np.array_equal(h[p], np.bincount(dig.astype(np.int64), minlength=1 << d))
+            assert h[p].sum() == len(a)
+
+
+@pytest.mark.parametrize("p", [0, 3, 7])
+def test_counting_pass_equals_argsort_on_digit(p):
+    a = CASES["uniform"]
+    dig = (a["key"] >> np.uint64(8 * p)) & np.uint64(255)
+    assert np.array_equal(O.counting_pass(a, p, 8), a[np.argsort(dig, kind="stable")])
+
+
+def test_zipf_generator_shape():
+    a = O.gen_zipf(1 << 18, ranks=4)
+    _, counts = np.unique(a["key"], return_counts=True)
+    top = np.sort(counts)[::-1]
+    # Zipf(1.1) over 2^20 keys: the top key holds about 12.4% of the mass,
+    # the top ten about 33.2% (closed form of the normalised weights).
+    w = np.arange(1, (1 << 20) + 1, dtype=np.float64) ** -1.1
+    w /= w.sum()
+    assert abs(top[0] / len(a) - w[0]) < 0.01
+    assert abs(top[:10].sum() / len(a) - w[:10].sum()) < 0.01
+
+
+def test_checker_accepts_correct_and_catches_planted_mistakes():
+    a = O.gen_zipf(4000, ranks=2, m=64)
+    s = O.stable_sort(a)
+    keys = a["key"]
+    assert O.check(s, keys) is None
+    bad = s.copy(); bad[[10, 3000]] = bad[[3000, 10]]           # order
+    assert O.check(bad, keys)[1] == "keys out of order"
+    bad = s.copy(); bad[5]["val"] = bad[6]["val"]                # duplicate
+    assert O.check(bad, keys) is not None
+    bad = s.copy(); bad[7]["key"] ^= np.uint64(1)                # payload/key mismatch
+    assert O.check(bad, keys) is not None
+    i = int(np.nonzero(s["key"][1:] == s["key"][:-1])[0][0])
+    bad = s.copy(); bad[[i, i + 1]] = bad[[i + 1, i]]            # unstable swap
+    assert O.check(bad, keys)[1] == "equal keys out of index order (unstable)"
+    bad = s.copy(); bad[0]["val"] = 10**9                        # out of range
+    assert O.check(bad, keys)[1] == "val out of range"
+
+
+def test_reference_binary_agrees_with_oracle():
+    """Runs the reference program itself (built from its sources by
+    oracle/Makefile) at a size where it prints every element."""
+    mpirun = shutil.which("mpirun") or "/opt/conda/bin/mpirun"
+    if not os.path.exists(O.REF_BIN) or not os.path.exists(mpirun):
+        pytest.skip("reference binary oracle/_ref/mpi_lsbsort or mpirun not present")
+    n, R = 37, 4
+    p = subprocess.run([mpirun, "-n", str(R), O.REF_BIN, "--n", str(n), "--print", "--verify"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    rows = [l for l in p.stdout.splitlines() if l.startswith("A[")]
+    assert len(rows) == 2 * n
+    s = O.lsd_sort(O.gen_uniform(n, ranks=R), 64, 8)
+    for line in rows[n:]:
+        idx = int(line[2:line.index("]")])
+        key, val = line.split("(")[1].rstrip(")").split(",")
+        assert (int(s[idx]["key"]), int(s[idx]["val"])) == (int(key, 16), int(val))
