@@ -48,9 +48,14 @@ def cpu_baseline(sample_n, max_cores, reps):
     if os.path.exists(ref) and os.path.exists(mpirun):
         rates = []
         try:
-            for _ in range(reps):
+            probe = subprocess.run([mpirun, "-n", "2", ref, "--n", "1000", "--no-verify"],
+                                   capture_output=True, text=True, timeout=60)
+            if probe.returncode != 0:
+                raise RuntimeError("mpirun probe failed: " + probe.stderr[-300:])
+            for r in range(reps):
+                log(f"  reference mpi_lsbsort rep {r + 1}/{reps}: n={sample_n}, {cores} ranks")
                 p = subprocess.run([mpirun, "-n", str(cores), ref, "--n", str(sample_n),
-                                    "--no-verify"], capture_output=True, text=True, timeout=300)
+                                    "--no-verify"], capture_output=True, text=True, timeout=150)
                 m = re.search(r"That's ([0-9.eE+]+) M elements sorted / s", p.stdout)
                 if p.returncode != 0 or not m:
                     raise RuntimeError(p.stdout[-500:] + p.stderr[-500:])

@@ -10,6 +10,7 @@
 //   k_copy_back result B -> A if an odd number of passes ran
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string.h>
@@ -240,6 +241,158 @@ int enqueue_sort(lsb_elem* A, lsb_elem* B, uint32_t n, const Plan& p, char* ws,
   return LSB_OK;
 }
 
+
+// ------------------------------------------------------------ hipGraphs
+// Captured launch sequences, keyed by everything the kernels' arguments
+// depend on.  Capture runs on a library stream (capture only records), the
+// instantiated graph is launched on the caller's stream.
+std::mutex g_graph_mu;
+std::map<std::vector<uint64_t>, hipGraphExec_t> g_graphs;
+std::map<int, hipStream_t> g_capture_streams;
+constexpr size_t kMaxGraphs = 256;
+
+template <class Enqueue>
+int launch_graph(const std::vector<uint64_t>& key, hipStream_t s, Enqueue enqueue) {
+  std::lock_guard<std::mutex> lock(g_graph_mu);
+  auto it = g_graphs.find(key);
+  hipGraphExec_t exec = nullptr;
+  if (it != g_graphs.end()) {
+    exec = it->second;
+  } else {
+    int dev = 0;
+    LSB_HIP(hipGetDevice(&dev));
+    hipStream_t cs = g_capture_streams[dev];
+    if (!cs) {
+      LSB_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+      g_capture_streams[dev] = cs;
+    }
+    if (g_graphs.size() >= kMaxGraphs) {
+      LSB_HIP(hipDeviceSynchronize());  // no evicted graph may still run
+      for (auto& kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
+      g_graphs.clear();
+    }
+    LSB_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(cs);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(cs, &g);
+    if (rc) {
+      if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    LSB_HIP(e);
+    const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    LSB_HIP(ie);
+    g_graphs[key] = exec;
+  }
+  LSB_HIP(hipGraphLaunch(exec, s));
+  return LSB_OK;
+}
+
+// ------------------------------------------------------- segmented sort
+struct SegLayout {
+  const lsb::SegVariant* v;
+  uint32_t key_bits, d, tile, passes, maxp, nb;
+  size_t cap_n;
+  uint32_t cap_segs, max_tiles;
+  size_t hdr, meta, counts, plan, tickets, status, total;
+};
+
+int make_seg_layout(size_t cap_n, uint32_t cap_segs, const lsb_config& c, SegLayout* L) {
+  L->key_bits = c.key_bits ? c.key_bits : 64;
+  if (L->key_bits > 64) return LSB_EINVAL;
+  const uint32_t d = c.digit_bits ? c.digit_bits : 8;
+  int nv = 0;
+  const lsb::SegVariant* vs = lsb::seg_variants(&nv);
+  L->v = nullptr;
+  for (int i = 0; i < nv; ++i)
+    if (vs[i].digit_bits == d && (c.tile == 0 || vs[i].tile == c.tile)) {
+      L->v = &vs[i];
+      break;
+    }
+  if (!L->v) return LSB_EINVAL;
+  L->d = d;
+  L->tile = L->v->tile;
+  L->passes = (L->key_bits + d - 1) / d;
+  L->maxp = (64 + d - 1) / d;
+  L->nb = 1u << d;
+  L->cap_n = cap_n;
+  L->cap_segs = cap_segs ? cap_segs : 1;
+  const uint64_t tiles = cap_n / L->tile + L->cap_segs;
+  if (tiles > 0x7FFFFFFFull) return LSB_ERANGE;
+  L->max_tiles = (uint32_t)tiles;
+  size_t off = 0;
+  L->hdr = off;     off += align_up(sizeof(lsb::SegHeader));
+  L->meta = off;    off += align_up((size_t)L->cap_segs * sizeof(lsb::SegLarge));
+  L->counts = off;  off += align_up((size_t)L->cap_segs * L->maxp * L->nb * 4);
+  L->plan = off;    off += align_up((size_t)L->cap_segs * 8);
+  L->tickets = off; off += align_up(16 * 4);
+  L->status = off;  off += align_up(2 * (size_t)L->max_tiles * L->nb * 4);
+  L->total = off;
+  return LSB_OK;
+}
+
+// Library-managed segmented workspaces, one per (device, stream), with the
+// capacity their layout was made for, and a pinned staging buffer for the
+// per-call batch shape.
+struct SegWorkspace {
+  void* ptr = nullptr;
+  size_t bytes = 0, cap_n = 0;
+  uint32_t cap_segs = 0;
+  void* stage = nullptr;
+  size_t stage_bytes = 0;
+  hipEvent_t staged = nullptr;
+};
+std::map<std::pair<int, void*>, SegWorkspace> g_seg_ws;
+
+uint32_t device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+      (void)hipGetLastError();
+      cus = 256;
+    }
+  }
+  return (uint32_t)cus;
+}
+
+int enqueue_segmented(lsb_elem* A, lsb_elem* B, const SegLayout& L, char* ws, uint32_t flags,
+                      hipStream_t s) {
+  const lsb::SegHeader* hdr = reinterpret_cast<const lsb::SegHeader*>(ws + L.hdr);
+  const lsb::SegLarge* large = reinterpret_cast<const lsb::SegLarge*>(ws + L.meta);
+  const lsb::SegSmall* small = reinterpret_cast<const lsb::SegSmall*>(ws + L.meta);
+  uint32_t* counts = reinterpret_cast<uint32_t*>(ws + L.counts);
+  uint32_t* plan = reinterpret_cast<uint32_t*>(ws + L.plan);
+  uint32_t* tickets = reinterpret_cast<uint32_t*>(ws + L.tickets);
+  uint32_t* status = reinterpret_cast<uint32_t*>(ws + L.status);
+  const uint32_t cus = device_cus();
+  LSB_HIP(lsb::launch_seg_prepare(L.d, A, hdr, large, L.tile, L.key_bits, L.passes, counts, plan,
+                                  tickets, status, flags, 4 * cus, s));
+  LSB_HIP(lsb::launch_seg_small(A, hdr, small, L.key_bits, 4 * cus, s));
+  const uint32_t grid = L.v->blocks_per_cu() * cus;
+  for (uint32_t q = 0; q < L.passes; ++q) {
+    lsb::SegPassArgs a;
+    a.A = A;
+    a.B = B;
+    a.hdr = hdr;
+    a.large = large;
+    a.seg_plan = plan;
+    a.counts = counts;
+    a.tickets = tickets;
+    a.status = status;
+    a.max_tiles = L.max_tiles;
+    a.pass = q;
+    a.shift = q * L.d;
+    const uint32_t w = L.key_bits - a.shift < L.d ? L.key_bits - a.shift : L.d;
+    a.mask = (1u << w) - 1u;
+    LSB_HIP(L.v->scatter(a, grid, s));
+  }
+  LSB_HIP(lsb::launch_seg_copy(A, B, hdr, large, plan, L.tile, 4 * cus, s));
+  return LSB_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -306,14 +459,164 @@ int lsb_sort(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg) {
   if (n <= 1) return LSB_OK;
   const size_t bytes = n * sizeof(lsb_elem);
   if ((rc = check_device_range(A, bytes)) || (rc = check_device_range(B, bytes))) return rc;
+  hipStream_t s = (hipStream_t)c.stream;
+  // Arrays that fit one workgroup are sorted in LDS by a single launch
+  // (auto configuration only; an explicit digit width or tile, or
+  // profiling, runs the pass pipeline).
+  if (c.digit_bits == 0 && c.tile == 0 && !(c.flags & LSB_PROFILE) &&
+      n <= lsb::small_sort_capacity()) {
+    LSB_HIP(lsb::launch_small_sort(A, (uint32_t)n, p.key_bits, s));
+    if (c.flags & LSB_SYNC) LSB_HIP(hipStreamSynchronize(s));
+    return LSB_OK;
+  }
   void* ws = c.workspace;
   if (ws) {
     if (c.workspace_bytes < p.total || ((uintptr_t)ws & 255u)) return LSB_ENOMEM;
   } else if ((rc = get_workspace(c.stream, p.total, &ws))) {
     return rc;
   }
+  if (c.flags & LSB_GRAPH) {
+    if (c.flags & LSB_PROFILE) return LSB_EINVAL;
+    int dev = 0;
+    LSB_HIP(hipGetDevice(&dev));
+    const std::vector<uint64_t> key = {0u, (uint64_t)(uintptr_t)A, (uint64_t)(uintptr_t)B, n,
+                                       (uint64_t)(uintptr_t)ws, p.d, p.tile, p.key_bits,
+                                       (uint64_t)(c.flags & ~LSB_SYNC), (uint64_t)dev};
+    char* w = static_cast<char*>(ws);
+    rc = launch_graph(key, s, [&](hipStream_t cs) {
+      return enqueue_sort(A, B, (uint32_t)n, p, w, c.flags, cs);
+    });
+  } else {
+    rc = enqueue_sort(A, B, (uint32_t)n, p, static_cast<char*>(ws), c.flags, s);
+  }
+  if (rc) return rc;
+  if (c.flags & LSB_SYNC) LSB_HIP(hipStreamSynchronize(s));
+  return LSB_OK;
+}
+
+
+size_t lsb_segmented_workspace_size(size_t total_n, uint32_t nseg, const lsb_config* cfg) {
+  const lsb_config c = cfg ? *cfg : defaults();
+  SegLayout L;
+  if (make_seg_layout(total_n, nseg, c, &L)) return 0;
+  return L.total;
+}
+
+int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, uint32_t nseg,
+                       const lsb_config* cfg) {
+  const lsb_config c = cfg ? *cfg : defaults();
+  if (nseg == 0) return LSB_OK;
+  if (!seg_offsets) return LSB_EINVAL;
+  if (c.flags & LSB_PROFILE) return LSB_EINVAL;
+  const uint32_t kb = c.key_bits ? c.key_bits : 64;
+  if (kb > 64) return LSB_EINVAL;
+  const uint64_t total_n = seg_offsets[nseg];
+  uint32_t n_large = 0, n_tiny = 0, n_small = 0;
+  for (uint32_t i = 0; i < nseg; ++i) {
+    if (seg_offsets[i + 1] < seg_offsets[i]) return LSB_EINVAL;
+    const uint64_t n = seg_offsets[i + 1] - seg_offsets[i];
+    if (n > LSB_MAX_N) return LSB_ERANGE;
+    if (n <= 1) continue;
+    if (n <= lsb::seg_tiny_capacity()) ++n_tiny;
+    else if (n <= lsb::seg_small_capacity()) ++n_small;
+    else ++n_large;
+  }
+  if (n_large + n_tiny + n_small == 0) return LSB_OK;
+  if (!A || !B || (((uintptr_t)A | (uintptr_t)B) & 15u)) return LSB_EINVAL;
+  if (overlaps(A, B, total_n * sizeof(lsb_elem))) return LSB_EINVAL;
+  int rc;
+  SegLayout L;
+  if ((rc = make_seg_layout(total_n, nseg, c, &L))) return rc;
+  const size_t bytes = total_n * sizeof(lsb_elem);
+  if ((rc = check_device_range(A, bytes)) || (rc = check_device_range(B, bytes))) return rc;
   hipStream_t s = (hipStream_t)c.stream;
-  rc = enqueue_sort(A, B, (uint32_t)n, p, static_cast<char*>(ws), c.flags, s);
+  int dev = 0;
+  LSB_HIP(hipGetDevice(&dev));
+
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  SegWorkspace& W = g_seg_ws[std::make_pair(dev, c.stream)];
+  char* ws = static_cast<char*>(c.workspace);
+  if (ws) {
+    if (c.workspace_bytes < L.total || ((uintptr_t)ws & 255u)) return LSB_ENOMEM;
+  } else {
+    if (W.cap_n < total_n || W.cap_segs < nseg || !W.ptr) {
+      const size_t cap_n = std::max<size_t>(total_n, W.cap_n + W.cap_n / 4);
+      const uint32_t cap_s = std::max<uint32_t>(nseg, W.cap_segs + W.cap_segs / 4);
+      SegLayout G;
+      if ((rc = make_seg_layout(cap_n, cap_s, c, &G))) return rc;
+      if (W.ptr) {
+        LSB_HIP(hipStreamSynchronize(s));
+        LSB_HIP(hipFree(W.ptr));
+        W.ptr = nullptr;
+      }
+      if (hipMalloc(&W.ptr, G.total) != hipSuccess) {
+        (void)hipGetLastError();
+        W.ptr = nullptr;
+        W.cap_n = 0;
+        W.cap_segs = 0;
+        return LSB_ENOMEM;
+      }
+      W.bytes = G.total;
+      W.cap_n = cap_n;
+      W.cap_segs = cap_s;
+    }
+    // The layout of a library workspace follows its capacity, not the batch,
+    // so a captured graph stays valid for every batch that fits.
+    if ((rc = make_seg_layout(W.cap_n, W.cap_segs, c, &L))) return rc;
+    if (L.total > W.bytes) return LSB_ENOMEM;
+    ws = static_cast<char*>(W.ptr);
+  }
+
+  // Stage the batch shape in pinned memory and upload it on the stream.
+  const size_t meta_bytes = L.meta - L.hdr + (size_t)(n_large + n_tiny + n_small) * 16;
+  if (W.stage_bytes < meta_bytes) {
+    if (W.stage) {
+      if (W.staged) LSB_HIP(hipEventSynchronize(W.staged));
+      LSB_HIP(hipHostFree(W.stage));
+      W.stage = nullptr;
+    }
+    LSB_HIP(hipHostMalloc(&W.stage, meta_bytes * 2, hipHostMallocDefault));
+    W.stage_bytes = meta_bytes * 2;
+  }
+  if (!W.staged) LSB_HIP(hipEventCreateWithFlags(&W.staged, hipEventDisableTiming));
+  LSB_HIP(hipEventSynchronize(W.staged));  // the previous upload has left the buffer
+  char* h = static_cast<char*>(W.stage);
+  lsb::SegHeader* hdr = reinterpret_cast<lsb::SegHeader*>(h);
+  lsb::SegLarge* lg = reinterpret_cast<lsb::SegLarge*>(h + (L.meta - L.hdr));
+  lsb::SegSmall* tiny = reinterpret_cast<lsb::SegSmall*>(lg + n_large);
+  lsb::SegSmall* sml = tiny + n_tiny;
+  uint32_t il = 0, it = 0, is = 0, tiles = 0;
+  for (uint32_t i = 0; i < nseg; ++i) {
+    const uint64_t base = seg_offsets[i], n = seg_offsets[i + 1] - seg_offsets[i];
+    if (n <= 1) continue;
+    if (n <= lsb::seg_tiny_capacity()) {
+      tiny[it++] = lsb::SegSmall{base, (uint32_t)n, 0u};
+    } else if (n <= lsb::seg_small_capacity()) {
+      sml[is++] = lsb::SegSmall{base, (uint32_t)n, 0u};
+    } else {
+      lg[il++] = lsb::SegLarge{base, (uint32_t)n, tiles};
+      tiles += (uint32_t)((n + L.tile - 1) / L.tile);
+    }
+  }
+  hdr->n_large = n_large;
+  hdr->n_tiny = n_tiny;
+  hdr->n_small = n_small;
+  hdr->total_tiles = tiles;
+  if (tiles > L.max_tiles) return LSB_ENOMEM;
+  LSB_HIP(hipMemcpyAsync(ws + L.hdr, h, meta_bytes, hipMemcpyHostToDevice, s));
+  LSB_HIP(hipEventRecord(W.staged, s));
+
+  if (c.flags & LSB_GRAPH) {
+    const std::vector<uint64_t> key = {1u, (uint64_t)(uintptr_t)A, (uint64_t)(uintptr_t)B,
+                                       (uint64_t)(uintptr_t)ws, L.cap_n, L.cap_segs, L.d, L.tile,
+                                       L.key_bits, (uint64_t)(c.flags & ~LSB_SYNC),
+                                       (uint64_t)dev};
+    rc = launch_graph(key, s, [&](hipStream_t cs) {
+      return enqueue_segmented(A, B, L, ws, c.flags, cs);
+    });
+  } else {
+    rc = enqueue_segmented(A, B, L, ws, c.flags, s);
+  }
   if (rc) return rc;
   if (c.flags & LSB_SYNC) LSB_HIP(hipStreamSynchronize(s));
   return LSB_OK;

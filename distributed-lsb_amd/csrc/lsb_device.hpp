@@ -89,6 +89,22 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t key, uint32_t shift, uint3
   return (uint32_t)(key >> shift) & mask;
 }
 
+// Adds one count for digit `d` of every valid lane.  When every valid lane
+// of the wave holds the same digit (sorted keys, zero high digits, heavy
+// duplicates) a single lane adds the whole count, which avoids a 64-way
+// same-address LDS atomic.
+__device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d, bool valid) {
+  const uint64_t act = __ballot(valid);
+  if (act == 0) return;
+  const uint32_t first = __builtin_amdgcn_readfirstlane(d);
+  const uint64_t same = __ballot(valid && d == first);
+  if (same == act) {
+    if (valid && count_below(act) == 0) atomicAdd(&h[first], (uint32_t)__popcll(act));
+  } else if (valid) {
+    atomicAdd(&h[d], 1u);
+  }
+}
+
 // ------------------------------------------------------------- pcg64 (device)
 // PCG XSL-RR 128/64 with the pcg-cpp default multiplier and increment, the
 // generator behind the reference keygen (mpi/mpi_lsbsort.cpp:686-690).

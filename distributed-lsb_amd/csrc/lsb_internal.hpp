@@ -48,9 +48,68 @@ hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_
 hipError_t launch_scan_plan(uint32_t d, const uint32_t* counts, uint32_t* starts,
                             uint32_t* plan, uint32_t* tickets, uint32_t n, uint32_t passes,
                             uint32_t flags, hipStream_t s);
+// Whole sort of one array of n <= small_sort_capacity() elements inside one
+// workgroup (one HBM read and write per element).
+uint32_t small_sort_capacity();
+hipError_t launch_small_sort(void* A, uint32_t n, uint32_t key_bits, hipStream_t s);
 // Copies B to A when plan says the result is in B.
 void launch_copy_back(void* A, const void* B, uint32_t n, const uint32_t* flag,
                       hipStream_t s);
+
+// ------------------------------------------------------ segmented sort
+// Batch shape, uploaded per call; every segmented kernel reads it from the
+// device so one captured graph serves any batch that fits its workspace.
+struct SegHeader {
+  uint32_t n_large;      // segments sorted by the tile pipeline
+  uint32_t n_tiny;       // segments <= seg_tiny_capacity()
+  uint32_t n_small;      // segments in (tiny, seg_small_capacity()]
+  uint32_t total_tiles;  // tiles over all large segments
+};
+struct SegLarge {
+  uint64_t base;        // first element of the segment in A and B
+  uint32_t n;
+  uint32_t tile_begin;  // global index of the segment's first tile
+};
+struct SegSmall {
+  uint64_t base;
+  uint32_t n;
+  uint32_t pad;
+};
+struct SegPassArgs {
+  void* A;
+  void* B;
+  const SegHeader* hdr;
+  const SegLarge* large;
+  const uint32_t* seg_plan;  // [2*s] executed-pass mask, [2*s+1] result-in-B flag
+  const uint32_t* counts;    // per segment [MAXP][2^d] starts (after the scan)
+  uint32_t* tickets;
+  uint32_t* status;          // two slices of max_tiles * 2^d words
+  uint32_t max_tiles;
+  uint32_t pass;
+  uint32_t shift;
+  uint32_t mask;
+};
+typedef hipError_t (*SegScatterLaunch)(const SegPassArgs&, uint32_t grid, hipStream_t);
+struct SegVariant {
+  uint32_t digit_bits;
+  uint32_t tile;
+  uint32_t block;
+  SegScatterLaunch scatter;
+  uint32_t (*blocks_per_cu)();
+};
+const SegVariant* seg_variants(int* count);
+uint32_t seg_tiny_capacity();
+uint32_t seg_small_capacity();
+hipError_t launch_seg_small(void* A, const SegHeader* hdr, const SegSmall* list,
+                            uint32_t key_bits, uint32_t grid, hipStream_t s);
+hipError_t launch_seg_prepare(uint32_t d, const void* A, const SegHeader* hdr,
+                              const SegLarge* large, uint32_t tile, uint32_t key_bits,
+                              uint32_t passes, uint32_t* counts, uint32_t* seg_plan,
+                              uint32_t* tickets, uint32_t* status0, uint32_t flags,
+                              uint32_t grid, hipStream_t s);
+hipError_t launch_seg_copy(void* A, const void* B, const SegHeader* hdr, const SegLarge* large,
+                           const uint32_t* seg_plan, uint32_t tile, uint32_t grid,
+                           hipStream_t s);
 
 // Workload generation and verification (lsb_workload.hip).
 hipError_t launch_generate(void* A, uint64_t n, uint32_t kind, uint64_t per,

@@ -19,6 +19,7 @@
 //                 odd (shmem/shmem_lsbsort.cpp:477-481 keeps it even instead).
 #include "lsb_device.hpp"
 #include "lsb_internal.hpp"
+#include "lsb_onesweep.hpp"
 
 namespace lsb {
 
@@ -36,22 +37,6 @@ void launch_zero(uint32_t* p, size_t words, hipStream_t s) {
 }
 
 // ------------------------------------------------------------ k_histogram
-// Adds one count for digit `d` of every valid lane.  When every valid lane
-// of the wave holds the same digit (sorted keys, zero high digits, heavy
-// duplicates) a single lane adds the whole count, which avoids a 64-way
-// same-address LDS atomic.
-__device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d, bool valid) {
-  const uint64_t act = __ballot(valid);
-  if (act == 0) return;
-  const uint32_t first = __builtin_amdgcn_readfirstlane(d);
-  const uint64_t same = __ballot(valid && d == first);
-  if (same == act) {
-    if (valid && count_below(act) == 0) atomicAdd(&h[first], (uint32_t)__popcll(act));
-  } else if (valid) {
-    atomicAdd(&h[d], 1u);
-  }
-}
-
 template <int D>
 __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A, uint32_t n,
                                                    uint32_t key_bits, uint32_t passes,
@@ -178,156 +163,29 @@ hipError_t launch_scan_plan(uint32_t d, const uint32_t* counts, uint32_t* starts
 }
 
 // -------------------------------------------------------------- k_onesweep
-// Tile layout: wave w of the block owns the contiguous run of 64*ITEMS
-// elements starting at w*64*ITEMS; its item i, lane l is element
-// i*64 + l of that run.  Every global load of a wave is one contiguous 1 KiB.
-// Ranking visits (wave, item, lane) in element order, so the position an
-// element gets inside its bucket follows input order: the pass is stable.
+// One launch per digit pass, one tile per block.  Tile ids come from a
+// ticket, so a tile only ever waits on tiles whose blocks are already
+// running: look-back cannot deadlock whatever order the hardware dispatches
+// blocks in.
 template <int D, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(PassArgs args) {
   constexpr int NB = 1 << D;
-  constexpr int W = BLOCK / kWave;
-  constexpr int T = BLOCK * ITEMS;
-  constexpr int DPT = (NB + BLOCK - 1) / BLOCK;  // digits per thread
-  static_assert(W * NB * 4 <= T * 16, "per-wave counters must fit in the tile buffer");
-
-  // The tile buffer first holds the per-wave digit counters, then the
-  // elements in digit order.
-  __shared__ elem_t s_elems[T];
-  __shared__ uint32_t s_start[NB];  // tile-local exclusive bucket starts
-  __shared__ uint32_t s_gbase[NB];  // global index of tile-local position 0, per bucket
-  __shared__ uint32_t s_wsum[W];
-  __shared__ uint32_t s_tile;
-
+  __shared__ TileSmem<D, BLOCK, ITEMS> sm;
   const uint32_t* plan = args.plan;
   const uint32_t pass = args.pass;
   if (!plan[4 * pass]) return;  // single-bucket digit: identity pass
   const uint32_t ord = plan[4 * pass + 1];
   const bool src_b = plan[4 * pass + 2] != 0;
-  const elem_t* __restrict__ src = static_cast<const elem_t*>(src_b ? args.B : args.A);
-  elem_t* __restrict__ dst = static_cast<elem_t*>(src_b ? args.A : args.B);
+  const elem_t* src = static_cast<const elem_t*>(src_b ? args.B : args.A);
+  elem_t* dst = static_cast<elem_t*>(src_b ? args.A : args.B);
   const size_t slice = (size_t)args.ntiles * NB;
   uint32_t* st_cur = args.status + (size_t)(ord & 1u) * slice;
   uint32_t* st_next = args.status + (size_t)((ord + 1u) & 1u) * slice;
-  const uint32_t n = args.n, shift = args.shift, mask = args.mask;
-
-  const uint32_t t = threadIdx.x;
-  const uint32_t lane = t & (kWave - 1);
-  const uint32_t wave = t / kWave;
-  uint32_t* whist = reinterpret_cast<uint32_t*>(s_elems);
-
-  // Tile ids come from a ticket, so a tile only ever waits on tiles whose
-  // blocks are already running: look-back cannot deadlock whatever order the
-  // hardware dispatches blocks in.
-  if (t == 0) s_tile = atomicAdd(&args.tickets[pass], 1u);
-  for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
+  if (threadIdx.x == 0) sm.scalar[0] = atomicAdd(&args.tickets[pass], 1u);
   __syncthreads();
-  const uint32_t tile = s_tile;
-  for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) st_next[(size_t)tile * NB + b] = 0u;
-
-  const size_t tile_base = (size_t)tile * T;
-  const size_t wbase = tile_base + (size_t)wave * (kWave * ITEMS) + lane;
-  elem_t e[ITEMS];
-  uint32_t dg[ITEMS], rk[ITEMS];
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const size_t idx = wbase + (size_t)i * kWave;
-    if (idx < n) e[i] = src[idx];
-    else { e[i].key = 0; e[i].val = 0; }
-  }
-
-  // Wave-level ranking: peers = lanes holding the same digit, found with D
-  // ballots.  The lowest peer bumps the wave's counter for the digit; every
-  // peer's rank is the old counter plus the peers below it.
-  uint32_t* wh = whist + wave * NB;
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const bool valid = wbase + (size_t)i * kWave < n;
-    const uint32_t d = digit_of(e[i].key, shift, mask);
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < D; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bal = __ballot(bit);
-      peers &= bit ? bal : ~bal;
-    }
-    const uint32_t below = count_below(peers);
-    uint32_t old = 0;
-    if (valid) old = wh[d];
-    if (valid && below == 0) wh[d] = old + (uint32_t)__popcll(peers);
-    dg[i] = d;
-    rk[i] = old + below;
-  }
-  __syncthreads();
-
-  // Per digit: exclusive prefix over waves (in place) and the tile total,
-  // published at once as this tile's aggregate.
-  uint32_t tot[DPT];
-#pragma unroll
-  for (int k = 0; k < DPT; ++k) {
-    const uint32_t b = t + k * BLOCK;
-    tot[k] = 0;
-    if (b < (uint32_t)NB) {
-      uint32_t s = 0;
-#pragma unroll
-      for (int w = 0; w < W; ++w) {
-        const uint32_t c = whist[w * NB + b];
-        whist[w * NB + b] = s;
-        s += c;
-      }
-      tot[k] = s;
-      s_start[b] = s;
-      st_status(st_cur + (size_t)tile * NB + b, tile == 0 ? ((s + 1u) | kFlagInclusive) : (s + 1u));
-    }
-  }
-  block_exclusive_scan<NB, BLOCK>(s_start, s_wsum);
-
-  uint32_t pos[ITEMS];
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) pos[i] = s_start[dg[i]] + wh[dg[i]] + rk[i];
-  __syncthreads();  // counters are dead from here: the buffer takes elements
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i)
-    if (wbase + (size_t)i * kWave < n) s_elems[pos[i]] = e[i];
-
-  // Decoupled look-back, one thread per digit: sum earlier tiles' aggregates
-  // until one publishes an inclusive prefix.
-#pragma unroll
-  for (int k = 0; k < DPT; ++k) {
-    const uint32_t b = t + k * BLOCK;
-    if (b < (uint32_t)NB) {
-      uint32_t prefix = 0;
-      if (tile > 0) {
-        int64_t j = (int64_t)tile - 1;
-        while (true) {
-          const uint32_t w = ld_status(st_cur + (size_t)j * NB + b);
-          if (w == 0u) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-          }
-          prefix += (w & kCountMask) - 1u;
-          if (w & kFlagInclusive) break;
-          --j;
-        }
-        st_status(st_cur + (size_t)tile * NB + b, (prefix + tot[k] + 1u) | kFlagInclusive);
-      }
-      s_gbase[b] = args.starts[b] + prefix - s_start[b];
-    }
-  }
-  __syncthreads();
-
-  // Write out in digit order: consecutive lanes write consecutive slots of
-  // one bucket run.
-  const uint32_t tile_n = (n - tile_base) < (size_t)T ? (uint32_t)(n - tile_base) : (uint32_t)T;
-#pragma unroll
-  for (int k = 0; k < ITEMS; ++k) {
-    const uint32_t j = k * BLOCK + t;
-    if (j < tile_n) {
-      const elem_t x = s_elems[j];
-      const uint32_t d = digit_of(x.key, shift, mask);
-      dst[(size_t)(uint32_t)(s_gbase[d] + j)] = x;
-    }
-  }
+  const uint32_t tile = sm.scalar[0];
+  onesweep_tile<D, BLOCK, ITEMS>(sm, src, dst, args.n, tile, st_cur, st_next, args.starts,
+                                 args.shift, args.mask);
 }
 
 template <int D, int BLOCK, int ITEMS>
@@ -345,6 +203,27 @@ static const SortVariant kVariants[] = {
 const SortVariant* sort_variants(int* count) {
   *count = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
   return kVariants;
+}
+
+// ----------------------------------------------------------- k_small_sort
+// Whole sort of one array of n <= BLOCK*ITEMS elements in one block (LDS).
+template <int D, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_small_sort(elem_t* A, uint32_t n, uint32_t key_bits) {
+  __shared__ TileSmem<D, BLOCK, ITEMS> sm;
+  small_sort<D, BLOCK, ITEMS>(sm, A, n, key_bits);
+}
+
+uint32_t small_sort_capacity() { return 512 * 16; }
+
+hipError_t launch_small_sort(void* A, uint32_t n, uint32_t key_bits, hipStream_t s) {
+  if (n > small_sort_capacity()) return hipErrorInvalidValue;
+  if (n <= 256 * 8)
+    hipLaunchKernelGGL((k_small_sort<8, 256, 8>), dim3(1), dim3(256), 0, s,
+                       static_cast<elem_t*>(A), n, key_bits);
+  else
+    hipLaunchKernelGGL((k_small_sort<8, 512, 16>), dim3(1), dim3(512), 0, s,
+                       static_cast<elem_t*>(A), n, key_bits);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------ k_copy_back

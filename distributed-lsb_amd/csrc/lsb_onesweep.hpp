@@ -1,0 +1,275 @@
+// Device building blocks of one stable counting-sort pass over 16-byte
+// elements, shared by the plain sort (k_onesweep), the segmented sort
+// (k_seg_onesweep) and the LDS-resident small-array sort (k_small_sort).
+//
+// A tile is BLOCK*ITEMS elements.  Wave w owns the contiguous run of
+// 64*ITEMS elements starting at w*64*ITEMS; its item i, lane l is element
+// i*64 + l of that run, so every global load of a wave reads one contiguous
+// 1 KiB, and visiting (wave, item, lane) in order visits the tile in input
+// order -- which is what makes every pass stable.
+#pragma once
+
+#include "lsb_device.hpp"
+
+namespace lsb {
+
+// ------------------------------------------------------------- wave_rank
+// Ranks this wave's ITEMS rows into its LDS counters wh[0 .. 2^D).  Row i,
+// lane l is valid when i*64 + l < lim.  For every valid element: dg = its
+// digit, rk = number of elements of the same digit earlier in the wave's
+// run.  Peers (lanes holding the same digit) are found with D ballots; the
+// lowest peer bumps the counter, the others read the old value.
+template <int D, int ITEMS>
+__device__ __forceinline__ void wave_rank(const elem_t (&e)[ITEMS], uint32_t lim, uint32_t shift,
+                                          uint32_t mask, uint32_t* wh, uint32_t (&dg)[ITEMS],
+                                          uint32_t (&rk)[ITEMS]) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const bool valid = (uint32_t)i * kWave + lane < lim;
+    const uint32_t d = digit_of(e[i].key, shift, mask);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    const uint32_t below = count_below(peers);
+    uint32_t old = 0;
+    if (valid) old = wh[d];
+    if (valid && below == 0) wh[d] = old + (uint32_t)__popcll(peers);
+    dg[i] = d;
+    rk[i] = old + below;
+  }
+}
+
+// ------------------------------------------------------------- look-back
+// Exclusive prefix of digit b over tiles [0, tile) of one pass: walks back
+// from tile-1 adding AGGREGATE words until an INCLUSIVE word ends the walk.
+// Each round issues kLookback independent status loads, so a run of
+// aggregates costs one memory round trip per kLookback tiles.  Tile 0 always
+// publishes an inclusive word, so the walk never passes it; slots before
+// tile 0 read as "not ready" and are never consumed.
+constexpr int kLookback = 4;
+
+__device__ __forceinline__ uint32_t lookback(const uint32_t* __restrict__ st, uint32_t tile,
+                                             uint32_t nb, uint32_t b) {
+  uint32_t prefix = 0;
+  int64_t j = (int64_t)tile - 1;
+  while (true) {
+    uint32_t w[kLookback];
+#pragma unroll
+    for (int k = 0; k < kLookback; ++k)
+      w[k] = (j - k >= 0) ? ld_status(st + (size_t)(j - k) * nb + b) : 0u;
+    uint32_t used = 0;
+    bool stop = false, done = false;
+#pragma unroll
+    for (int k = 0; k < kLookback; ++k) {
+      if (!stop) {
+        if (w[k] == 0u) {
+          stop = true;
+        } else {
+          prefix += (w[k] & kCountMask) - 1u;
+          ++used;
+          if (w[k] & kFlagInclusive) stop = done = true;
+        }
+      }
+    }
+    if (done) return prefix;
+    j -= used;
+    if (used == 0) __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Shared memory of one tile pass.  The element buffer first holds the
+// per-wave digit counters (while the elements sit in registers), then the
+// elements in digit order.
+template <int D, int BLOCK, int ITEMS>
+struct TileSmem {
+  static constexpr int NB = 1 << D;
+  static constexpr int W = BLOCK / kWave;
+  static constexpr int T = BLOCK * ITEMS;
+  static_assert(W * NB * 4 <= T * 16, "per-wave counters must fit in the tile buffer");
+  elem_t elems[T];
+  uint32_t start[NB];  // tile-local exclusive bucket starts
+  uint32_t gbase[NB];  // destination index of tile-local slot 0, per bucket
+  uint32_t wsum[W];
+  uint32_t scalar[4];
+};
+
+// --------------------------------------------------------- onesweep_tile
+// One tile of one pass over an array (or one segment) of n elements:
+// src/dst point at element 0 of that array; `tile` counts from its first
+// tile; st_cur/st_next are the array's rows of the two look-back slices;
+// starts[b] is the global exclusive start of bucket b for this digit.
+// Must be entered by the whole block after a barrier.
+template <int D, int BLOCK, int ITEMS>
+__device__ __forceinline__ void onesweep_tile(TileSmem<D, BLOCK, ITEMS>& sm,
+                                              const elem_t* __restrict__ src,
+                                              elem_t* __restrict__ dst, uint32_t n,
+                                              uint32_t tile, uint32_t* st_cur,
+                                              uint32_t* st_next,
+                                              const uint32_t* __restrict__ starts,
+                                              uint32_t shift, uint32_t mask) {
+  using S = TileSmem<D, BLOCK, ITEMS>;
+  constexpr int NB = S::NB, W = S::W, T = S::T;
+  constexpr int DPT = (NB + BLOCK - 1) / BLOCK;  // digits per thread
+  const uint32_t t = threadIdx.x;
+  const uint32_t lane = t & (kWave - 1);
+  const uint32_t wave = t / kWave;
+  uint32_t* whist = reinterpret_cast<uint32_t*>(sm.elems);
+
+  for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
+  for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) st_next[(size_t)tile * NB + b] = 0u;
+
+  const uint32_t tile_base = tile * (uint32_t)T;
+  const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
+  const uint32_t wrun = wave * (kWave * ITEMS);  // this wave's first slot in the tile
+  const uint32_t lim = tile_n > wrun ? tile_n - wrun : 0u;
+  elem_t e[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    if ((uint32_t)i * kWave + lane < lim) {
+      e[i] = src[(size_t)tile_base + wrun + (uint32_t)i * kWave + lane];
+    } else {
+      e[i].key = 0;
+      e[i].val = 0;
+    }
+  }
+  __syncthreads();
+
+  uint32_t dg[ITEMS], rk[ITEMS];
+  uint32_t* wh = whist + wave * NB;
+  wave_rank<D, ITEMS>(e, lim, shift, mask, wh, dg, rk);
+  __syncthreads();
+
+  // Per digit: exclusive prefix over waves (in place) and the tile total,
+  // published at once as this tile's aggregate; then the look-back, so the
+  // inclusive prefix is out early for the tiles behind this one.
+  uint32_t gstart[DPT];
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const uint32_t b = t + k * BLOCK;
+    if (b < (uint32_t)NB) {
+      uint32_t s = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const uint32_t c = whist[w * NB + b];
+        whist[w * NB + b] = s;
+        s += c;
+      }
+      sm.start[b] = s;
+      uint32_t* mine = st_cur + (size_t)tile * NB + b;
+      uint32_t prefix = 0;
+      if (tile == 0) {
+        st_status(mine, (s + 1u) | kFlagInclusive);
+      } else {
+        st_status(mine, s + 1u);
+        prefix = lookback(st_cur, tile, NB, b);
+        st_status(mine, (prefix + s + 1u) | kFlagInclusive);
+      }
+      gstart[k] = starts[b] + prefix;
+    }
+  }
+  block_exclusive_scan<NB, BLOCK>(sm.start, sm.wsum);
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const uint32_t b = t + k * BLOCK;
+    if (b < (uint32_t)NB) sm.gbase[b] = gstart[k] - sm.start[b];
+  }
+
+  uint32_t pos[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) pos[i] = sm.start[dg[i]] + wh[dg[i]] + rk[i];
+  __syncthreads();  // counters are dead from here: the buffer takes elements
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if ((uint32_t)i * kWave + lane < lim) sm.elems[pos[i]] = e[i];
+  __syncthreads();
+
+  // Write out in digit order: consecutive lanes write consecutive slots of
+  // one bucket run.
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t j = k * BLOCK + t;
+    if (j < tile_n) {
+      const elem_t x = sm.elems[j];
+      const uint32_t d = digit_of(x.key, shift, mask);
+      dst[(size_t)(uint32_t)(sm.gbase[d] + j)] = x;
+    }
+  }
+}
+
+// ----------------------------------------------------------- small_sort
+// Sorts n <= BLOCK*ITEMS elements of one array entirely inside the block:
+// every digit pass ranks the elements held in registers, writes them to
+// LDS in digit order and reloads them in the wave-contiguous layout.  Only
+// one read and one write of HBM per element, whatever the pass count.
+// Digits whose elements all share one bucket are skipped.
+template <int D, int BLOCK, int ITEMS>
+__device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, elem_t* __restrict__ a,
+                                           uint32_t n, uint32_t key_bits) {
+  using S = TileSmem<D, BLOCK, ITEMS>;
+  constexpr int NB = S::NB, W = S::W;
+  const uint32_t t = threadIdx.x;
+  const uint32_t lane = t & (kWave - 1);
+  const uint32_t wave = t / kWave;
+  uint32_t* whist = reinterpret_cast<uint32_t*>(sm.elems);
+  const uint32_t wrun = wave * (kWave * ITEMS);
+  const uint32_t lim = n > wrun ? n - wrun : 0u;
+
+  elem_t e[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    if ((uint32_t)i * kWave + lane < lim) {
+      e[i] = a[wrun + (uint32_t)i * kWave + lane];
+    } else {
+      e[i].key = 0;
+      e[i].val = 0;
+    }
+  }
+  const uint32_t passes = (key_bits + D - 1) / D;
+  for (uint32_t p = 0; p < passes; ++p) {
+    const uint32_t shift = p * D;
+    const uint32_t w = key_bits - shift < (uint32_t)D ? key_bits - shift : (uint32_t)D;
+    const uint32_t mask = (1u << w) - 1u;
+    __syncthreads();  // previous pass's reload is done with the buffer
+    for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
+    if (t == 0) sm.scalar[0] = 0u;
+    __syncthreads();
+    uint32_t dg[ITEMS], rk[ITEMS];
+    uint32_t* wh = whist + wave * NB;
+    wave_rank<D, ITEMS>(e, lim, shift, mask, wh, dg, rk);
+    __syncthreads();
+    for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) {
+      uint32_t s = 0;
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        const uint32_t c = whist[k * NB + b];
+        whist[k * NB + b] = s;
+        s += c;
+      }
+      sm.start[b] = s;
+      if (s == n) sm.scalar[0] = 1u;  // one bucket holds everything
+    }
+    block_exclusive_scan<NB, BLOCK>(sm.start, sm.wsum);
+    if (sm.scalar[0]) continue;  // identity pass (uniform across the block)
+    uint32_t pos[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) pos[i] = sm.start[dg[i]] + wh[dg[i]] + rk[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i)
+      if ((uint32_t)i * kWave + lane < lim) sm.elems[pos[i]] = e[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i)
+      if ((uint32_t)i * kWave + lane < lim) e[i] = sm.elems[wrun + (uint32_t)i * kWave + lane];
+  }
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if ((uint32_t)i * kWave + lane < lim) a[wrun + (uint32_t)i * kWave + lane] = e[i];
+}
+
+}  // namespace lsb

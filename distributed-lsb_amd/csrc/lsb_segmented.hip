@@ -1,0 +1,320 @@
+// Batched segmented sort: thousands of independent arrays ("segments") of
+// uneven sizes, each sorted stably by key on its own, in one launch sequence.
+// The reference sorts one array per rank (mySort, mpi/mpi_lsbsort.cpp:
+// 591-597); a segment here is one such array.
+//
+// Every kernel reads the batch shape from a device header and runs a grid
+// sized to the device, not to the batch, so one captured hipGraph replays
+// for any batch that fits its workspace:
+//   k_seg_small     segments of at most one workgroup's capacity, sorted
+//                   entirely in LDS (small_sort): one HBM read + write.
+//   k_seg_zero      clears the per-segment digit counters.
+//   k_seg_hist      per-segment all-digit histograms over the tiles of the
+//                   large segments (+ clears look-back slice 0).
+//   k_seg_scan      per-segment exclusive scans, skip mask, pass parity.
+//   k_seg_onesweep  one launch per digit pass, persistent: blocks take tile
+//                   tickets across ALL large segments; look-back stays
+//                   inside a segment (its first tile publishes inclusive).
+//   k_seg_copy      moves a segment from B to A when it ran an odd number
+//                   of passes.
+#include "lsb_device.hpp"
+#include "lsb_internal.hpp"
+#include "lsb_onesweep.hpp"
+
+namespace lsb {
+
+// Binary search: the large segment holding global tile `tile`.
+__device__ __forceinline__ uint32_t find_segment(const SegLarge* __restrict__ large,
+                                                 uint32_t nlarge, uint32_t tile) {
+  uint32_t lo = 0, hi = nlarge - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) / 2;
+    if (large[mid].tile_begin <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ----------------------------------------------------------- small segments
+template <int D, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_seg_small(elem_t* A, const SegHeader* __restrict__ hdr,
+                                                     const SegSmall* __restrict__ list,
+                                                     int which, uint32_t key_bits) {
+  __shared__ TileSmem<D, BLOCK, ITEMS> sm;
+  // The metadata array holds the large segments, then the tiny, then the
+  // small ones (16 bytes each).
+  const uint32_t count = which == 0 ? hdr->n_tiny : hdr->n_small;
+  const SegSmall* l = list + hdr->n_large + (which == 0 ? 0u : hdr->n_tiny);
+  for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+    const SegSmall s = l[i];
+    small_sort<D, BLOCK, ITEMS>(sm, A + s.base, s.n, key_bits);
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ counters
+__global__ __launch_bounds__(256) void k_seg_zero(const SegHeader* __restrict__ hdr,
+                                                  uint32_t* __restrict__ counts,
+                                                  uint32_t words_per_seg) {
+  const size_t words = (size_t)hdr->n_large * words_per_seg;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (size_t)gridDim.x * 256)
+    counts[i] = 0u;
+}
+
+// Each block owns a contiguous range of tiles, accumulates the digit counts
+// of consecutive tiles of one segment in LDS and flushes them when the
+// segment changes.
+template <int D>
+__global__ __launch_bounds__(256) void k_seg_hist(const elem_t* __restrict__ A,
+                                                  const SegHeader* __restrict__ hdr,
+                                                  const SegLarge* __restrict__ large,
+                                                  uint32_t tile, uint32_t key_bits,
+                                                  uint32_t passes, uint32_t* __restrict__ counts,
+                                                  uint32_t* __restrict__ status0) {
+  constexpr int NB = 1 << D;
+  constexpr int MAXP = (64 + D - 1) / D;
+  __shared__ uint32_t h[MAXP * NB];
+  const uint32_t t = threadIdx.x;
+  const uint32_t total = hdr->total_tiles;
+  const uint32_t nlarge = hdr->n_large;
+  const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
+  const uint32_t t0 = blockIdx.x * per;
+  const uint32_t t1 = t0 + per < total ? t0 + per : total;
+  if (t0 >= t1) return;
+
+  // clear look-back slice 0 for these tiles
+  {
+    uint4* z = reinterpret_cast<uint4*>(status0 + (size_t)t0 * NB);
+    const size_t z4 = (size_t)(t1 - t0) * NB / 4;
+    for (size_t i = t; i < z4; i += 256) z[i] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  for (uint32_t i = t; i < passes * NB; i += 256) h[i] = 0u;
+  __syncthreads();
+
+  uint32_t seg = find_segment(large, nlarge, t0);
+  for (uint32_t tl = t0; tl < t1; ++tl) {
+    if (seg + 1 < nlarge && large[seg + 1].tile_begin <= tl) {
+      // flush the finished segment
+      __syncthreads();
+      for (uint32_t i = t; i < passes * NB; i += 256) {
+        const uint32_t c = h[i];
+        if (c) atomicAdd(&counts[(size_t)seg * MAXP * NB + i], c);
+        h[i] = 0u;
+      }
+      __syncthreads();
+      seg = find_segment(large, nlarge, tl);
+    }
+    const SegLarge L = large[seg];
+    const uint32_t first = (tl - L.tile_begin) * tile;
+    const uint32_t cnt = L.n - first < tile ? L.n - first : tile;
+    const elem_t* a = A + L.base + first;
+    for (uint32_t i0 = 0; i0 < cnt; i0 += 256 * 4) {
+      uint64_t k[4];
+      bool v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t i = i0 + u * 256 + t;
+        v[u] = i < cnt;
+        k[u] = v[u] ? a[i].key : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        for (uint32_t p = 0; p < passes; ++p) {
+          const uint32_t shift = p * D;
+          const uint32_t w = key_bits - shift < (uint32_t)D ? key_bits - shift : (uint32_t)D;
+          hist_add(h + p * NB, digit_of(k[u], shift, (1u << w) - 1u), v[u]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = t; i < passes * NB; i += 256) {
+    const uint32_t c = h[i];
+    if (c) atomicAdd(&counts[(size_t)seg * MAXP * NB + i], c);
+  }
+}
+
+// One block per large segment (grid-stride): in-place exclusive scan of
+// every pass's counts, then the segment's pass mask and final parity.
+template <int D>
+__global__ __launch_bounds__(1024) void k_seg_scan(const SegHeader* __restrict__ hdr,
+                                                   const SegLarge* __restrict__ large,
+                                                   uint32_t* __restrict__ counts,
+                                                   uint32_t* __restrict__ seg_plan,
+                                                   uint32_t* __restrict__ tickets,
+                                                   uint32_t passes, uint32_t flags) {
+  constexpr int NB = 1 << D;
+  constexpr int MAXP = (64 + D - 1) / D;
+  __shared__ uint32_t a[NB];
+  __shared__ uint32_t wsum[16];
+  const uint32_t t = threadIdx.x;
+  if (blockIdx.x == 0 && t < 16) tickets[t] = 0u;
+  const uint32_t nlarge = hdr->n_large;
+  for (uint32_t s = blockIdx.x; s < nlarge; s += gridDim.x) {
+    const uint32_t n = large[s].n;
+    uint32_t mask = 0;
+    for (uint32_t p = 0; p < passes; ++p) {
+      uint32_t* c = counts + (size_t)s * MAXP * NB + (size_t)p * NB;
+      int triv = 0;
+      for (uint32_t i = t; i < (uint32_t)NB; i += 1024) {
+        const uint32_t v = c[i];
+        a[i] = v;
+        triv |= (v == n);
+      }
+      triv = __syncthreads_or(triv);
+      if (!triv || (flags & 2u)) mask |= 1u << p;
+      block_exclusive_scan<NB, 1024>(a, wsum);
+      for (uint32_t i = t; i < (uint32_t)NB; i += 1024) c[i] = a[i];
+      __syncthreads();
+    }
+    if (t == 0) {
+      seg_plan[2 * s] = mask;
+      seg_plan[2 * s + 1] = (uint32_t)__popc(mask) & 1u;
+    }
+  }
+}
+
+// --------------------------------------------------------- k_seg_onesweep
+template <int D, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_seg_onesweep(SegPassArgs args) {
+  constexpr int NB = 1 << D;
+  constexpr int MAXP = (64 + D - 1) / D;
+  constexpr uint32_t T = BLOCK * ITEMS;
+  __shared__ TileSmem<D, BLOCK, ITEMS> sm;
+  const uint32_t total = args.hdr->total_tiles;
+  const uint32_t nlarge = args.hdr->n_large;
+  const uint32_t pass = args.pass;
+  const size_t slice = (size_t)args.max_tiles * NB;
+  while (true) {
+    if (threadIdx.x == 0) {
+      const uint32_t tile = atomicAdd(&args.tickets[pass], 1u);
+      uint32_t seg = 0, mask = 0;
+      if (tile < total) {
+        seg = find_segment(args.large, nlarge, tile);
+        mask = args.seg_plan[2 * seg];
+      }
+      sm.scalar[0] = tile;
+      sm.scalar[1] = seg;
+      sm.scalar[2] = mask;
+    }
+    __syncthreads();
+    const uint32_t tile = sm.scalar[0];
+    if (tile >= total) return;
+    const uint32_t seg = sm.scalar[1];
+    const uint32_t mask = sm.scalar[2];
+    if (mask & (1u << pass)) {
+      const SegLarge L = args.large[seg];
+      const uint32_t ord = (uint32_t)__popc(mask & ((1u << pass) - 1u));
+      elem_t* a = static_cast<elem_t*>(args.A) + L.base;
+      elem_t* b = static_cast<elem_t*>(args.B) + L.base;
+      const elem_t* src = (ord & 1u) ? b : a;
+      elem_t* dst = (ord & 1u) ? a : b;
+      uint32_t* st_cur = args.status + (size_t)(ord & 1u) * slice + (size_t)L.tile_begin * NB;
+      uint32_t* st_next =
+          args.status + (size_t)((ord + 1u) & 1u) * slice + (size_t)L.tile_begin * NB;
+      const uint32_t* starts = args.counts + (size_t)seg * MAXP * NB + (size_t)pass * NB;
+      onesweep_tile<D, BLOCK, ITEMS>(sm, src, dst, L.n, tile - L.tile_begin, st_cur, st_next,
+                                     starts, args.shift, args.mask);
+      (void)T;
+    }
+    __syncthreads();
+  }
+}
+
+// -------------------------------------------------------------- k_seg_copy
+__global__ __launch_bounds__(256) void k_seg_copy(elem_t* __restrict__ A,
+                                                  const elem_t* __restrict__ B,
+                                                  const SegHeader* __restrict__ hdr,
+                                                  const SegLarge* __restrict__ large,
+                                                  const uint32_t* __restrict__ seg_plan,
+                                                  uint32_t tile) {
+  const uint32_t total = hdr->total_tiles;
+  const uint32_t nlarge = hdr->n_large;
+  for (uint32_t tl = blockIdx.x; tl < total; tl += gridDim.x) {
+    const uint32_t seg = find_segment(large, nlarge, tl);
+    if (!seg_plan[2 * seg + 1]) continue;
+    const SegLarge L = large[seg];
+    const uint32_t first = (tl - L.tile_begin) * tile;
+    const uint32_t cnt = L.n - first < tile ? L.n - first : tile;
+    for (uint32_t i = threadIdx.x; i < cnt; i += 256)
+      A[L.base + first + i] = B[L.base + first + i];
+  }
+}
+
+// ----------------------------------------------------------------- launch
+template <int D, int BLOCK, int ITEMS>
+static hipError_t seg_onesweep_launch(const SegPassArgs& a, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_seg_onesweep<D, BLOCK, ITEMS>), dim3(grid), dim3(BLOCK), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int D, int BLOCK, int ITEMS>
+static uint32_t seg_onesweep_occupancy() {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_seg_onesweep<D, BLOCK, ITEMS>,
+                                                   BLOCK, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    blocks = 1;
+  }
+  return blocks > 0 ? (uint32_t)blocks : 1u;
+}
+
+static const SegVariant kSegVariants[] = {
+    {8, 256 * 16, 256, &seg_onesweep_launch<8, 256, 16>, &seg_onesweep_occupancy<8, 256, 16>},
+    {8, 512 * 16, 512, &seg_onesweep_launch<8, 512, 16>, &seg_onesweep_occupancy<8, 512, 16>},
+    {8, 256 * 8, 256, &seg_onesweep_launch<8, 256, 8>, &seg_onesweep_occupancy<8, 256, 8>},
+    {11, 512 * 12, 512, &seg_onesweep_launch<11, 512, 12>, &seg_onesweep_occupancy<11, 512, 12>},
+};
+
+const SegVariant* seg_variants(int* count) {
+  *count = (int)(sizeof(kSegVariants) / sizeof(kSegVariants[0]));
+  return kSegVariants;
+}
+
+uint32_t seg_tiny_capacity() { return 256 * 8; }
+uint32_t seg_small_capacity() { return 512 * 16; }
+
+hipError_t launch_seg_small(void* A, const SegHeader* hdr, const SegSmall* list,
+                            uint32_t key_bits, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_seg_small<8, 256, 8>), dim3(grid), dim3(256), 0, s,
+                     static_cast<elem_t*>(A), hdr, list, 0, key_bits);
+  hipLaunchKernelGGL((k_seg_small<8, 512, 16>), dim3(grid), dim3(512), 0, s,
+                     static_cast<elem_t*>(A), hdr, list, 1, key_bits);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_prepare(uint32_t d, const void* A, const SegHeader* hdr,
+                              const SegLarge* large, uint32_t tile, uint32_t key_bits,
+                              uint32_t passes, uint32_t* counts, uint32_t* seg_plan,
+                              uint32_t* tickets, uint32_t* status0, uint32_t flags,
+                              uint32_t grid, hipStream_t s) {
+  const elem_t* a = static_cast<const elem_t*>(A);
+  if (d == 8) {
+    constexpr int MAXP = 8;
+    hipLaunchKernelGGL(k_seg_zero, dim3(grid), dim3(256), 0, s, hdr, counts, MAXP * 256u);
+    hipLaunchKernelGGL(k_seg_hist<8>, dim3(grid), dim3(256), 0, s, a, hdr, large, tile,
+                       key_bits, passes, counts, status0);
+    hipLaunchKernelGGL(k_seg_scan<8>, dim3(grid), dim3(1024), 0, s, hdr, large, counts,
+                       seg_plan, tickets, passes, flags);
+  } else if (d == 11) {
+    constexpr int MAXP = 6;
+    hipLaunchKernelGGL(k_seg_zero, dim3(grid), dim3(256), 0, s, hdr, counts, MAXP * 2048u);
+    hipLaunchKernelGGL(k_seg_hist<11>, dim3(grid), dim3(256), 0, s, a, hdr, large, tile,
+                       key_bits, passes, counts, status0);
+    hipLaunchKernelGGL(k_seg_scan<11>, dim3(grid), dim3(1024), 0, s, hdr, large, counts,
+                       seg_plan, tickets, passes, flags);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_copy(void* A, const void* B, const SegHeader* hdr, const SegLarge* large,
+                           const uint32_t* seg_plan, uint32_t tile, uint32_t grid,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_copy, dim3(grid), dim3(256), 0, s, static_cast<elem_t*>(A),
+                     static_cast<const elem_t*>(B), hdr, large, seg_plan, tile);
+  return hipGetLastError();
+}
+
+}  // namespace lsb

@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(PKG, "liblsb_amd.so")
 
 LSB_OK, LSB_EINVAL, LSB_ENOMEM, LSB_EHIP, LSB_ERANGE, LSB_EBUSY, LSB_ENOTFOUND = (
     0, -1, -2, -3, -4, -5, -6)
-LSB_SYNC, LSB_NO_SKIP, LSB_PROFILE = 1, 2, 4
+LSB_SYNC, LSB_NO_SKIP, LSB_PROFILE, LSB_GRAPH = 1, 2, 4, 8
 LSB_MAX_N = 0x7FFFFFFE
 GEN_UNIFORM, GEN_SORTED, GEN_REVERSE, GEN_ZIPF = 0, 1, 2, 3
 
@@ -75,6 +75,8 @@ SIGNATURES = {
     "lsb_strerror": (ctypes.c_char_p, [_I]),
     "lsb_version": (ctypes.c_char_p, []),
     "lsb_last_hip_error": (_I, []),
+    "lsb_sort_segmented": (_I, [_P, _P, ctypes.POINTER(_U64), _U32, ctypes.POINTER(Config)]),
+    "lsb_segmented_workspace_size": (_SZ, [_SZ, _U32, ctypes.POINTER(Config)]),
     "lsb_profile_read": (_I, [ctypes.POINTER(Profile)]),
     "lsb_debug_histogram": (_I, [_P, _SZ, ctypes.POINTER(Config), _P]),
     "lsb_generate": (_I, [_P, _SZ, ctypes.POINTER(GenSpec), _P]),
@@ -180,13 +182,34 @@ def workspace_size(n, key_bits=64, digit_bits=0, tile=0):
 
 
 def sort(A, B, n=None, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,
-         no_skip=False, workspace=None, workspace_bytes=0, profile=False):
+         no_skip=False, workspace=None, workspace_bytes=0, profile=False, graph=False):
     """mySort(A, B): stable sort of A[0:n] by key, B is scratch, result in A."""
     n = _nelem(A) if n is None else n
     flags = ((LSB_SYNC if sync else 0) | (LSB_NO_SKIP if no_skip else 0)
-             | (LSB_PROFILE if profile else 0))
+             | (LSB_PROFILE if profile else 0) | (LSB_GRAPH if graph else 0))
     c = make_config(key_bits, digit_bits, tile, stream, flags, workspace, workspace_bytes)
     _check(lib().lsb_sort(_ptr(A), _ptr(B), n, ctypes.byref(c)), "lsb_sort")
+
+
+def sort_segmented(A, B, offsets, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,
+                   no_skip=False, graph=False, workspace=None, workspace_bytes=0):
+    """Sorts every segment [offsets[s], offsets[s+1]) of A independently and
+    stably by key; B is scratch; results in A.  offsets: host sequence of
+    nseg + 1 non-decreasing element offsets."""
+    import numpy as _np
+    off = _np.ascontiguousarray(_np.asarray(offsets, dtype=_np.uint64))
+    nseg = len(off) - 1
+    flags = ((LSB_SYNC if sync else 0) | (LSB_NO_SKIP if no_skip else 0)
+             | (LSB_GRAPH if graph else 0))
+    c = make_config(key_bits, digit_bits, tile, stream, flags, workspace, workspace_bytes)
+    _check(lib().lsb_sort_segmented(_ptr(A), _ptr(B),
+                                    off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                    max(nseg, 0), ctypes.byref(c)), "lsb_sort_segmented")
+
+
+def segmented_workspace_size(total_n, nseg, key_bits=64, digit_bits=0, tile=0):
+    c = make_config(key_bits, digit_bits, tile, stream=0)
+    return lib().lsb_segmented_workspace_size(total_n, nseg, ctypes.byref(c))
 
 
 def profile_read():

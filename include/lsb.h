@@ -53,6 +53,7 @@ typedef struct lsb_elem {
 #define LSB_SYNC 1u     /* return only after the sort finished on the stream */
 #define LSB_NO_SKIP 2u  /* run every digit pass, even single-bucket ones */
 #define LSB_PROFILE 4u  /* record HIP events around each kernel of this call (lsb_profile_read) */
+#define LSB_GRAPH 8u    /* run the launch sequence as a captured hipGraph, cached per buffer set */
 
 typedef struct lsb_config {
   uint32_t key_bits;          /* sort by the low key_bits bits of key (1..64); 0 = 64.
@@ -96,6 +97,24 @@ const char* lsb_strerror(int code);
 const char* lsb_version(void);
 /* hipError_t of the last failed HIP call on this thread (0 if none). */
 int lsb_last_hip_error(void);
+
+/* ------------------------------------------------------ batched segments */
+
+/* Sorts each segment [seg_offsets[s], seg_offsets[s+1]) of A, s < nseg,
+ * stably by key and independently of the others; B (same extent) is
+ * scratch; results in A.  seg_offsets is a HOST array of nseg+1
+ * non-decreasing element offsets; each segment holds at most LSB_MAX_N
+ * elements.  One launch sequence covers the whole batch: segments of up to
+ * one workgroup's capacity are sorted entirely in LDS, larger ones share
+ * persistent onesweep passes whose look-back stays inside each segment.  Each
+ * segment is one rank's mySort (mpi/mpi_lsbsort.cpp:591-597); the reference
+ * has no batched form.  With LSB_GRAPH the sequence is a cached hipGraph
+ * that is replayed for every batch fitting the library workspace. */
+int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, uint32_t nseg,
+                       const lsb_config* cfg);
+
+/* Device workspace bytes for a caller-provided segmented workspace. */
+size_t lsb_segmented_workspace_size(size_t total_n, uint32_t nseg, const lsb_config* cfg);
 
 /* Kernel timings of the last lsb_sort call made with LSB_PROFILE on this
  * thread, from HIP events recorded on its stream between its launches.  No

@@ -88,6 +88,13 @@ def gen_zipf(n, ranks=1, seed_base=0, m=1 << 20, s=1.1, table_seed=12345,
     return out
 
 
+def c4_sizes(nseg, seed=2024, lo_log2=10, span_log2=12):
+    """Batched-sort segment sizes: floor(2^(lo + span*u)), u = 53-bit uniform
+    from output s of pcg64(seed)."""
+    u = (pcg64(seed, nseg) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return np.floor(2.0 ** (lo_log2 + span_log2 * u)).astype(np.int64)
+
+
 def n_digits(key_bits, d):
     return (key_bits + d - 1) // d
 

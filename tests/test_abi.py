@@ -88,3 +88,29 @@ def test_generate_and_verify_reject_bad_args():
     assert first.value == -1
     assert L.lsb_verify(None, None, 5, 0, 64, ctypes.byref(bad), ctypes.byref(first),
                         None) == lsb_amd.LSB_EINVAL
+
+
+def _seg_rc(A, B, offsets, **kw):
+    off = (ctypes.c_uint64 * len(offsets))(*offsets)
+    c = lsb_amd.make_config(stream=0, **kw)
+    return lsb_amd.lib().lsb_sort_segmented(A, B, off, len(offsets) - 1, ctypes.byref(c))
+
+
+def test_segmented_argument_errors_need_no_gpu():
+    a, b = 0x10000, 0x80000000
+    assert _seg_rc(a, b, [0]) == 0                          # no segments
+    assert _seg_rc(None, None, [0, 0, 1, 1, 2]) == 0        # only 0/1-element segments
+    assert _seg_rc(a, b, [0, 10, 5]) == lsb_amd.LSB_EINVAL  # decreasing
+    assert _seg_rc(None, b, [0, 10]) == lsb_amd.LSB_EINVAL
+    assert _seg_rc(a + 8, b, [0, 10]) == lsb_amd.LSB_EINVAL
+    assert _seg_rc(a, a + 32, [0, 10]) == lsb_amd.LSB_EINVAL  # A and B overlap
+    assert _seg_rc(a, b, [0, lsb_amd.LSB_MAX_N + 1]) == lsb_amd.LSB_ERANGE
+    assert _seg_rc(a, b, [0, 10], key_bits=70) == lsb_amd.LSB_EINVAL
+    assert _seg_rc(a, b, [0, 10], flags=lsb_amd.LSB_PROFILE) == lsb_amd.LSB_EINVAL
+
+
+def test_segmented_workspace_grows_with_capacity():
+    w1 = lsb_amd.segmented_workspace_size(1 << 20, 16)
+    w2 = lsb_amd.segmented_workspace_size(1 << 24, 16)
+    w3 = lsb_amd.segmented_workspace_size(1 << 24, 4096)
+    assert 0 < w1 < w2 < w3

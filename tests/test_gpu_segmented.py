@@ -1,0 +1,176 @@
+"""GPU parity of the batched segmented sort: every segment of a batch must
+equal numpy's stable sort of that segment alone, bit for bit, whatever the
+other segments hold -- in eager mode and when replayed from a captured
+hipGraph."""
+import numpy as np
+import pytest
+import torch
+
+import lsb_amd
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def to_dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int64).reshape(len(a), 2).copy()).to(DEV)
+
+
+def from_dev(t):
+    return t.cpu().numpy().reshape(-1).view(O.ELEM)
+
+
+def expected(a, offsets, key_bits=64):
+    out = a.copy()
+    m = np.uint64((1 << key_bits) - 1) if key_bits < 64 else None
+    for s in range(len(offsets) - 1):
+        lo, hi = offsets[s], offsets[s + 1]
+        seg = a[lo:hi]
+        k = seg["key"] if m is None else seg["key"] & m
+        out[lo:hi] = seg[np.argsort(k, kind="stable")]
+    return out
+
+
+def batch(sizes, seed=0, kind="uniform"):
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    n = int(offsets[-1])
+    if kind == "uniform":
+        a = O.gen_uniform(n, ranks=7, seed_base=seed)
+    elif kind == "zipf":
+        a = O.gen_zipf(n, ranks=3, m=1 << 10, seed_base=seed)
+    elif kind == "low32":
+        a = O.gen_uniform(n, ranks=5, key_bits=32, seed_base=seed)
+    else:
+        raise ValueError(kind)
+    return a, offsets
+
+
+MIXED = [0, 1, 2, 3, 64, 1000, 2048, 2049, 4096, 8191, 8192, 8193, 9000, 20000, 70001,
+         0, 5, 300000, 2047, 12345]
+
+
+@pytest.mark.parametrize("kind", ["uniform", "zipf", "low32"])
+def test_segmented_mixed_sizes(kind):
+    a, off = batch(MIXED, seed=3, kind=kind)
+    want = expected(a, off)
+    for d, tile in lsb_amd.variants():
+        A = to_dev(a)
+        B = torch.empty_like(A)
+        lsb_amd.sort_segmented(A, B, off, digit_bits=d, tile=tile, sync=True)
+        assert np.array_equal(from_dev(A), want), (kind, d, tile)
+
+
+def test_segmented_skips_and_parity():
+    """Segments whose keys leave some or all digits in one bucket run an odd
+    or even number of passes; each must still end in A."""
+    sizes = [50000, 30000, 9000, 100000, 7000]
+    a, off = batch(sizes, seed=9)
+    a["key"][off[0]:off[1]] = 77                                         # no pass runs
+    a["key"][off[1]:off[2]] &= np.uint64(0xFF)                           # one pass
+    a["key"][off[2]:off[3]] &= np.uint64(0xFFFFFF)                       # three passes
+    a["key"][off[3]:off[4]] = (a["key"][off[3]:off[4]] % np.uint64(5)) << np.uint64(56)
+    want = expected(a, off)
+    for no_skip in (False, True):
+        A = to_dev(a)
+        B = torch.empty_like(A)
+        lsb_amd.sort_segmented(A, B, off, sync=True, no_skip=no_skip)
+        assert np.array_equal(from_dev(A), want)
+
+
+@pytest.mark.parametrize("key_bits", [8, 21, 32])
+def test_segmented_key_bits(key_bits):
+    a, off = batch([3000, 40000, 10000, 1], seed=4)
+    want = expected(a, off, key_bits)
+    A = to_dev(a)
+    B = torch.empty_like(A)
+    lsb_amd.sort_segmented(A, B, off, key_bits=key_bits, sync=True)
+    assert np.array_equal(from_dev(A), want)
+
+
+def test_segment_alone_equals_segment_in_batch():
+    """A segment's result does not depend on its neighbours."""
+    a, off = batch([20000, 5000, 90000, 3], seed=5)
+    A = to_dev(a)
+    B = torch.empty_like(A)
+    lsb_amd.sort_segmented(A, B, off, sync=True)
+    batched = from_dev(A)
+    for s in range(len(off) - 1):
+        lo, hi = int(off[s]), int(off[s + 1])
+        A1 = to_dev(a[lo:hi])
+        B1 = torch.empty_like(A1)
+        lsb_amd.sort(A1, B1, sync=True)
+        assert np.array_equal(from_dev(A1), batched[lo:hi])
+
+
+def test_graph_replay_many_batches():
+    """One captured graph per buffer set, replayed for batches of different
+    shapes, with eager work and synchronisations between replays."""
+    cap = 400_000
+    A = torch.empty((cap, 2), dtype=torch.int64, device=DEV)
+    B = torch.empty_like(A)
+    rng = np.random.default_rng(11)
+    for rep in range(5):
+        sizes = list(rng.integers(0, 40000, size=int(rng.integers(1, 12))))
+        a, off = batch(sizes, seed=rep)
+        n = int(off[-1])
+        A[:n].copy_(to_dev(a))
+        lsb_amd.sort_segmented(A, B, off, graph=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(A[:n]), expected(a, off)), rep
+        # unrelated eager work between replays
+        x = torch.arange(1000, device=DEV).sum()
+        assert int(x) == 499500
+
+
+def test_plain_sort_graph_replay():
+    n = 300_000
+    a = O.gen_uniform(n, ranks=2)
+    want = a[np.argsort(a["key"], kind="stable")]
+    A = to_dev(a)
+    B = torch.empty_like(A)
+    src = A.clone()
+    for rep in range(3):
+        A.copy_(src)
+        lsb_amd.sort(A, B, graph=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(A), want), rep
+
+
+def test_segmented_rejects_bad_offsets():
+    A = torch.empty((100, 2), dtype=torch.int64, device=DEV)
+    B = torch.empty_like(A)
+    with pytest.raises(lsb_amd.LsbError):
+        lsb_amd.sort_segmented(A, B, [0, 50, 40, 100])
+    lsb_amd.sort_segmented(A, B, [0])            # no segments
+    lsb_amd.sort_segmented(A, B, [0, 0, 1, 1])   # only trivial segments
+
+
+def test_c4_batched_4096_segments_full_scale():
+    """4096 segments, sizes floor(2^(10 + 12u)), u uniform, ~2^31 elements;
+    each segment checked on the device with the O(n) invariants."""
+    sizes = O.c4_sizes(4096, seed=2024)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    total = int(off[-1])
+    A = torch.empty((total, 2), dtype=torch.int64, device=DEV)
+    for s in range(len(sizes)):
+        lo, n = int(off[s]), int(sizes[s])
+        lsb_amd.generate(A[lo:lo + n], kind=lsb_amd.GEN_UNIFORM, ranks=1, seed_base=1000 + s,
+                         val_base=0)
+    keys = torch.empty(total, dtype=torch.int64, device=DEV)
+    lsb_amd.extract_keys(A, keys)
+    B = torch.empty_like(A)
+    lsb_amd.sort_segmented(A, B, off, sync=True)
+    del B
+    for s in range(len(sizes)):
+        lo, n = int(off[s]), int(sizes[s])
+        bad, first = lsb_amd.verify(A[lo:lo + n], keys[lo:lo + n], n=n, val_base=0)
+        assert bad == 0, (s, n, first)
+    # and a few segments bit-exactly against the oracle
+    for s in (0, 1, int(np.argmin(sizes))):
+        lo, n = int(off[s]), int(sizes[s])
+        want = O.gen_uniform(n, ranks=1, seed_base=1000 + s)
+        want = want[np.argsort(want["key"], kind="stable")]
+        assert np.array_equal(from_dev(A[lo:lo + n]), want)
+    del A, keys
+    torch.cuda.empty_cache()
