@@ -60,6 +60,24 @@ class Profile(ctypes.Structure):
                 ("pass_ms", ctypes.c_float * 16)]
 
 
+class ServerConfig(ctypes.Structure):
+    """struct lsb_server_config."""
+    _fields_ = [("memory_budget", ctypes.c_size_t), ("max_pending", ctypes.c_uint32),
+                ("batch_max_elems", ctypes.c_uint32), ("batch_max_total", ctypes.c_uint64),
+                ("streams", ctypes.c_uint32), ("use_graphs", ctypes.c_uint32),
+                ("batch_window_us", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    """struct lsb_server_stats."""
+    _fields_ = [("jobs_done", ctypes.c_uint64), ("elems_done", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("batched_jobs", ctypes.c_uint64),
+                ("plain_jobs", ctypes.c_uint64), ("memory_in_use", ctypes.c_uint64),
+                ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double),
+                ("max_us", ctypes.c_double), ("mean_us", ctypes.c_double),
+                ("span_s", ctypes.c_double)]
+
+
 _lib = None
 
 # Every function include/lsb.h declares, with its ctypes signature.
@@ -77,6 +95,16 @@ SIGNATURES = {
     "lsb_last_hip_error": (_I, []),
     "lsb_sort_segmented": (_I, [_P, _P, ctypes.POINTER(_U64), _U32, ctypes.POINTER(Config)]),
     "lsb_segmented_workspace_size": (_SZ, [_SZ, _U32, ctypes.POINTER(Config)]),
+    "lsb_server_config_init": (None, [ctypes.POINTER(ServerConfig)]),
+    "lsb_server_create": (_I, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
+    "lsb_server_destroy": (_I, [_P]),
+    "lsb_server_submit": (_I, [_P, _P, _SZ, _U32, ctypes.POINTER(_U64)]),
+    "lsb_server_poll": (_I, [_P, _U64]),
+    "lsb_server_wait": (_I, [_P, _U64, ctypes.c_double]),
+    "lsb_server_drain": (_I, [_P]),
+    "lsb_server_latency": (_I, [_P, _U64, ctypes.POINTER(ctypes.c_double)]),
+    "lsb_server_stats_get": (_I, [_P, ctypes.POINTER(ServerStats)]),
+    "lsb_server_reset_stats": (_I, [_P]),
     "lsb_profile_read": (_I, [ctypes.POINTER(Profile)]),
     "lsb_debug_histogram": (_I, [_P, _SZ, ctypes.POINTER(Config), _P]),
     "lsb_generate": (_I, [_P, _SZ, ctypes.POINTER(GenSpec), _P]),
@@ -253,3 +281,73 @@ def extract_keys(A, keys, n=None, stream=None):
     n = _nelem(A) if n is None else n
     _check(lib().lsb_extract_keys(_ptr(A), n, _ptr(keys), _stream_ptr(stream)),
            "lsb_extract_keys")
+
+
+class Server:
+    """Continuous-batching sort service (lsb_server_*): jobs are device
+    arrays sorted in place; small jobs arriving while the device is busy are
+    coalesced into segmented launches."""
+
+    def __init__(self, memory_budget=0, max_pending=0, batch_max_elems=0, batch_max_total=0,
+                 streams=0, use_graphs=True, batch_window_us=0):
+        c = ServerConfig()
+        lib().lsb_server_config_init(ctypes.byref(c))
+        if memory_budget:
+            c.memory_budget = memory_budget
+        if max_pending:
+            c.max_pending = max_pending
+        if batch_max_elems:
+            c.batch_max_elems = batch_max_elems
+        if batch_max_total:
+            c.batch_max_total = batch_max_total
+        if streams:
+            c.streams = streams
+        if batch_window_us:
+            c.batch_window_us = batch_window_us
+        c.use_graphs = 1 if use_graphs else 0
+        h = ctypes.c_void_p()
+        _check(lib().lsb_server_create(ctypes.byref(c), ctypes.byref(h)), "lsb_server_create")
+        self._h = h
+
+    def submit(self, A, n=None, key_bits=64):
+        n = _nelem(A) if n is None else n
+        jid = ctypes.c_uint64()
+        _check(lib().lsb_server_submit(self._h, _ptr(A), n, key_bits, ctypes.byref(jid)),
+               "lsb_server_submit")
+        return jid.value
+
+    def poll(self, job):
+        rc = lib().lsb_server_poll(self._h, job)
+        if rc < 0:
+            raise LsbError(rc, "lsb_server_poll")
+        return rc == 1
+
+    def wait(self, job, timeout_s=-1.0):
+        _check(lib().lsb_server_wait(self._h, job, timeout_s), "lsb_server_wait")
+
+    def drain(self):
+        _check(lib().lsb_server_drain(self._h), "lsb_server_drain")
+
+    def latency_us(self, job):
+        v = ctypes.c_double()
+        _check(lib().lsb_server_latency(self._h, job, ctypes.byref(v)), "lsb_server_latency")
+        return v.value
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().lsb_server_stats_get(self._h, ctypes.byref(st)), "lsb_server_stats_get")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        _check(lib().lsb_server_reset_stats(self._h), "lsb_server_reset_stats")
+
+    def close(self):
+        if self._h:
+            _check(lib().lsb_server_destroy(self._h), "lsb_server_destroy")
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -116,6 +116,60 @@ int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, ui
 /* Device workspace bytes for a caller-provided segmented workspace. */
 size_t lsb_segmented_workspace_size(size_t total_n, uint32_t nseg, const lsb_config* cfg);
 
+/* ---------------------------------------------------------------- serving */
+
+/* A sort service over jobs resident in HBM.  Jobs are admitted in arrival
+ * order against a memory budget that charges each job its doubled buffers
+ * (A + B = 32 bytes per element); small jobs admitted while the device is
+ * busy are coalesced into one segmented launch (replayed from a hipGraph),
+ * larger ones run as plain sorts; launches spread over several streams so
+ * their work overlaps.  Each job is sorted in place exactly as lsb_sort
+ * would sort it alone.  No counterpart in the reference (one array per run). */
+typedef struct lsb_server lsb_server;
+
+typedef struct lsb_server_config {
+  size_t memory_budget;      /* bytes; 0 = 200e9 */
+  uint32_t max_pending;      /* submitted, not yet admitted; 0 = 65536 */
+  uint32_t batch_max_elems;  /* jobs of at most this many elements are coalesced; 0 = 2^20 */
+  uint64_t batch_max_total;  /* elements per coalesced launch (staging slab); 0 = 2^26 */
+  uint32_t streams;          /* device streams; 0 = 4 */
+  uint32_t use_graphs;       /* 1 = coalesced launches replay captured hipGraphs */
+  uint32_t batch_window_us;  /* longest wait before an open batch is launched; 0 = 200 */
+  uint32_t reserved;
+} lsb_server_config;
+
+typedef struct lsb_server_stats {
+  uint64_t jobs_done;
+  uint64_t elems_done;
+  uint64_t batches;          /* coalesced launches */
+  uint64_t batched_jobs;     /* jobs that ran inside them */
+  uint64_t plain_jobs;       /* jobs that ran as plain sorts */
+  uint64_t memory_in_use;    /* budget bytes currently charged */
+  double p50_us, p99_us, max_us, mean_us;  /* completion - submission */
+  double span_s;             /* last completion - first submission */
+} lsb_server_stats;
+
+void lsb_server_config_init(lsb_server_config* cfg);
+int lsb_server_create(const lsb_server_config* cfg, lsb_server** out);
+/* Finishes every submitted job, then frees the server. */
+int lsb_server_destroy(lsb_server* srv);
+/* Queues a sort of A[0..n) (device memory, sorted in place by the low
+ * key_bits of key; 0 = 64).  A must stay valid until the job completes.
+ * LSB_EBUSY when max_pending jobs are waiting; LSB_ENOMEM when the job alone
+ * exceeds the budget. */
+int lsb_server_submit(lsb_server* srv, lsb_elem* A, size_t n, uint32_t key_bits,
+                      uint64_t* job_id);
+/* 1 = done, 0 = not yet, negative = the job failed with that code. */
+int lsb_server_poll(lsb_server* srv, uint64_t job_id);
+/* Waits up to timeout_s (< 0: forever); LSB_EBUSY on timeout. */
+int lsb_server_wait(lsb_server* srv, uint64_t job_id, double timeout_s);
+/* Waits until every submitted job has completed. */
+int lsb_server_drain(lsb_server* srv);
+int lsb_server_latency(lsb_server* srv, uint64_t job_id, double* us);
+int lsb_server_stats_get(lsb_server* srv, lsb_server_stats* out);
+/* Clears the statistics and forgets completed jobs. */
+int lsb_server_reset_stats(lsb_server* srv);
+
 /* Kernel timings of the last lsb_sort call made with LSB_PROFILE on this
  * thread, from HIP events recorded on its stream between its launches.  No
  * counterpart in the reference, which times only the whole sort with

@@ -114,3 +114,21 @@ def test_segmented_workspace_grows_with_capacity():
     w2 = lsb_amd.segmented_workspace_size(1 << 24, 16)
     w3 = lsb_amd.segmented_workspace_size(1 << 24, 4096)
     assert 0 < w1 < w2 < w3
+
+
+def test_server_rejects_bad_config_without_gpu():
+    L = lsb_amd.lib()
+    c = lsb_amd.ServerConfig()
+    L.lsb_server_config_init(ctypes.byref(c))
+    assert c.memory_budget == 200_000_000_000 and c.streams == 4
+    h = ctypes.c_void_p()
+    bad = lsb_amd.ServerConfig()
+    L.lsb_server_config_init(ctypes.byref(bad))
+    bad.streams = 99
+    assert L.lsb_server_create(ctypes.byref(bad), ctypes.byref(h)) == lsb_amd.LSB_EINVAL
+    L.lsb_server_config_init(ctypes.byref(bad))
+    bad.batch_max_elems, bad.batch_max_total = 1 << 20, 1 << 10
+    assert L.lsb_server_create(ctypes.byref(bad), ctypes.byref(h)) == lsb_amd.LSB_EINVAL
+    assert L.lsb_server_create(ctypes.byref(c), None) == lsb_amd.LSB_EINVAL
+    jid = ctypes.c_uint64()
+    assert L.lsb_server_submit(None, None, 5, 64, ctypes.byref(jid)) == lsb_amd.LSB_EINVAL

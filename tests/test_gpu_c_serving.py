@@ -1,0 +1,125 @@
+"""The serving loop returns, for every job, exactly what a lone lsb_sort of
+that job returns -- whether the job ran by itself, inside a coalesced batch,
+or overlapped with other jobs on other streams -- and its admission respects
+the memory budget."""
+import threading
+
+import numpy as np
+import pytest
+import torch
+
+import lsb_amd
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def to_dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int64).reshape(len(a), 2).copy()).to(DEV)
+
+
+def from_dev(t):
+    return t.cpu().numpy().reshape(-1).view(O.ELEM)
+
+
+def make_jobs(count, seed, max_log2=20):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(count):
+        n = int(2 ** rng.uniform(0, max_log2))
+        kb = 32 if rng.random() < 0.5 else 64
+        if rng.random() < 0.5:
+            a = O.gen_uniform(n, ranks=2, key_bits=kb, seed_base=1000 * seed + i)
+        else:
+            a = O.gen_zipf(n, ranks=2, m=1 << 12, seed_base=1000 * seed + i)
+            if kb == 32:
+                a["key"] &= np.uint64(0xFFFFFFFF)
+        jobs.append((a, kb))
+    return jobs
+
+
+def alone(a, kb):
+    A = to_dev(a)
+    if len(a) > 1:
+        B = torch.empty_like(A)
+        lsb_amd.sort(A, B, key_bits=kb, sync=True)
+    return from_dev(A)
+
+
+def expected(a, kb):
+    k = a["key"] if kb >= 64 else a["key"] & np.uint64((1 << kb) - 1)
+    return a[np.argsort(k, kind="stable")]
+
+
+def test_each_job_same_alone_and_in_the_mix():
+    jobs = make_jobs(120, seed=1)
+    with lsb_amd.Server(streams=4) as srv:
+        bufs = [to_dev(a) for a, _ in jobs]
+        torch.cuda.synchronize()
+        ids = [srv.submit(buf, key_bits=kb) for buf, (_, kb) in zip(bufs, jobs)]
+        srv.drain()
+        st = srv.stats()
+        for jid in ids:
+            assert srv.poll(jid)
+            assert srv.latency_us(jid) > 0
+    assert st["jobs_done"] == len(jobs)
+    assert st["batched_jobs"] + st["plain_jobs"] == len(jobs)
+    assert st["p99_us"] >= st["p50_us"] > 0
+    for (a, kb), buf in zip(jobs, bufs):
+        got = from_dev(buf)
+        assert np.array_equal(got, expected(a, kb))
+        assert np.array_equal(got, alone(a, kb))
+
+
+def test_coalescing_happens_under_load():
+    """Many small jobs submitted at once: most must run inside batches."""
+    rng = np.random.default_rng(5)
+    jobs = [O.gen_uniform(int(rng.integers(100, 60000)), ranks=1, seed_base=i) for i in range(200)]
+    with lsb_amd.Server(streams=2, batch_window_us=2000) as srv:
+        bufs = [to_dev(a) for a in jobs]
+        torch.cuda.synchronize()
+        for b in bufs:
+            srv.submit(b)
+        srv.drain()
+        st = srv.stats()
+    assert st["batches"] >= 1 and st["batched_jobs"] >= 100, st
+    for a, b in zip(jobs, bufs):
+        assert np.array_equal(from_dev(b), expected(a, 64))
+
+
+def test_budget_admission_and_rejection():
+    """A budget that holds one large job at a time still completes every job
+    (FIFO admission waits for memory); a job larger than the budget is
+    refused at submission."""
+    n = 1 << 20
+    budget = 32 * n + 32 * 1000
+    jobs = [O.gen_uniform(n, ranks=1, seed_base=s) for s in range(4)]
+    with lsb_amd.Server(memory_budget=budget, streams=3) as srv:
+        bufs = [to_dev(a) for a in jobs]
+        torch.cuda.synchronize()
+        ids = [srv.submit(b) for b in bufs]
+        with pytest.raises(lsb_amd.LsbError):
+            srv.submit(bufs[0], n=2 * n)
+        for jid in ids:
+            srv.wait(jid, 60.0)
+        assert srv.stats()["memory_in_use"] == 0
+    for a, b in zip(jobs, bufs):
+        assert np.array_equal(from_dev(b), expected(a, 64))
+
+
+def test_concurrent_submitters():
+    jobs = make_jobs(40, seed=7, max_log2=17)
+    bufs = [to_dev(a) for a, _ in jobs]
+    torch.cuda.synchronize()
+    with lsb_amd.Server() as srv:
+        def worker(lo):
+            for i in range(lo, len(jobs), 2):
+                srv.wait(srv.submit(bufs[i], key_bits=jobs[i][1]), 60.0)
+        ts = [threading.Thread(target=worker, args=(k,)) for k in (0, 1)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+    for (a, kb), b in zip(jobs, bufs):
+        assert np.array_equal(from_dev(b), expected(a, kb))
