@@ -65,7 +65,7 @@ int make_plan(size_t n, const lsb_config& c, Plan* p) {
   p->key_bits = c.key_bits ? c.key_bits : 64;
   if (p->key_bits > 64) return LSB_EINVAL;
   uint32_t d = c.digit_bits, tile = c.tile;
-  if (d == 0 && tile == 0) {
+  if (d == 0 && tile == 0 && c.variant == 0) {
     auto_choice(n, p->key_bits, &d, &tile);
   } else if (d == 0) {
     d = 8;
@@ -73,11 +73,19 @@ int make_plan(size_t n, const lsb_config& c, Plan* p) {
   int nv = 0;
   const lsb::SortVariant* vs = lsb::sort_variants(&nv);
   const lsb::SortVariant* pick = nullptr;
-  for (int i = 0; i < nv; ++i) {
-    if (vs[i].digit_bits != d) continue;
-    if (tile == 0 || vs[i].tile == tile) {
-      pick = &vs[i];
-      break;
+  if (c.variant) {
+    if (c.variant > (uint32_t)nv) return LSB_EINVAL;
+    pick = &vs[c.variant - 1];
+    if ((c.digit_bits && c.digit_bits != pick->digit_bits) || (c.tile && c.tile != pick->tile))
+      return LSB_EINVAL;
+    d = pick->digit_bits;
+  } else {
+    for (int i = 0; i < nv; ++i) {
+      if (vs[i].digit_bits != d) continue;
+      if (tile == 0 || vs[i].tile == tile) {
+        pick = &vs[i];
+        break;
+      }
     }
   }
   if (!pick) return LSB_EINVAL;
@@ -418,12 +426,13 @@ const char* lsb_strerror(int code) {
 
 int lsb_last_hip_error(void) { return g_last_hip; }
 
-int lsb_variants(uint32_t* digit_bits, uint32_t* tiles, int cap) {
+int lsb_variants(uint32_t* digit_bits, uint32_t* tiles, uint32_t* persistent, int cap) {
   int nv = 0;
   const lsb::SortVariant* vs = lsb::sort_variants(&nv);
   for (int i = 0; i < nv && i < cap; ++i) {
     if (digit_bits) digit_bits[i] = vs[i].digit_bits;
     if (tiles) tiles[i] = vs[i].tile;
+    if (persistent) persistent[i] = vs[i].persistent;
   }
   return nv;
 }
@@ -463,7 +472,7 @@ int lsb_sort(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg) {
   // Arrays that fit one workgroup are sorted in LDS by a single launch
   // (auto configuration only; an explicit digit width or tile, or
   // profiling, runs the pass pipeline).
-  if (c.digit_bits == 0 && c.tile == 0 && !(c.flags & LSB_PROFILE) &&
+  if (c.digit_bits == 0 && c.tile == 0 && c.variant == 0 && !(c.flags & LSB_PROFILE) &&
       n <= lsb::small_sort_capacity()) {
     LSB_HIP(lsb::launch_small_sort(A, (uint32_t)n, p.key_bits, s));
     if (c.flags & LSB_SYNC) LSB_HIP(hipStreamSynchronize(s));

@@ -98,20 +98,45 @@ struct TileSmem {
   uint32_t scalar[4];
 };
 
+// ------------------------------------------------------------- tile_load
+// Loads tile `tile` of an array of n elements into registers in the
+// wave-contiguous layout; slots past the end read as zero.
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ void tile_load(elem_t (&e)[ITEMS], const elem_t* __restrict__ src,
+                                          uint32_t n, uint32_t tile) {
+  constexpr uint32_t T = BLOCK * ITEMS;
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t tile_base = tile * T;
+  const uint32_t tile_n = n - tile_base < T ? n - tile_base : T;
+  const uint32_t wrun = wave * (kWave * ITEMS);
+  const uint32_t lim = tile_n > wrun ? tile_n - wrun : 0u;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    if ((uint32_t)i * kWave + lane < lim) {
+      e[i] = src[(size_t)tile_base + wrun + (uint32_t)i * kWave + lane];
+    } else {
+      e[i].key = 0;
+      e[i].val = 0;
+    }
+  }
+}
+
 // --------------------------------------------------------- onesweep_tile
-// One tile of one pass over an array (or one segment) of n elements:
-// src/dst point at element 0 of that array; `tile` counts from its first
-// tile; st_cur/st_next are the array's rows of the two look-back slices;
-// starts[b] is the global exclusive start of bucket b for this digit.
-// Must be entered by the whole block after a barrier.
+// One tile of one pass over an array (or one segment) of n elements whose
+// elements are already in registers (tile_load): dst points at element 0 of
+// that array; `tile` counts from its first tile; st_cur/st_next are the
+// array's rows of the two look-back slices; starts[b] is the global
+// exclusive start of bucket b for this digit.  Must be entered by the whole
+// block after a barrier that ends every earlier use of sm.
 template <int D, int BLOCK, int ITEMS>
-__device__ __forceinline__ void onesweep_tile(TileSmem<D, BLOCK, ITEMS>& sm,
-                                              const elem_t* __restrict__ src,
-                                              elem_t* __restrict__ dst, uint32_t n,
-                                              uint32_t tile, uint32_t* st_cur,
-                                              uint32_t* st_next,
-                                              const uint32_t* __restrict__ starts,
-                                              uint32_t shift, uint32_t mask) {
+__device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm,
+                                                   const elem_t (&e)[ITEMS],
+                                                   elem_t* __restrict__ dst, uint32_t n,
+                                                   uint32_t tile, uint32_t* st_cur,
+                                                   uint32_t* st_next,
+                                                   const uint32_t* __restrict__ starts,
+                                                   uint32_t shift, uint32_t mask) {
   using S = TileSmem<D, BLOCK, ITEMS>;
   constexpr int NB = S::NB, W = S::W, T = S::T;
   constexpr int DPT = (NB + BLOCK - 1) / BLOCK;  // digits per thread
@@ -125,18 +150,8 @@ __device__ __forceinline__ void onesweep_tile(TileSmem<D, BLOCK, ITEMS>& sm,
 
   const uint32_t tile_base = tile * (uint32_t)T;
   const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
-  const uint32_t wrun = wave * (kWave * ITEMS);  // this wave's first slot in the tile
+  const uint32_t wrun = wave * (kWave * ITEMS);
   const uint32_t lim = tile_n > wrun ? tile_n - wrun : 0u;
-  elem_t e[ITEMS];
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    if ((uint32_t)i * kWave + lane < lim) {
-      e[i] = src[(size_t)tile_base + wrun + (uint32_t)i * kWave + lane];
-    } else {
-      e[i].key = 0;
-      e[i].val = 0;
-    }
-  }
   __syncthreads();
 
   uint32_t dg[ITEMS], rk[ITEMS];
@@ -199,6 +214,20 @@ __device__ __forceinline__ void onesweep_tile(TileSmem<D, BLOCK, ITEMS>& sm,
       dst[(size_t)(uint32_t)(sm.gbase[d] + j)] = x;
     }
   }
+}
+
+// Loads and processes one tile (the whole block, after a barrier).
+template <int D, int BLOCK, int ITEMS>
+__device__ __forceinline__ void onesweep_tile(TileSmem<D, BLOCK, ITEMS>& sm,
+                                              const elem_t* __restrict__ src,
+                                              elem_t* __restrict__ dst, uint32_t n,
+                                              uint32_t tile, uint32_t* st_cur,
+                                              uint32_t* st_next,
+                                              const uint32_t* __restrict__ starts,
+                                              uint32_t shift, uint32_t mask) {
+  elem_t e[ITEMS];
+  tile_load<BLOCK, ITEMS>(e, src, n, tile);
+  onesweep_tile_regs<D, BLOCK, ITEMS>(sm, e, dst, n, tile, st_cur, st_next, starts, shift, mask);
 }
 
 // ----------------------------------------------------------- small_sort

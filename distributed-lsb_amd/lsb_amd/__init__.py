@@ -37,7 +37,7 @@ class LsbError(RuntimeError):
 class Config(ctypes.Structure):
     """struct lsb_config."""
     _fields_ = [("key_bits", ctypes.c_uint32), ("digit_bits", ctypes.c_uint32),
-                ("tile", ctypes.c_uint32), ("reserved0", ctypes.c_uint32),
+                ("tile", ctypes.c_uint32), ("variant", ctypes.c_uint32),
                 ("stream", ctypes.c_void_p), ("workspace", ctypes.c_void_p),
                 ("workspace_bytes", ctypes.c_size_t), ("flags", ctypes.c_uint32),
                 ("reserved1", ctypes.c_uint32)]
@@ -89,7 +89,7 @@ SIGNATURES = {
     "lsb_workspace_size": (_SZ, [_SZ, ctypes.POINTER(Config)]),
     "lsb_plan": (_I, [_SZ, ctypes.POINTER(Config), ctypes.POINTER(_U32),
                       ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
-    "lsb_variants": (_I, [_P, _P, _I]),
+    "lsb_variants": (_I, [_P, _P, _P, _I]),
     "lsb_strerror": (ctypes.c_char_p, [_I]),
     "lsb_version": (ctypes.c_char_p, []),
     "lsb_last_hip_error": (_I, []),
@@ -145,20 +145,24 @@ def version():
 
 
 def variants():
-    n = lib().lsb_variants(None, None, 0)
+    """[(variant id, digit_bits, tile, persistent)] of the compiled scatter
+    kernels; pass the id as sort(..., variant=id)."""
+    n = lib().lsb_variants(None, None, None, 0)
     d = (ctypes.c_uint32 * n)()
     t = (ctypes.c_uint32 * n)()
-    lib().lsb_variants(d, t, n)
-    return [(d[i], t[i]) for i in range(n)]
+    p = (ctypes.c_uint32 * n)()
+    lib().lsb_variants(d, t, p, n)
+    return [(i + 1, d[i], t[i], p[i]) for i in range(n)]
 
 
 def make_config(key_bits=64, digit_bits=0, tile=0, stream=None, flags=0,
-                workspace=None, workspace_bytes=0):
+                workspace=None, workspace_bytes=0, variant=0):
     c = Config()
     lib().lsb_config_init(ctypes.byref(c))
     c.key_bits = key_bits
     c.digit_bits = digit_bits
     c.tile = tile
+    c.variant = variant
     c.stream = _stream_ptr(stream)
     c.flags = flags
     if workspace is not None:
@@ -196,26 +200,28 @@ def _nelem(x):
     raise TypeError("pass n explicitly for raw pointers")
 
 
-def plan(n, key_bits=64, digit_bits=0, tile=0):
-    c = make_config(key_bits, digit_bits, tile, stream=0)
+def plan(n, key_bits=64, digit_bits=0, tile=0, variant=0):
+    c = make_config(key_bits, digit_bits, tile, stream=0, variant=variant)
     d, t, p = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
     _check(lib().lsb_plan(n, ctypes.byref(c), ctypes.byref(d), ctypes.byref(t),
                           ctypes.byref(p)), "lsb_plan")
     return d.value, t.value, p.value
 
 
-def workspace_size(n, key_bits=64, digit_bits=0, tile=0):
-    c = make_config(key_bits, digit_bits, tile, stream=0)
+def workspace_size(n, key_bits=64, digit_bits=0, tile=0, variant=0):
+    c = make_config(key_bits, digit_bits, tile, stream=0, variant=variant)
     return lib().lsb_workspace_size(n, ctypes.byref(c))
 
 
 def sort(A, B, n=None, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,
-         no_skip=False, workspace=None, workspace_bytes=0, profile=False, graph=False):
+         no_skip=False, workspace=None, workspace_bytes=0, profile=False, graph=False,
+         variant=0):
     """mySort(A, B): stable sort of A[0:n] by key, B is scratch, result in A."""
     n = _nelem(A) if n is None else n
     flags = ((LSB_SYNC if sync else 0) | (LSB_NO_SKIP if no_skip else 0)
              | (LSB_PROFILE if profile else 0) | (LSB_GRAPH if graph else 0))
-    c = make_config(key_bits, digit_bits, tile, stream, flags, workspace, workspace_bytes)
+    c = make_config(key_bits, digit_bits, tile, stream, flags, workspace, workspace_bytes,
+                    variant)
     _check(lib().lsb_sort(_ptr(A), _ptr(B), n, ctypes.byref(c)), "lsb_sort")
 
 

@@ -62,7 +62,8 @@ typedef struct lsb_config {
                                  uses 16; 2^16 counters per wave do not fit 160 KiB LDS) */
   uint32_t tile;              /* elements per scatter tile: 0 = auto, else a compiled
                                  tile for the digit width (see lsb_variants) */
-  uint32_t reserved0;
+  uint32_t variant;           /* 0 = pick by digit_bits/tile; k > 0 = compiled variant k-1
+                                 of lsb_variants() exactly */
   void* stream;               /* hipStream_t to run on; NULL = default stream */
   void* workspace;            /* device scratch of >= lsb_workspace_size() bytes;
                                  NULL = library-managed, cached per (device, stream) */
@@ -89,9 +90,10 @@ size_t lsb_workspace_size(size_t n, const lsb_config* cfg);
 int lsb_plan(size_t n, const lsb_config* cfg, uint32_t* digit_bits, uint32_t* tile,
              uint32_t* passes);
 
-/* Compiled (digit_bits, tile) pairs: writes up to cap pairs and returns how
- * many exist. */
-int lsb_variants(uint32_t* digit_bits, uint32_t* tiles, int cap);
+/* Compiled scatter variants: digit width, tile, and whether the pass runs
+ * as a persistent grid that prefetches its next tile.  Writes up to cap
+ * entries (any pointer may be NULL) and returns how many exist. */
+int lsb_variants(uint32_t* digit_bits, uint32_t* tiles, uint32_t* persistent, int cap);
 
 const char* lsb_strerror(int code);
 const char* lsb_version(void);

@@ -72,9 +72,13 @@ def test_plan_and_workspace():
     d, tile, _ = lsb_amd.plan(1 << 24)
     assert w2 >= 2 * ((1 << 24) // tile) * (1 << d) * 4
     vs = lsb_amd.variants()
-    assert (8, 4096) in vs and any(d == 11 for d, _ in vs)
-    for d, t in vs:
+    assert any(d == 8 and t == 4096 for _, d, t, _ in vs) and any(d == 11 for _, d, _, _ in vs)
+    assert any(p for *_, p in vs)
+    for v, d, t, _ in vs:
         assert lsb_amd.plan(1 << 20, digit_bits=d, tile=t)[:2] == (d, t)
+        assert lsb_amd.plan(1 << 20, variant=v)[:2] == (d, t)
+    c = lsb_amd.make_config(stream=0, variant=len(vs) + 1)
+    assert lsb_amd.lib().lsb_plan(10, ctypes.byref(c), None, None, None) == lsb_amd.LSB_EINVAL
 
 
 def test_generate_and_verify_reject_bad_args():

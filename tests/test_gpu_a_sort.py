@@ -56,9 +56,9 @@ SIZES = [2, 3, 63, 64, 65, 1000, 2047, 2048, 2049, 4095, 4096, 4097, 6143, 6144,
 def test_sort_uniform_all_variants(n):
     a = O.gen_uniform(n, ranks=3)
     want = stable(a)
-    for d, tile in variants():
-        got = gpu_sort(a, digit_bits=d, tile=tile)
-        assert np.array_equal(got, want), (n, d, tile)
+    for v, d, tile, _ in variants():
+        got = gpu_sort(a, variant=v)
+        assert np.array_equal(got, want), (n, d, tile, v)
 
 
 def test_sort_default_and_empty():
@@ -79,12 +79,12 @@ def test_c1_reference_config_digest():
     torch.cuda.synchronize()
     host = from_dev(A)
     assert O.sha256(host) == "4a8c88b35e1bc097a11998afab7608c9c3f66c2d4dd56fb501233124a51d450d"
-    for d, tile in variants():
+    for v, d, tile, _ in variants():
         A2 = A.clone()
         B = torch.empty_like(A2)
-        lsb_amd.sort(A2, B, digit_bits=d, tile=tile, sync=True)
+        lsb_amd.sort(A2, B, variant=v, sync=True)
         assert O.sha256(from_dev(A2)) == \
-            "edbb62f7b024eaade41bc12278e7cb240d038889a9780f8913a336352530d0c9", (d, tile)
+            "edbb62f7b024eaade41bc12278e7cb240d038889a9780f8913a336352530d0c9", (d, tile, v)
 
 
 @pytest.mark.parametrize("kind,kw", [
@@ -152,9 +152,9 @@ def test_skew_and_stability(case):
     n = 200_003
     a = _skew_cases(n)[case]
     want = stable(a)
-    for d, tile in variants():
-        assert np.array_equal(gpu_sort(a, digit_bits=d, tile=tile), want), (case, d, tile)
-        assert np.array_equal(gpu_sort(a, digit_bits=d, tile=tile, no_skip=True), want)
+    for v, d, tile, _ in variants():
+        assert np.array_equal(gpu_sort(a, variant=v), want), (case, d, tile, v)
+        assert np.array_equal(gpu_sort(a, variant=v, no_skip=True), want)
 
 
 @pytest.mark.parametrize("key_bits", [1, 8, 16, 21, 32, 33, 40])
@@ -164,8 +164,8 @@ def test_key_bits_and_per_pass_prefix(key_bits):
     n = 150_001
     a = O.gen_uniform(n, ranks=2)
     want = stable(a, key_bits)
-    for d, tile in variants():
-        assert np.array_equal(gpu_sort(a, key_bits=key_bits, digit_bits=d, tile=tile), want)
+    for v, d, tile, _ in variants():
+        assert np.array_equal(gpu_sort(a, key_bits=key_bits, variant=v), want), (v, key_bits)
 
 
 def test_non_default_stream_and_caller_workspace():

@@ -54,7 +54,8 @@ MIXED = [0, 1, 2, 3, 64, 1000, 2048, 2049, 4096, 8191, 8192, 8193, 9000, 20000, 
 def test_segmented_mixed_sizes(kind):
     a, off = batch(MIXED, seed=3, kind=kind)
     want = expected(a, off)
-    for d, tile in lsb_amd.variants():
+    # tile widths compiled for the segmented passes
+    for d, tile in [(8, 4096), (8, 8192), (8, 2048), (11, 6144)]:
         A = to_dev(a)
         B = torch.empty_like(A)
         lsb_amd.sort_segmented(A, B, off, digit_bits=d, tile=tile, sync=True)
