@@ -548,31 +548,35 @@ int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, ui
   if (ws) {
     if (c.workspace_bytes < L.total || ((uintptr_t)ws & 255u)) return LSB_ENOMEM;
   } else {
-    if (W.cap_n < total_n || W.cap_segs < nseg || !W.ptr) {
-      const size_t cap_n = std::max<size_t>(total_n, W.cap_n + W.cap_n / 4);
-      const uint32_t cap_s = std::max<uint32_t>(nseg, W.cap_segs + W.cap_segs / 4);
-      SegLayout G;
-      if ((rc = make_seg_layout(cap_n, cap_s, c, &G))) return rc;
+    // The layout of a library workspace follows its capacity, not the batch,
+    // so a captured graph stays valid for every batch that fits.  It is
+    // regrown when the batch exceeds the capacity or the digit width / tile
+    // asks for a larger layout.
+    const size_t cap_n = std::max<size_t>(total_n, W.cap_n);
+    const uint32_t cap_s = std::max<uint32_t>(nseg, W.cap_segs);
+    if ((rc = make_seg_layout(cap_n, cap_s, c, &L))) return rc;
+    if (!W.ptr || cap_n > W.cap_n || cap_s > W.cap_segs || L.total > W.bytes) {
+      const size_t grow_n = cap_n > W.cap_n ? std::max<size_t>(cap_n, W.cap_n + W.cap_n / 4) : cap_n;
+      const uint32_t grow_s =
+          cap_s > W.cap_segs ? std::max<uint32_t>(cap_s, W.cap_segs + W.cap_segs / 4) : cap_s;
+      if ((rc = make_seg_layout(grow_n, grow_s, c, &L))) return rc;
       if (W.ptr) {
         LSB_HIP(hipStreamSynchronize(s));
         LSB_HIP(hipFree(W.ptr));
         W.ptr = nullptr;
+        W.bytes = 0;
       }
-      if (hipMalloc(&W.ptr, G.total) != hipSuccess) {
+      if (hipMalloc(&W.ptr, L.total) != hipSuccess) {
         (void)hipGetLastError();
         W.ptr = nullptr;
         W.cap_n = 0;
         W.cap_segs = 0;
         return LSB_ENOMEM;
       }
-      W.bytes = G.total;
-      W.cap_n = cap_n;
-      W.cap_segs = cap_s;
+      W.bytes = L.total;
+      W.cap_n = grow_n;
+      W.cap_segs = grow_s;
     }
-    // The layout of a library workspace follows its capacity, not the batch,
-    // so a captured graph stays valid for every batch that fits.
-    if ((rc = make_seg_layout(W.cap_n, W.cap_segs, c, &L))) return rc;
-    if (L.total > W.bytes) return LSB_ENOMEM;
     ws = static_cast<char*>(W.ptr);
   }
 
