@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <stdio.h>
 #include <string.h>
 #include <utility>
 #include <vector>
@@ -53,29 +54,49 @@ lsb_config defaults() {
   return c;
 }
 
-// Auto choice of digit width and tile.  d = 8 keeps the per-bucket runs of
-// a tile long (coalesced writes); d = 11 saves passes on wide keys.
-void auto_choice(size_t n, uint32_t key_bits, uint32_t* d, uint32_t* tile) {
-  (void)key_bits;
+// Tuned table (lsb_autotune / lsb_tune_load), guarded by g_tune_mu.
+std::mutex g_tune_mu;
+std::vector<lsb_tune_entry> g_tune;
+
+// Automatic variant: the tuned entry of the nearest size class with the
+// same key-width class (<= 32 bits or wider); without a table, 8-bit digits
+// in 4096-element tiles (long per-bucket runs, two blocks per CU).
+uint32_t auto_variant(size_t n, uint32_t key_bits) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  const bool narrow = key_bits <= 32;
+  uint32_t lg = 0;
+  while (lg < 63 && ((size_t)1 << (lg + 1)) <= n) ++lg;
+  const lsb_tune_entry* best = nullptr;
+  for (const lsb_tune_entry& e : g_tune) {
+    if ((e.key_bits <= 32) != narrow) continue;
+    const uint32_t dist = e.log2n > lg ? e.log2n - lg : lg - e.log2n;
+    const uint32_t bd = best ? (best->log2n > lg ? best->log2n - lg : lg - best->log2n) : ~0u;
+    if (dist < bd) best = &e;
+  }
+  return best ? best->variant : 0u;
+}
+
+void auto_choice(size_t n, uint32_t key_bits, uint32_t* d, uint32_t* tile, uint32_t* variant) {
+  *variant = auto_variant(n, key_bits);
   *d = 8;
-  *tile = n >= ((size_t)1 << 22) ? 256 * 16 : 256 * 16;
+  *tile = 256 * 16;
 }
 
 int make_plan(size_t n, const lsb_config& c, Plan* p) {
   p->key_bits = c.key_bits ? c.key_bits : 64;
   if (p->key_bits > 64) return LSB_EINVAL;
-  uint32_t d = c.digit_bits, tile = c.tile;
-  if (d == 0 && tile == 0 && c.variant == 0) {
-    auto_choice(n, p->key_bits, &d, &tile);
+  uint32_t d = c.digit_bits, tile = c.tile, variant = c.variant;
+  if (d == 0 && tile == 0 && variant == 0) {
+    auto_choice(n, p->key_bits, &d, &tile, &variant);
   } else if (d == 0) {
     d = 8;
   }
   int nv = 0;
   const lsb::SortVariant* vs = lsb::sort_variants(&nv);
   const lsb::SortVariant* pick = nullptr;
-  if (c.variant) {
-    if (c.variant > (uint32_t)nv) return LSB_EINVAL;
-    pick = &vs[c.variant - 1];
+  if (variant) {
+    if (variant > (uint32_t)nv) return LSB_EINVAL;
+    pick = &vs[variant - 1];
     if ((c.digit_bits && c.digit_bits != pick->digit_bits) || (c.tile && c.tile != pick->tile))
       return LSB_EINVAL;
     d = pick->digit_bits;
@@ -306,14 +327,33 @@ struct SegLayout {
   size_t hdr, meta, counts, plan, tickets, status, total;
 };
 
-int make_seg_layout(size_t cap_n, uint32_t cap_segs, const lsb_config& c, SegLayout* L) {
+int make_seg_layout(size_t cap_n, uint32_t cap_segs, const lsb_config& c, SegLayout* L,
+                    size_t typical_n = 0) {
   L->key_bits = c.key_bits ? c.key_bits : 64;
   if (L->key_bits > 64) return LSB_EINVAL;
-  const uint32_t d = c.digit_bits ? c.digit_bits : 8;
+  uint32_t d = c.digit_bits ? c.digit_bits : 8;
+  uint32_t want_tile = c.tile;
+  if (!c.digit_bits && !c.tile && typical_n) {
+    // follow the tuned plain-sort choice for segments of the typical size
+    const uint32_t v = auto_variant(typical_n, L->key_bits);
+    if (v) {
+      int nsv = 0;
+      const lsb::SortVariant* sv = lsb::sort_variants(&nsv);
+      if (v <= (uint32_t)nsv) {
+        d = sv[v - 1].digit_bits;
+        want_tile = sv[v - 1].tile;
+      }
+    }
+  }
   int nv = 0;
   const lsb::SegVariant* vs = lsb::seg_variants(&nv);
   L->v = nullptr;
-  for (int i = 0; i < nv; ++i)
+  for (int i = 0; i < nv && want_tile; ++i)
+    if (vs[i].digit_bits == d && vs[i].tile == want_tile) {
+      L->v = &vs[i];
+      break;
+    }
+  for (int i = 0; i < nv && !L->v; ++i)
     if (vs[i].digit_bits == d && (c.tile == 0 || vs[i].tile == c.tile)) {
       L->v = &vs[i];
       break;
@@ -521,6 +561,7 @@ int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, ui
   if (kb > 64) return LSB_EINVAL;
   const uint64_t total_n = seg_offsets[nseg];
   uint32_t n_large = 0, n_tiny = 0, n_small = 0;
+  std::vector<uint64_t> large_sizes;
   for (uint32_t i = 0; i < nseg; ++i) {
     if (seg_offsets[i + 1] < seg_offsets[i]) return LSB_EINVAL;
     const uint64_t n = seg_offsets[i + 1] - seg_offsets[i];
@@ -528,14 +569,22 @@ int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, ui
     if (n <= 1) continue;
     if (n <= lsb::seg_tiny_capacity()) ++n_tiny;
     else if (n <= lsb::seg_small_capacity()) ++n_small;
-    else ++n_large;
+    else { ++n_large; large_sizes.push_back(n); }
+  }
+  // The tile pipeline's variant follows the tuned choice for the median
+  // large segment.
+  size_t typical_n = 0;
+  if (!large_sizes.empty()) {
+    std::nth_element(large_sizes.begin(), large_sizes.begin() + large_sizes.size() / 2,
+                     large_sizes.end());
+    typical_n = large_sizes[large_sizes.size() / 2];
   }
   if (n_large + n_tiny + n_small == 0) return LSB_OK;
   if (!A || !B || (((uintptr_t)A | (uintptr_t)B) & 15u)) return LSB_EINVAL;
   if (overlaps(A, B, total_n * sizeof(lsb_elem))) return LSB_EINVAL;
   int rc;
   SegLayout L;
-  if ((rc = make_seg_layout(total_n, nseg, c, &L))) return rc;
+  if ((rc = make_seg_layout(total_n, nseg, c, &L, typical_n))) return rc;
   const size_t bytes = total_n * sizeof(lsb_elem);
   if ((rc = check_device_range(A, bytes)) || (rc = check_device_range(B, bytes))) return rc;
   hipStream_t s = (hipStream_t)c.stream;
@@ -554,12 +603,12 @@ int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, ui
     // asks for a larger layout.
     const size_t cap_n = std::max<size_t>(total_n, W.cap_n);
     const uint32_t cap_s = std::max<uint32_t>(nseg, W.cap_segs);
-    if ((rc = make_seg_layout(cap_n, cap_s, c, &L))) return rc;
+    if ((rc = make_seg_layout(cap_n, cap_s, c, &L, typical_n))) return rc;
     if (!W.ptr || cap_n > W.cap_n || cap_s > W.cap_segs || L.total > W.bytes) {
       const size_t grow_n = cap_n > W.cap_n ? std::max<size_t>(cap_n, W.cap_n + W.cap_n / 4) : cap_n;
       const uint32_t grow_s =
           cap_s > W.cap_segs ? std::max<uint32_t>(cap_s, W.cap_segs + W.cap_segs / 4) : cap_s;
-      if ((rc = make_seg_layout(grow_n, grow_s, c, &L))) return rc;
+      if ((rc = make_seg_layout(grow_n, grow_s, c, &L, typical_n))) return rc;
       if (W.ptr) {
         LSB_HIP(hipStreamSynchronize(s));
         LSB_HIP(hipFree(W.ptr));
@@ -632,6 +681,157 @@ int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, ui
   }
   if (rc) return rc;
   if (c.flags & LSB_SYNC) LSB_HIP(hipStreamSynchronize(s));
+  return LSB_OK;
+}
+
+// ------------------------------------------------------------- autotuner
+
+int lsb_autotune(uint32_t lo_log2, uint32_t hi_log2, uint32_t key_bits, void* stream) {
+  if (lo_log2 < 10 || hi_log2 < lo_log2 || hi_log2 > 30 || key_bits > 64) return LSB_EINVAL;
+  const uint32_t kb = key_bits ? key_bits : 64;
+  const size_t nmax = (size_t)1 << hi_log2;
+  hipStream_t s = (hipStream_t)stream;
+  lsb_elem *pristine = nullptr, *A = nullptr, *B = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = LSB_OK;
+  auto cleanup = [&]() {
+    if (pristine) (void)hipFree(pristine);
+    if (A) (void)hipFree(A);
+    if (B) (void)hipFree(B);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipGetLastError();
+  };
+  if (hipMalloc(&pristine, nmax * sizeof(lsb_elem)) != hipSuccess ||
+      hipMalloc(&A, nmax * sizeof(lsb_elem)) != hipSuccess ||
+      hipMalloc(&B, nmax * sizeof(lsb_elem)) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    cleanup();
+    return LSB_ENOMEM;
+  }
+  lsb_gen_spec g;
+  memset(&g, 0, sizeof g);
+  g.kind = LSB_GEN_UNIFORM;
+  g.ranks = 64;
+  g.key_bits = kb;
+  g.seed_base = 0x5eed;
+  if ((rc = lsb_generate(pristine, nmax, &g, stream))) {
+    cleanup();
+    return rc;
+  }
+  int nv = 0;
+  const lsb::SortVariant* vs = lsb::sort_variants(&nv);
+  std::vector<lsb_tune_entry> table;
+  for (uint32_t lg = lo_log2; lg <= hi_log2 && rc == LSB_OK; ++lg) {
+    const size_t n = (size_t)1 << lg;
+    lsb_tune_entry best;
+    memset(&best, 0, sizeof best);
+    best.ms = 1e30f;
+    best.runner_up_ms = 1e30f;
+    for (int v = 0; v < nv && rc == LSB_OK; ++v) {
+      lsb_config c = defaults();
+      c.key_bits = kb;
+      c.variant = (uint32_t)v + 1;
+      c.stream = stream;
+      std::vector<float> ts;
+      for (int rep = 0; rep < 4 && rc == LSB_OK; ++rep) {  // first run warms up
+        if (hipMemcpyAsync(A, pristine, n * sizeof(lsb_elem), hipMemcpyDeviceToDevice, s) !=
+                hipSuccess ||
+            hipEventRecord(e0, s) != hipSuccess) {
+          rc = LSB_EHIP;
+          break;
+        }
+        if ((rc = lsb_sort(A, B, n, &c))) break;
+        float ms = 0;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          rc = LSB_EHIP;
+          break;
+        }
+        if (rep) ts.push_back(ms);
+      }
+      if (rc) break;
+      std::sort(ts.begin(), ts.end());
+      const float med = ts[ts.size() / 2];
+      if (med < best.ms) {
+        best.runner_up_ms = best.ms;
+        best.ms = med;
+        best.variant = (uint32_t)v + 1;
+        best.digit_bits = vs[v].digit_bits;
+        best.tile = vs[v].tile;
+        best.persistent = vs[v].persistent;
+      } else if (med < best.runner_up_ms) {
+        best.runner_up_ms = med;
+      }
+    }
+    best.log2n = lg;
+    best.key_bits = kb;
+    table.push_back(best);
+  }
+  cleanup();
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  // replace entries of the same key-width class and sizes
+  const bool narrow = kb <= 32;
+  std::vector<lsb_tune_entry> keep;
+  for (const lsb_tune_entry& e : g_tune)
+    if ((e.key_bits <= 32) != narrow || e.log2n < lo_log2 || e.log2n > hi_log2) keep.push_back(e);
+  for (const lsb_tune_entry& e : table) keep.push_back(e);
+  g_tune.swap(keep);
+  return LSB_OK;
+}
+
+int lsb_tune_table(lsb_tune_entry* out, int cap) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  for (int i = 0; i < (int)g_tune.size() && i < cap; ++i) out[i] = g_tune[i];
+  return (int)g_tune.size();
+}
+
+int lsb_tune_clear(void) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune.clear();
+  return LSB_OK;
+}
+
+int lsb_tune_save(const char* path) {
+  if (!path) return LSB_EINVAL;
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  FILE* f = fopen(path, "w");
+  if (!f) return LSB_EINVAL;
+  fprintf(f, "# lsb tune table: log2n key_bits digit_bits tile persistent ms runner_up_ms\n");
+  for (const lsb_tune_entry& e : g_tune)
+    fprintf(f, "%u %u %u %u %u %.6f %.6f\n", e.log2n, e.key_bits, e.digit_bits, e.tile,
+            e.persistent, e.ms, e.runner_up_ms);
+  fclose(f);
+  return LSB_OK;
+}
+
+// Entries name variants by (digit_bits, tile, persistent), so a table stays
+// valid if the compiled variant list is reordered; unknown ones are dropped.
+int lsb_tune_load(const char* path) {
+  if (!path) return LSB_EINVAL;
+  FILE* f = fopen(path, "r");
+  if (!f) return LSB_EINVAL;
+  int nv = 0;
+  const lsb::SortVariant* vs = lsb::sort_variants(&nv);
+  std::vector<lsb_tune_entry> t;
+  char line[256];
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    lsb_tune_entry e;
+    memset(&e, 0, sizeof e);
+    if (sscanf(line, "%u %u %u %u %u %f %f", &e.log2n, &e.key_bits, &e.digit_bits, &e.tile,
+               &e.persistent, &e.ms, &e.runner_up_ms) != 7)
+      continue;
+    for (int v = 0; v < nv; ++v)
+      if (vs[v].digit_bits == e.digit_bits && vs[v].tile == e.tile &&
+          vs[v].persistent == e.persistent)
+        e.variant = (uint32_t)v + 1;
+    if (e.variant) t.push_back(e);
+  }
+  fclose(f);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune.swap(t);
   return LSB_OK;
 }
 

@@ -78,6 +78,14 @@ class ServerStats(ctypes.Structure):
                 ("span_s", ctypes.c_double)]
 
 
+class TuneEntry(ctypes.Structure):
+    """struct lsb_tune_entry."""
+    _fields_ = [("log2n", ctypes.c_uint32), ("key_bits", ctypes.c_uint32),
+                ("digit_bits", ctypes.c_uint32), ("tile", ctypes.c_uint32),
+                ("persistent", ctypes.c_uint32), ("variant", ctypes.c_uint32),
+                ("ms", ctypes.c_float), ("runner_up_ms", ctypes.c_float)]
+
+
 _lib = None
 
 # Every function include/lsb.h declares, with its ctypes signature.
@@ -105,6 +113,11 @@ SIGNATURES = {
     "lsb_server_latency": (_I, [_P, _U64, ctypes.POINTER(ctypes.c_double)]),
     "lsb_server_stats_get": (_I, [_P, ctypes.POINTER(ServerStats)]),
     "lsb_server_reset_stats": (_I, [_P]),
+    "lsb_autotune": (_I, [_U32, _U32, _U32, _P]),
+    "lsb_tune_table": (_I, [ctypes.POINTER(TuneEntry), _I]),
+    "lsb_tune_save": (_I, [ctypes.c_char_p]),
+    "lsb_tune_load": (_I, [ctypes.c_char_p]),
+    "lsb_tune_clear": (_I, []),
     "lsb_profile_read": (_I, [ctypes.POINTER(Profile)]),
     "lsb_debug_histogram": (_I, [_P, _SZ, ctypes.POINTER(Config), _P]),
     "lsb_generate": (_I, [_P, _SZ, ctypes.POINTER(GenSpec), _P]),
@@ -244,6 +257,32 @@ def sort_segmented(A, B, offsets, key_bits=64, digit_bits=0, tile=0, stream=None
 def segmented_workspace_size(total_n, nseg, key_bits=64, digit_bits=0, tile=0):
     c = make_config(key_bits, digit_bits, tile, stream=0)
     return lib().lsb_segmented_workspace_size(total_n, nseg, ctypes.byref(c))
+
+
+def autotune(lo_log2=12, hi_log2=26, key_bits=64, stream=None):
+    """Times every compiled variant per size class and makes the fastest the
+    automatic choice; returns the resulting table."""
+    _check(lib().lsb_autotune(lo_log2, hi_log2, key_bits, _stream_ptr(stream)), "lsb_autotune")
+    return tune_table()
+
+
+def tune_table():
+    n = lib().lsb_tune_table(None, 0)
+    arr = (TuneEntry * max(n, 1))()
+    lib().lsb_tune_table(arr, n)
+    return [{f: getattr(arr[i], f) for f, _ in TuneEntry._fields_} for i in range(n)]
+
+
+def tune_save(path):
+    _check(lib().lsb_tune_save(path.encode()), "lsb_tune_save")
+
+
+def tune_load(path):
+    _check(lib().lsb_tune_load(path.encode()), "lsb_tune_load")
+
+
+def tune_clear():
+    _check(lib().lsb_tune_clear(), "lsb_tune_clear")
 
 
 def profile_read():

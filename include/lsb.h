@@ -95,6 +95,34 @@ int lsb_plan(size_t n, const lsb_config* cfg, uint32_t* digit_bits, uint32_t* ti
  * entries (any pointer may be NULL) and returns how many exist. */
 int lsb_variants(uint32_t* digit_bits, uint32_t* tiles, uint32_t* persistent, int cap);
 
+/* ------------------------------------------------------------ autotuner */
+
+/* One tuned size class: the fastest compiled scatter variant for arrays of
+ * about 2^log2n elements whose keys are at most key_bits wide. */
+typedef struct lsb_tune_entry {
+  uint32_t log2n;
+  uint32_t key_bits;
+  uint32_t digit_bits;  /* of the chosen variant */
+  uint32_t tile;
+  uint32_t persistent;
+  uint32_t variant;     /* 1-based id into lsb_variants() */
+  float ms;             /* its median sort time */
+  float runner_up_ms;   /* the next-best variant's time */
+} lsb_tune_entry;
+
+/* Times every compiled variant with HIP events on uniform keys of width
+ * key_bits at each size class 2^lo_log2 .. 2^hi_log2, keeps the fastest per
+ * class, and makes the table the automatic choice of lsb_sort and
+ * lsb_sort_segmented (digit_bits = tile = variant = 0).  Needs 48 bytes x
+ * 2^hi_log2 of free device memory while it runs.  Synchronous. */
+int lsb_autotune(uint32_t lo_log2, uint32_t hi_log2, uint32_t key_bits, void* stream);
+/* Copies up to cap entries of the current table; returns the entry count. */
+int lsb_tune_table(lsb_tune_entry* out, int cap);
+/* Saves / loads / clears the table (text file, one size class per line). */
+int lsb_tune_save(const char* path);
+int lsb_tune_load(const char* path);
+int lsb_tune_clear(void);
+
 const char* lsb_strerror(int code);
 const char* lsb_version(void);
 /* hipError_t of the last failed HIP call on this thread (0 if none). */

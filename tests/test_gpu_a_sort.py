@@ -263,3 +263,19 @@ def test_full_2p30_uniform():
     assert _device_check(A, keys, n) == (0, -1)
     del A, keys
     torch.cuda.empty_cache()
+
+
+def test_autotune_table_and_sorts_after_tuning():
+    """The autotuner times every variant per class; sorts that follow use the
+    tuned choice and stay bit-exact."""
+    try:
+        table = lsb_amd.autotune(12, 18, key_bits=64)
+        assert [e["log2n"] for e in table] == list(range(12, 19))
+        ids = {v for v, *_ in variants()}
+        for e in table:
+            assert e["variant"] in ids and 0 < e["ms"] <= e["runner_up_ms"]
+        for n in (5000, 70000, 250001):
+            a = O.gen_uniform(n, ranks=2, seed_base=n)
+            assert np.array_equal(gpu_sort(a), stable(a))
+    finally:
+        lsb_amd.tune_clear()
