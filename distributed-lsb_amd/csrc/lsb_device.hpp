@@ -89,17 +89,15 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t key, uint32_t shift, uint3
   return (uint32_t)(key >> shift) & mask;
 }
 
-// Adds one count for digit `d` of every valid lane.  When every valid lane
-// of the wave holds the same digit (sorted keys, zero high digits, heavy
-// duplicates) a single lane adds the whole count, which avoids a 64-way
-// same-address LDS atomic.
+// Adds one count for digit `d` of every valid lane.  When every active lane
+// is valid and holds the same digit (sorted keys, zero high digits, heavy
+// duplicates) one lane adds the whole count, which avoids a 64-way
+// same-address LDS atomic; the test is one compare and one ballot.
 __device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d, bool valid) {
-  const uint64_t act = __ballot(valid);
-  if (act == 0) return;
   const uint32_t first = __builtin_amdgcn_readfirstlane(d);
-  const uint64_t same = __ballot(valid && d == first);
-  if (same == act) {
-    if (valid && count_below(act) == 0) atomicAdd(&h[first], (uint32_t)__popcll(act));
+  if (__ballot(!valid || d != first) == 0) {
+    if (lane_id() == (uint32_t)__builtin_ctzll(__ballot(1)))
+      atomicAdd(&h[first], (uint32_t)__popcll(__ballot(1)));
   } else if (valid) {
     atomicAdd(&h[d], 1u);
   }

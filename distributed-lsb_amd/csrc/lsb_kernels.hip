@@ -261,6 +261,7 @@ static void launch_onesweep(const PassArgs& a, hipStream_t s) {
 // first one per digit width is the default for that width.
 static const SortVariant kVariants[] = {
     {8, 256 * 16, 256, 0, &launch_onesweep<8, 256, 16>},
+    {8, 256 * 12, 256, 0, &launch_onesweep<8, 256, 12>},
     {8, 512 * 16, 512, 0, &launch_onesweep<8, 512, 16>},
     {8, 256 * 8, 256, 0, &launch_onesweep<8, 256, 8>},
     {11, 512 * 12, 512, 0, &launch_onesweep<11, 512, 12>},

@@ -53,15 +53,31 @@ __device__ __forceinline__ void wave_rank(const elem_t (&e)[ITEMS], uint32_t lim
 // tile 0 read as "not ready" and are never consumed.
 constexpr int kLookback = 4;
 
-__device__ __forceinline__ uint32_t lookback(const uint32_t* __restrict__ st, uint32_t tile,
-                                             uint32_t nb, uint32_t b) {
+// Issues the first round of a look-back (tiles tile-1 .. tile-kLookback)
+// without waiting for it, so the caller can do other work meanwhile.
+__device__ __forceinline__ void lookback_issue(const uint32_t* __restrict__ st, uint32_t tile,
+                                               uint32_t nb, uint32_t b,
+                                               uint32_t (&w)[kLookback]) {
+  const int64_t j = (int64_t)tile - 1;
+#pragma unroll
+  for (int k = 0; k < kLookback; ++k)
+    w[k] = (j - k >= 0) ? ld_status(st + (size_t)(j - k) * nb + b) : 0u;
+}
+
+// Finishes a look-back whose first round lookback_issue read into w.
+__device__ __forceinline__ uint32_t lookback_finish(const uint32_t* __restrict__ st,
+                                                    uint32_t tile, uint32_t nb, uint32_t b,
+                                                    uint32_t (&w)[kLookback]) {
   uint32_t prefix = 0;
   int64_t j = (int64_t)tile - 1;
+  bool first = true;
   while (true) {
-    uint32_t w[kLookback];
+    if (!first) {
 #pragma unroll
-    for (int k = 0; k < kLookback; ++k)
-      w[k] = (j - k >= 0) ? ld_status(st + (size_t)(j - k) * nb + b) : 0u;
+      for (int k = 0; k < kLookback; ++k)
+        w[k] = (j - k >= 0) ? ld_status(st + (size_t)(j - k) * nb + b) : 0u;
+    }
+    first = false;
     uint32_t used = 0;
     bool stop = false, done = false;
 #pragma unroll
@@ -80,6 +96,13 @@ __device__ __forceinline__ uint32_t lookback(const uint32_t* __restrict__ st, ui
     j -= used;
     if (used == 0) __builtin_amdgcn_s_sleep(1);
   }
+}
+
+__device__ __forceinline__ uint32_t lookback(const uint32_t* __restrict__ st, uint32_t tile,
+                                             uint32_t nb, uint32_t b) {
+  uint32_t w[kLookback];
+  lookback_issue(st, tile, nb, b, w);
+  return lookback_finish(st, tile, nb, b, w);
 }
 
 // Shared memory of one tile pass.  The element buffer first holds the
@@ -160,13 +183,18 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
   __syncthreads();
 
   // Per digit: exclusive prefix over waves (in place) and the tile total,
-  // published at once as this tile's aggregate; then the look-back, so the
-  // inclusive prefix is out early for the tiles behind this one.
-  uint32_t gstart[DPT];
+  // published at once as this tile's aggregate.  The first look-back round
+  // is issued right away and consumed only after the tile-local scan and
+  // the reorder through LDS, so its memory latency hides under that work.
+  uint32_t tot[DPT], gstart[DPT];
+  uint32_t lbw[DPT][kLookback];
 #pragma unroll
   for (int k = 0; k < DPT; ++k) {
     const uint32_t b = t + k * BLOCK;
+    tot[k] = 0;
+    gstart[k] = 0;
     if (b < (uint32_t)NB) {
+      gstart[k] = starts[b];
       uint32_t s = 0;
 #pragma unroll
       for (int w = 0; w < W; ++w) {
@@ -174,25 +202,18 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
         whist[w * NB + b] = s;
         s += c;
       }
+      tot[k] = s;
       sm.start[b] = s;
       uint32_t* mine = st_cur + (size_t)tile * NB + b;
-      uint32_t prefix = 0;
       if (tile == 0) {
         st_status(mine, (s + 1u) | kFlagInclusive);
       } else {
         st_status(mine, s + 1u);
-        prefix = lookback(st_cur, tile, NB, b);
-        st_status(mine, (prefix + s + 1u) | kFlagInclusive);
+        lookback_issue(st_cur, tile, NB, b, lbw[k]);
       }
-      gstart[k] = starts[b] + prefix;
     }
   }
   block_exclusive_scan<NB, BLOCK>(sm.start, sm.wsum);
-#pragma unroll
-  for (int k = 0; k < DPT; ++k) {
-    const uint32_t b = t + k * BLOCK;
-    if (b < (uint32_t)NB) sm.gbase[b] = gstart[k] - sm.start[b];
-  }
 
   uint32_t pos[ITEMS];
 #pragma unroll
@@ -201,6 +222,19 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
     if ((uint32_t)i * kWave + lane < lim) sm.elems[pos[i]] = e[i];
+
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const uint32_t b = t + k * BLOCK;
+    if (b < (uint32_t)NB) {
+      uint32_t prefix = 0;
+      if (tile > 0) {
+        prefix = lookback_finish(st_cur, tile, NB, b, lbw[k]);
+        st_status(st_cur + (size_t)tile * NB + b, (prefix + tot[k] + 1u) | kFlagInclusive);
+      }
+      sm.gbase[b] = gstart[k] + prefix - sm.start[b];
+    }
+  }
   __syncthreads();
 
   // Write out in digit order: consecutive lanes write consecutive slots of
