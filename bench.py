@@ -90,6 +90,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log2n", type=int, default=26)
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-segments", type=int, default=4096)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-jobs", type=int, default=96)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -195,7 +199,163 @@ def main():
         "verified_bit_exact": verified,
         "cpu_baseline": base,
     }
+    del pristine, A, B
+    torch.cuda.empty_cache()
+    if not args.no_batched:
+        log("batched segmented sort ...")
+        out["batched"] = bench_batched(args.batched_segments)
+        torch.cuda.empty_cache()
+    if not args.no_serving:
+        log("serving stream ...")
+        out["serving"] = bench_serving(args.serving_jobs)
+        torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
+
+
+def _timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def bench_batched(nseg, seed=2024, reps=3):
+    """Batched segmented sort: nseg independent arrays, sizes log-uniform over
+    [2^10, 2^22], uniform 64-bit keys, val = index within the segment; one
+    segmented launch sequence (replayed from a hipGraph) against one plain
+    lsb_sort launch per segment, both on HBM-resident data."""
+    import torch
+    import lsb_amd
+    from lsb_amd import workloads as W
+    sizes = W.batched_sizes(nseg, seed)
+    off = [0]
+    for n in sizes:
+        off.append(off[-1] + n)
+    total = off[-1]
+    dev = torch.device("cuda:0")
+    A = torch.empty((total, 2), dtype=torch.int64, device=dev)
+    for s, n in enumerate(sizes):
+        lsb_amd.generate(A[off[s]:off[s] + n], kind=lsb_amd.GEN_UNIFORM, ranks=1,
+                         seed_base=1000 + s, val_base=0)
+    pristine = A.clone()
+    B = torch.empty_like(A)
+    ts = []
+    for r in range(reps + 1):
+        A.copy_(pristine)
+        t = _timed(lambda: lsb_amd.sort_segmented(A, B, off, graph=True))
+        if r:
+            ts.append(t)
+    t_batched = sorted(ts)[len(ts) // 2]
+    # spot-check segments (the test suite checks all of them)
+    ok = True
+    keys = torch.empty(max(sizes), dtype=torch.int64, device=dev)
+    for s in list(range(0, nseg, max(1, nseg // 64))):
+        lo, n = off[s], sizes[s]
+        lsb_amd.extract_keys(pristine[lo:lo + n], keys, n=n)
+        bad, _ = lsb_amd.verify(A[lo:lo + n], keys, n=n)
+        ok &= bad == 0
+    del keys
+
+    def plain():
+        for s, n in enumerate(sizes):
+            lsb_amd.sort(A[off[s]:off[s] + n], B[off[s]:off[s] + n])
+    tp = []
+    for r in range(2):
+        A.copy_(pristine)
+        tp.append(_timed(plain))
+    t_plain = min(tp)
+    del A, B, pristine
+    return {"workload": f"{nseg} segments, sizes log-uniform [2^10, 2^22], uniform 64-bit keys",
+            "elements": total, "ms": round(t_batched * 1e3, 3),
+            "melem_s": round(total / t_batched / 1e6, 1),
+            "plain_per_segment_ms": round(t_plain * 1e3, 3),
+            "plain_per_segment_melem_s": round(total / t_plain / 1e6, 1),
+            "speedup_vs_plain": round(t_plain / t_batched, 3), "verified_sample": ok}
+
+
+def bench_serving(jobs, seed=77):
+    """Serving stream: jobs of log-uniform size over [2^12, 2^28], 32- or
+    64-bit keys, uniform or Zipf(1.1), resident in HBM.  Three runs on the
+    same jobs: one plain lsb_sort per job in arrival order (each synchronised);
+    the server with every job submitted at once (capacity); and the server fed
+    by Poisson arrivals at 0.8 of that capacity (latency)."""
+    import torch
+    import lsb_amd
+    from lsb_amd import workloads as W
+    trace = W.serving_trace(jobs, seed)
+    dev = torch.device("cuda:0")
+    bufs, pris = [], []
+    for j, job in enumerate(trace):
+        a = torch.empty((job["n"], 2), dtype=torch.int64, device=dev)
+        lsb_amd.generate(a, kind=lsb_amd.GEN_ZIPF if job["zipf"] else lsb_amd.GEN_UNIFORM,
+                         ranks=4, key_bits=job["key_bits"], seed_base=7000 + 16 * j, val_base=0)
+        bufs.append(a)
+        pris.append(a.clone())
+    total = sum(job["n"] for job in trace)
+
+    def restore():
+        for a, p in zip(bufs, pris):
+            a.copy_(p)
+        torch.cuda.synchronize()
+
+    # one plain launch per job, in arrival order
+    scratch = torch.empty((max(j["n"] for j in trace), 2), dtype=torch.int64, device=dev)
+    restore()
+    t_plain = 0.0
+    for a, job in zip(bufs, trace):
+        t_plain += _timed(lambda: lsb_amd.sort(a, scratch, key_bits=job["key_bits"]))
+    del scratch
+    torch.cuda.empty_cache()
+    out = {"workload": f"{jobs} jobs, sizes log-uniform [2^12, 2^28], 32/64-bit keys, "
+                       "uniform/Zipf(1.1), HBM-resident",
+           "elements": total, "plain_melem_s": round(total / t_plain / 1e6, 1)}
+    with lsb_amd.Server() as srv:
+        # warm the server's streams, workspaces and graphs
+        restore()
+        for a, job in zip(bufs, trace):
+            srv.submit(a, key_bits=job["key_bits"])
+        srv.drain()
+        # capacity: everything arrives at once
+        restore()
+        srv.reset_stats()
+        for a, job in zip(bufs, trace):
+            srv.submit(a, key_bits=job["key_bits"])
+        srv.drain()
+        st = srv.stats()
+        cap_s = st["span_s"]
+        out.update({"saturated_melem_s": round(total / cap_s / 1e6, 1),
+                    "saturated_p50_ms": round(st["p50_us"] / 1e3, 3),
+                    "saturated_p99_ms": round(st["p99_us"] / 1e3, 3),
+                    "served_vs_plain": round((total / cap_s) / (total / t_plain), 3),
+                    "batches": st["batches"], "batched_jobs": st["batched_jobs"]})
+        # latency: Poisson arrivals at 0.8 of capacity
+        lam = 0.8 * jobs / cap_s
+        restore()
+        srv.reset_stats()
+        t0 = time.perf_counter()
+        at = 0.0
+        for a, job in zip(bufs, trace):
+            at += job["gap"] / lam
+            while time.perf_counter() - t0 < at:
+                pass
+            srv.submit(a, key_bits=job["key_bits"])
+        srv.drain()
+        st = srv.stats()
+        out.update({"poisson_rate_jobs_s": round(lam, 2),
+                    "poisson_melem_s": round(total / st["span_s"] / 1e6, 1),
+                    "p50_ms": round(st["p50_us"] / 1e3, 3), "p99_ms": round(st["p99_us"] / 1e3, 3)})
+    # every job's result is the stable sort of its input
+    ok = True
+    keys = torch.empty(max(j["n"] for j in trace), dtype=torch.int64, device=dev)
+    for a, p, job in zip(bufs, pris, trace):
+        lsb_amd.extract_keys(p, keys, n=job["n"])
+        bad, _ = lsb_amd.verify(a, keys, n=job["n"], key_bits=job["key_bits"])
+        ok &= bad == 0
+    out["verified_bit_exact"] = ok
+    del bufs, pris, keys
+    return out
 
 
 if __name__ == "__main__":

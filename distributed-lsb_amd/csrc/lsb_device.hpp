@@ -94,10 +94,13 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t key, uint32_t shift, uint3
 // duplicates) one lane adds the whole count, which avoids a 64-way
 // same-address LDS atomic; the test is one compare and one ballot.
 __device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d, bool valid) {
+  // The active mask is taken before branching: inside the one-lane branch a
+  // ballot would see that lane alone.
+  const uint64_t active = __ballot(1);
   const uint32_t first = __builtin_amdgcn_readfirstlane(d);
   if (__ballot(!valid || d != first) == 0) {
-    if (lane_id() == (uint32_t)__builtin_ctzll(__ballot(1)))
-      atomicAdd(&h[first], (uint32_t)__popcll(__ballot(1)));
+    if (lane_id() == (uint32_t)__builtin_ctzll(active))
+      atomicAdd(&h[first], (uint32_t)__popcll(active));
   } else if (valid) {
     atomicAdd(&h[d], 1u);
   }

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void k_generate(elem_t* __restrict__ A, uint64
         if (u < cdf[mid]) hi = mid;
         else lo = mid + 1;
       }
-      key = table[lo];
+      key = table[lo] & key_mask;
     } else {
       key &= key_mask;
     }

@@ -125,7 +125,7 @@ void orc_gen_ordered(orc_elem* out, uint64_t n, int reverse) {
  * summed in increasing k in double.  The rank of element g is the smallest
  * k with u < cdf[k], u = (x >> 11) * 2^-53 where x is the element's pcg64
  * draw laid out exactly like orc_gen_uniform; its key is output k of
- * pcg64(table_seed). */
+ * pcg64(table_seed), masked to the low key_bits. */
 void orc_zipf_cdf(double* cdf, uint32_t m, double s) {
   double acc = 0.0;
   for (uint32_t k = 0; k < m; k++) { acc += pow((double)(k + 1), -s); cdf[k] = acc; }
@@ -145,10 +145,11 @@ static uint32_t zipf_rank(const double* cdf, uint32_t m, uint64_t x) {
 
 void orc_gen_zipf(orc_elem* out, uint64_t n, uint32_t ranks, uint64_t seed_base,
                   const double* cdf, const uint64_t* table, uint32_t m,
-                  uint64_t first, uint64_t count) {
+                  uint64_t first, uint64_t count, uint32_t key_bits) {
+  uint64_t mk = key_mask(key_bits);
   orc_gen_uniform(out, n, ranks, 64, seed_base, first, count);
   for (uint64_t k = 0; k < count; k++)
-    out[k].key = table[zipf_rank(cdf, m, out[k].key)];
+    out[k].key = table[zipf_rank(cdf, m, out[k].key)] & mk;
 }
 
 /* ------------------------------------------------------- radix sort core */

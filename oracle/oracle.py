@@ -35,7 +35,7 @@ def lib():
         L.orc_gen_uniform.argtypes = [p, u64, u32, u32, u64, u64, u64]
         L.orc_gen_ordered.argtypes = [p, u64, ctypes.c_int]
         L.orc_zipf_cdf.argtypes = [p, u32, ctypes.c_double]
-        L.orc_gen_zipf.argtypes = [p, u64, u32, u64, p, p, u32, u64, u64]
+        L.orc_gen_zipf.argtypes = [p, u64, u32, u64, p, p, u32, u64, u64, u32]
         L.orc_histograms.argtypes = [p, u64, u32, u32, p]
         L.orc_counting_pass.argtypes = [p, p, u64, u32, u32, u32]
         L.orc_lsd_sort.argtypes = [p, p, u64, u32, u32]
@@ -78,13 +78,13 @@ def zipf_cdf(m, s):
 
 
 def gen_zipf(n, ranks=1, seed_base=0, m=1 << 20, s=1.1, table_seed=12345,
-             first=0, count=None):
+             first=0, count=None, key_bits=64):
     count = n - first if count is None else count
     cdf = zipf_cdf(m, s)
     table = pcg64(table_seed, m)
     out = np.empty(count, dtype=ELEM)
     lib().orc_gen_zipf(_ptr(out), n, ranks, seed_base, _ptr(cdf), _ptr(table),
-                       m, first, count)
+                       m, first, count, key_bits)
     return out
 
 

@@ -93,6 +93,7 @@ def test_c1_reference_config_digest():
     (lsb_amd.GEN_SORTED, {}),
     (lsb_amd.GEN_REVERSE, {}),
     (lsb_amd.GEN_ZIPF, dict(ranks=5, zipf_m=1 << 20)),
+    (lsb_amd.GEN_ZIPF, dict(ranks=3, zipf_m=1 << 16, key_bits=32)),
 ])
 def test_generator_matches_oracle(kind, kw):
     n = (1 << 18) + 12345
@@ -104,7 +105,7 @@ def test_generator_matches_oracle(kind, kw):
         want = O.gen_uniform(n, ranks=kw["ranks"], key_bits=kw.get("key_bits", 64),
                              seed_base=kw.get("seed_base", 0))
     elif kind == lsb_amd.GEN_ZIPF:
-        want = O.gen_zipf(n, ranks=kw["ranks"], m=kw["zipf_m"])
+        want = O.gen_zipf(n, ranks=kw["ranks"], m=kw["zipf_m"], key_bits=kw.get("key_bits", 64))
     else:
         want = O.gen_ordered(n, reverse=kind == lsb_amd.GEN_REVERSE)
     assert np.array_equal(got, want)

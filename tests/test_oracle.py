@@ -192,3 +192,17 @@ def test_reference_binary_agrees_with_oracle():
         idx = int(line[2:line.index("]")])
         key, val = line.split("(")[1].rstrip(")").split(",")
         assert (int(s[idx]["key"]), int(s[idx]["val"])) == (int(key, 16), int(val))
+
+
+def test_product_workload_definitions_match_oracle():
+    """The benchmark's own pcg64 restatement and size draws equal the
+    oracle's (they generate the batched and serving workloads)."""
+    from lsb_amd import workloads as W
+    for seed in (0, 3, 2024):
+        assert W.pcg64_stream(seed, 50) == [int(x) for x in O.pcg64(seed, 50)]
+    assert W.batched_sizes(4096, 2024) == [int(x) for x in O.c4_sizes(4096, 2024)]
+    t = W.serving_trace(200, seed=77)
+    sizes = [j["n"] for j in t]
+    assert min(sizes) >= 1 << 12 and max(sizes) < 1 << 28
+    assert 60 < sum(j["key_bits"] == 32 for j in t) < 140
+    assert 60 < sum(j["zipf"] for j in t) < 140
