@@ -32,7 +32,7 @@ struct SortVariant {
   uint32_t digit_bits;
   uint32_t tile;  // elements per tile (block * items)
   uint32_t block;
-  uint32_t persistent;  // 1 = persistent grid prefetching its next tile
+  uint32_t persistent;  // 1 = persistent grid prefetching its next tile, 2 = split staging
   ScatterLaunch scatter;
 };
 

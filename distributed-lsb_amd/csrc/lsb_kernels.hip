@@ -252,6 +252,35 @@ static void launch_onesweep_pf(const PassArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_onesweep_pf<D, BLOCK, ITEMS>), dim3(g), dim3(BLOCK), 0, s, a);
 }
 
+// Split-staging form (keys, then values, through an 8-byte buffer).
+template <int D, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_onesweep_split(PassArgs args) {
+  constexpr int NB = 1 << D;
+  __shared__ TileSmemSplit<D, BLOCK, ITEMS> sm;
+  const uint32_t* plan = args.plan;
+  const uint32_t pass = args.pass;
+  if (!plan[4 * pass]) return;
+  const uint32_t ord = plan[4 * pass + 1];
+  const bool src_b = plan[4 * pass + 2] != 0;
+  const elem_t* src = static_cast<const elem_t*>(src_b ? args.B : args.A);
+  elem_t* dst = static_cast<elem_t*>(src_b ? args.A : args.B);
+  const size_t slice = (size_t)args.ntiles * NB;
+  uint32_t* st_cur = args.status + (size_t)(ord & 1u) * slice;
+  uint32_t* st_next = args.status + (size_t)((ord + 1u) & 1u) * slice;
+  if (threadIdx.x == 0) sm.scalar[0] = atomicAdd(&args.tickets[pass], 1u);
+  __syncthreads();
+  const uint32_t tile = sm.scalar[0];
+  elem_t e[ITEMS];
+  tile_load<BLOCK, ITEMS>(e, src, args.n, tile);
+  onesweep_tile_split<D, BLOCK, ITEMS>(sm, e, dst, args.n, tile, st_cur, st_next, args.starts,
+                                       args.shift, args.mask);
+}
+
+template <int D, int BLOCK, int ITEMS>
+static void launch_onesweep_split(const PassArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_onesweep_split<D, BLOCK, ITEMS>), dim3(a.ntiles), dim3(BLOCK), 0, s, a);
+}
+
 template <int D, int BLOCK, int ITEMS>
 static void launch_onesweep(const PassArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_onesweep<D, BLOCK, ITEMS>), dim3(a.ntiles), dim3(BLOCK), 0, s, a);
@@ -265,6 +294,9 @@ static const SortVariant kVariants[] = {
     {8, 512 * 16, 512, 0, &launch_onesweep<8, 512, 16>},
     {8, 256 * 8, 256, 0, &launch_onesweep<8, 256, 8>},
     {11, 512 * 12, 512, 0, &launch_onesweep<11, 512, 12>},
+    {8, 256 * 16, 256, 2, &launch_onesweep_split<8, 256, 16>},
+    {8, 512 * 12, 512, 2, &launch_onesweep_split<8, 512, 12>},
+    {8, 256 * 12, 256, 2, &launch_onesweep_split<8, 256, 12>},
     {8, 256 * 12, 256, 1, &launch_onesweep_pf<8, 256, 12>},
     {8, 256 * 8, 256, 1, &launch_onesweep_pf<8, 256, 8>},
     {11, 512 * 8, 512, 1, &launch_onesweep_pf<11, 512, 8>},

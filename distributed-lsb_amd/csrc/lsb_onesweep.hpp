@@ -250,6 +250,132 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
   }
 }
 
+// ------------------------------------------------- split-staging variant
+// Same pass with half the LDS: the tile is reordered through an 8-byte-per-
+// element buffer twice, first the keys, then the values.  Each thread keeps
+// the destination of the slots it stores in registers between the halves,
+// so the value half needs no digit.  Halving the buffer lets more tiles be
+// resident per CU (each tile spends most of its life loading, ranking and
+// looking back, when the buffer is idle); the two 8-byte stores of an
+// element meet again in L2 before the line goes to HBM.
+template <int D, int BLOCK, int ITEMS>
+struct TileSmemSplit {
+  static constexpr int NB = 1 << D;
+  static constexpr int W = BLOCK / kWave;
+  static constexpr int T = BLOCK * ITEMS;
+  static_assert(W * NB * 4 <= T * 8, "per-wave counters must fit in the tile buffer");
+  uint64_t half[T];
+  uint32_t start[NB];
+  uint32_t gbase[NB];
+  uint32_t wsum[W];
+  uint32_t scalar[4];
+};
+
+template <int D, int BLOCK, int ITEMS>
+__device__ __forceinline__ void onesweep_tile_split(TileSmemSplit<D, BLOCK, ITEMS>& sm,
+                                                    const elem_t (&e)[ITEMS],
+                                                    elem_t* __restrict__ dst, uint32_t n,
+                                                    uint32_t tile, uint32_t* st_cur,
+                                                    uint32_t* st_next,
+                                                    const uint32_t* __restrict__ starts,
+                                                    uint32_t shift, uint32_t mask) {
+  using S = TileSmemSplit<D, BLOCK, ITEMS>;
+  constexpr int NB = S::NB, W = S::W, T = S::T;
+  constexpr int DPT = (NB + BLOCK - 1) / BLOCK;
+  const uint32_t t = threadIdx.x;
+  const uint32_t lane = t & (kWave - 1);
+  const uint32_t wave = t / kWave;
+  uint32_t* whist = reinterpret_cast<uint32_t*>(sm.half);
+
+  for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
+  for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) st_next[(size_t)tile * NB + b] = 0u;
+  const uint32_t tile_base = tile * (uint32_t)T;
+  const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
+  const uint32_t wrun = wave * (kWave * ITEMS);
+  const uint32_t lim = tile_n > wrun ? tile_n - wrun : 0u;
+  __syncthreads();
+
+  uint32_t dg[ITEMS], rk[ITEMS];
+  uint32_t* wh = whist + wave * NB;
+  wave_rank<D, ITEMS>(e, lim, shift, mask, wh, dg, rk);
+  __syncthreads();
+
+  uint32_t tot[DPT], gstart[DPT];
+  uint32_t lbw[DPT][kLookback];
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const uint32_t b = t + k * BLOCK;
+    tot[k] = 0;
+    gstart[k] = 0;
+    if (b < (uint32_t)NB) {
+      gstart[k] = starts[b];
+      uint32_t s = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const uint32_t c = whist[w * NB + b];
+        whist[w * NB + b] = s;
+        s += c;
+      }
+      tot[k] = s;
+      sm.start[b] = s;
+      uint32_t* mine = st_cur + (size_t)tile * NB + b;
+      if (tile == 0) {
+        st_status(mine, (s + 1u) | kFlagInclusive);
+      } else {
+        st_status(mine, s + 1u);
+        lookback_issue(st_cur, tile, NB, b, lbw[k]);
+      }
+    }
+  }
+  block_exclusive_scan<NB, BLOCK>(sm.start, sm.wsum);
+
+  uint32_t pos[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) pos[i] = sm.start[dg[i]] + wh[dg[i]] + rk[i];
+  __syncthreads();  // counters dead: the buffer takes the keys
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if ((uint32_t)i * kWave + lane < lim) sm.half[pos[i]] = e[i].key;
+
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const uint32_t b = t + k * BLOCK;
+    if (b < (uint32_t)NB) {
+      uint32_t prefix = 0;
+      if (tile > 0) {
+        prefix = lookback_finish(st_cur, tile, NB, b, lbw[k]);
+        st_status(st_cur + (size_t)tile * NB + b, (prefix + tot[k] + 1u) | kFlagInclusive);
+      }
+      sm.gbase[b] = gstart[k] + prefix - sm.start[b];
+    }
+  }
+  __syncthreads();
+
+  // keys out; remember where each of this thread's slots went
+  uint32_t dest[ITEMS];
+  uint64_t* out = reinterpret_cast<uint64_t*>(dst);
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t j = k * BLOCK + t;
+    dest[k] = 0;
+    if (j < tile_n) {
+      const uint64_t key = sm.half[j];
+      dest[k] = sm.gbase[digit_of(key, shift, mask)] + j;
+      out[2 * (size_t)dest[k]] = key;
+    }
+  }
+  __syncthreads();  // keys read: the buffer takes the values
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if ((uint32_t)i * kWave + lane < lim) sm.half[pos[i]] = e[i].val;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t j = k * BLOCK + t;
+    if (j < tile_n) out[2 * (size_t)dest[k] + 1] = sm.half[j];
+  }
+}
+
 // Loads and processes one tile (the whole block, after a barrier).
 template <int D, int BLOCK, int ITEMS>
 __device__ __forceinline__ void onesweep_tile(TileSmem<D, BLOCK, ITEMS>& sm,
