@@ -55,7 +55,7 @@ constexpr int kLookback = 4;
 
 // Issues the first round of a look-back (tiles tile-1 .. tile-kLookback)
 // without waiting for it, so the caller can do other work meanwhile.
-__device__ __forceinline__ void lookback_issue(const uint32_t* st, uint32_t tile,
+__device__ __forceinline__ void lookback_issue(const uint32_t* __restrict__ st, uint32_t tile,
                                                uint32_t nb, uint32_t b,
                                                uint32_t (&w)[kLookback]) {
   const int64_t j = (int64_t)tile - 1;
@@ -65,29 +65,12 @@ __device__ __forceinline__ void lookback_issue(const uint32_t* st, uint32_t tile
 }
 
 // Finishes a look-back whose first round lookback_issue read into w.
-//
-// Front propagation: when the walk ends at an inclusive word, the walker
-// also publishes the inclusive prefix of the last kFixup aggregate tiles it
-// consumed (those nearest the inclusive front).  That value is unique --
-// inclusive(x) = prefix(this tile) - (aggregates of the tiles between x and
-// this tile) -- so it equals what tile x itself will publish, and concurrent
-// writers agree.  A tile's own aggregate is always written before any walker
-// can read it, hence before any such store.  Without this the front only
-// advances when a tile finishes its own walk, and walks grow to dozens of
-// tiles under load.
-constexpr int kFixup = 4;
-
-__device__ __forceinline__ uint32_t lookback_finish(uint32_t* __restrict__ st, uint32_t tile,
-                                                    uint32_t nb, uint32_t b,
+__device__ __forceinline__ uint32_t lookback_finish(const uint32_t* __restrict__ st,
+                                                    uint32_t tile, uint32_t nb, uint32_t b,
                                                     uint32_t (&w)[kLookback]) {
   uint32_t prefix = 0;
   int64_t j = (int64_t)tile - 1;
   bool first = true;
-  // last kFixup consumed aggregates: tile index and the prefix before it
-  int64_t fx_tile[kFixup];
-  uint32_t fx_before[kFixup];
-#pragma unroll
-  for (int r = 0; r < kFixup; ++r) fx_tile[r] = -1;
   while (true) {
     if (!first) {
 #pragma unroll
@@ -102,38 +85,21 @@ __device__ __forceinline__ uint32_t lookback_finish(uint32_t* __restrict__ st, u
       if (!stop) {
         if (w[k] == 0u) {
           stop = true;
-        } else if (w[k] & kFlagInclusive) {
-          prefix += (w[k] & kCountMask) - 1u;
-          ++used;
-          stop = done = true;
         } else {
-#pragma unroll
-          for (int r = kFixup - 1; r > 0; --r) {
-            fx_tile[r] = fx_tile[r - 1];
-            fx_before[r] = fx_before[r - 1];
-          }
-          fx_tile[0] = j - k;
-          fx_before[0] = prefix;
           prefix += (w[k] & kCountMask) - 1u;
           ++used;
+          if (w[k] & kFlagInclusive) stop = done = true;
         }
       }
     }
-    if (done) {
-#pragma unroll
-      for (int r = 0; r < kFixup; ++r)
-        if (fx_tile[r] >= 0)
-          st_status(st + (size_t)fx_tile[r] * nb + b,
-                    ((prefix - fx_before[r]) + 1u) | kFlagInclusive);
-      return prefix;
-    }
+    if (done) return prefix;
     j -= used;
     if (used == 0) __builtin_amdgcn_s_sleep(1);
   }
 }
 
-__device__ __forceinline__ uint32_t lookback(uint32_t* st, uint32_t tile, uint32_t nb,
-                                             uint32_t b) {
+__device__ __forceinline__ uint32_t lookback(const uint32_t* __restrict__ st, uint32_t tile,
+                                             uint32_t nb, uint32_t b) {
   uint32_t w[kLookback];
   lookback_issue(st, tile, nb, b, w);
   return lookback_finish(st, tile, nb, b, w);
