@@ -41,10 +41,10 @@ constexpr size_t kAlign = 256;
 inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 struct Plan {
-  uint32_t key_bits, d, tile, passes, nb, ntiles;
+  uint32_t key_bits, d, tile, passes, nb, ntiles, region_tiles;
   const lsb::SortVariant* v;
   // workspace offsets
-  size_t counts, starts, plan, tickets, status, total;
+  size_t counts, regnext, starts, plan, tickets, status, total;
 };
 
 lsb_config defaults() {
@@ -116,11 +116,13 @@ int make_plan(size_t n, const lsb_config& c, Plan* p) {
   p->passes = (p->key_bits + d - 1) / d;
   p->nb = 1u << d;
   p->ntiles = (uint32_t)((n + p->tile - 1) / p->tile);
+  p->region_tiles = (p->ntiles + lsb::kRegions - 1) / lsb::kRegions;
   size_t off = 0;
-  p->counts = off;  off += align_up((size_t)p->passes * p->nb * 4);
+  p->counts = off;  off += align_up((size_t)p->passes * lsb::kRegions * p->nb * 4);
+  p->regnext = off; off += align_up((size_t)p->passes * lsb::kRegions * p->nb * 4);
   p->starts = off;  off += align_up((size_t)p->passes * p->nb * 4);
   p->plan = off;    off += align_up((size_t)(4 * p->passes + 4) * 4);
-  p->tickets = off; off += align_up((size_t)p->passes * 4);
+  p->tickets = off; off += align_up((size_t)p->passes * lsb::kRegions * 4);
   p->status = off;  off += align_up(2 * (size_t)p->ntiles * p->nb * 4);
   p->total = off;
   return LSB_OK;
@@ -223,6 +225,7 @@ int enqueue_sort(lsb_elem* A, lsb_elem* B, uint32_t n, const Plan& p, char* ws,
   const bool prof = (flags & LSB_PROFILE) != 0;
   int rc;
   uint32_t* counts = reinterpret_cast<uint32_t*>(ws + p.counts);
+  uint32_t* regnext = reinterpret_cast<uint32_t*>(ws + p.regnext);
   uint32_t* starts = reinterpret_cast<uint32_t*>(ws + p.starts);
   uint32_t* plan = reinterpret_cast<uint32_t*>(ws + p.plan);
   uint32_t* tickets = reinterpret_cast<uint32_t*>(ws + p.tickets);
@@ -234,10 +237,12 @@ int enqueue_sort(lsb_elem* A, lsb_elem* B, uint32_t n, const Plan& p, char* ws,
     if (p.passes > 16) return LSB_EINVAL;
   }
   if ((rc = prof_mark(prof, 0, s))) return rc;
-  lsb::launch_zero(counts, (size_t)p.passes * p.nb, s);
-  LSB_HIP(lsb::launch_histogram(p.d, A, n, p.key_bits, p.passes, counts, status, slice, s));
+  lsb::launch_zero(counts, (size_t)p.passes * lsb::kRegions * p.nb, s);
+  LSB_HIP(lsb::launch_histogram(p.d, A, n, p.key_bits, p.passes, counts,
+                                p.region_tiles * p.tile, status, slice, s));
   if ((rc = prof_mark(prof, 1, s))) return rc;
-  LSB_HIP(lsb::launch_scan_plan(p.d, counts, starts, plan, tickets, n, p.passes, flags, s));
+  LSB_HIP(lsb::launch_scan_plan(p.d, counts, starts, plan, tickets, regnext, n, p.passes, flags,
+                                s));
   if ((rc = prof_mark(prof, 2, s))) return rc;
   for (uint32_t q = 0; q < p.passes; ++q) {
     lsb::PassArgs a;
@@ -253,6 +258,11 @@ int enqueue_sort(lsb_elem* A, lsb_elem* B, uint32_t n, const Plan& p, char* ws,
     a.tickets = tickets;
     a.status = status;
     a.ntiles = p.ntiles;
+    a.passes = p.passes;
+    a.key_bits = p.key_bits;
+    a.region_tiles = p.region_tiles;
+    a.regcount = counts;
+    a.regnext = regnext;
     p.v->scatter(a, s);
     if ((rc = prof_mark(prof, 3 + q, s))) return rc;
   }
@@ -866,9 +876,19 @@ int lsb_debug_histogram(const lsb_elem* A, size_t n, const lsb_config* cfg, uint
   int rc = make_plan(n > 1 ? n : 2, c, &p);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)c.stream;
-  lsb::launch_zero(counts, (size_t)p.passes * p.nb, s);
-  if (n) LSB_HIP(lsb::launch_histogram(p.d, A, (uint32_t)n, p.key_bits, p.passes, counts,
-                                       nullptr, 0, s));
+  // One region spanning the whole array: regcount[p][0][b] is then the
+  // global count, the other regions stay empty.  The caller's buffer holds
+  // P rows, so the kRegions-strided result goes through a scratch copy.
+  uint32_t* tmp = nullptr;
+  const size_t words = (size_t)p.passes * lsb::kRegions * p.nb;
+  LSB_HIP(hipMallocAsync(&tmp, words * 4, s));
+  lsb::launch_zero(tmp, words, s);
+  if (n) LSB_HIP(lsb::launch_histogram(p.d, A, (uint32_t)n, p.key_bits, p.passes, tmp,
+                                       (uint32_t)n, nullptr, 0, s));
+  for (uint32_t q = 0; q < p.passes; ++q)
+    LSB_HIP(hipMemcpyAsync(counts + (size_t)q * p.nb, tmp + (size_t)q * lsb::kRegions * p.nb,
+                           p.nb * 4, hipMemcpyDeviceToDevice, s));
+  LSB_HIP(hipFreeAsync(tmp, s));
   LSB_HIP(hipGetLastError());
   if (c.flags & LSB_SYNC) LSB_HIP(hipStreamSynchronize(s));
   return LSB_OK;

@@ -8,6 +8,11 @@
 
 namespace lsb {
 
+// Each scatter pass cuts the array into this many regions of whole tiles;
+// every region has its own tile tickets and look-back chain, so eight
+// chains advance side by side instead of one.
+constexpr int kRegions = 8;
+
 // Per-pass plan written by the scan kernel, read by the scatter kernels:
 // plan[4*p + 0] = pass p executes, [4*p+1] = its ordinal among executed
 // passes, [4*p+2] = its source is B (else A).  plan[4*P] = 1 when the
@@ -21,9 +26,15 @@ struct PassArgs {
   uint32_t mask;
   const uint32_t* plan;
   const uint32_t* starts;  // starts of this pass: global exclusive bucket offsets
-  uint32_t* tickets;       // one tile ticket counter per pass
+  uint32_t* tickets;       // tile ticket counters [pass][region]
   uint32_t* status;        // two look-back slices of tiles * 2^d words
   uint32_t ntiles;
+  // region form
+  uint32_t passes;
+  uint32_t key_bits;
+  uint32_t region_tiles;        // tiles per region (the last region may hold fewer)
+  const uint32_t* regcount;     // histogram kernel's per-region counts [P][R][2^d]
+  uint32_t* regnext;            // per-region counts filled by the pass before [P][R][2^d]
 };
 
 typedef void (*ScatterLaunch)(const PassArgs&, hipStream_t);
@@ -32,23 +43,25 @@ struct SortVariant {
   uint32_t digit_bits;
   uint32_t tile;  // elements per tile (block * items)
   uint32_t block;
-  uint32_t persistent;  // 1 = persistent grid prefetching its next tile, 2 = split staging
+  uint32_t persistent;  // 1 = persistent prefetching, 2 = split staging, 3 = regions
   ScatterLaunch scatter;
 };
 
 // Compiled variants (lsb_kernels.hip).
 const SortVariant* sort_variants(int* count);
 
-// Zeroes counts[0..words) (and tickets), launched before the histogram.
+// Zeroes p[0..words), launched before the histogram.
 void launch_zero(uint32_t* p, size_t words, hipStream_t s);
-// All-digit histogram: counts[p * 2^d + b] += ..., zeroes zero[0..zero_words).
+// All-digit histogram per region: regcount[p][region][b] += ..., with
+// regions of region_elems elements; zeroes zero[0..zero_words).
 hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
-                            uint32_t passes, uint32_t* counts, uint32_t* zero,
-                            size_t zero_words, hipStream_t s);
-// Exclusive scan of every pass's counts, skip detection, plan, tickets reset.
-hipError_t launch_scan_plan(uint32_t d, const uint32_t* counts, uint32_t* starts,
-                            uint32_t* plan, uint32_t* tickets, uint32_t n, uint32_t passes,
-                            uint32_t flags, hipStream_t s);
+                            uint32_t passes, uint32_t* regcount, uint32_t region_elems,
+                            uint32_t* zero, size_t zero_words, hipStream_t s);
+// Global counts (sum of regions), exclusive scans, skip detection, plan;
+// clears tickets[P][R] and regnext[P][R][2^d].
+hipError_t launch_scan_plan(uint32_t d, const uint32_t* regcount, uint32_t* starts,
+                            uint32_t* plan, uint32_t* tickets, uint32_t* regnext, uint32_t n,
+                            uint32_t passes, uint32_t flags, hipStream_t s);
 // Whole sort of one array of n <= small_sort_capacity() elements inside one
 // workgroup (one HBM read and write per element).
 uint32_t small_sort_capacity();

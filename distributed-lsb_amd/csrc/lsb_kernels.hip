@@ -37,10 +37,16 @@ void launch_zero(uint32_t* p, size_t words, hipStream_t s) {
 }
 
 // ------------------------------------------------------------ k_histogram
+// The array is cut into kRegions equal regions of `region_elems` elements
+// (whole tiles); the scatter passes give each region its own look-back
+// chain.  Each block owns one contiguous chunk inside one region and adds
+// its counts for every digit into regcount[p][region][b].
 template <int D>
 __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A, uint32_t n,
                                                    uint32_t key_bits, uint32_t passes,
-                                                   uint32_t* __restrict__ counts,
+                                                   uint32_t* __restrict__ regcount,
+                                                   uint32_t region_elems,
+                                                   uint32_t blocks_per_region,
                                                    uint32_t* __restrict__ zero,
                                                    size_t zero_words) {
   constexpr int NB = 1 << D;
@@ -59,14 +65,20 @@ __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A,
   }
   __syncthreads();
 
-  const size_t step = (size_t)gridDim.x * 256 * U;
-  for (size_t base = (size_t)blockIdx.x * 256 * U; base < n; base += step) {
+  const uint32_t region = blockIdx.x / blocks_per_region;
+  const uint32_t chunk = blockIdx.x % blocks_per_region;
+  const size_t r0 = (size_t)region * region_elems;
+  const size_t r1 = r0 + region_elems < n ? r0 + region_elems : n;
+  const size_t per = (region_elems + blocks_per_region - 1) / blocks_per_region;
+  const size_t c0 = r0 + (size_t)chunk * per;
+  const size_t c1 = c0 + per < r1 ? c0 + per : r1;
+  for (size_t base = c0; base < c1; base += 256 * U) {
     uint64_t k[U];
     bool v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const size_t i = base + (size_t)u * 256 + t;
-      v[u] = i < n;
+      v[u] = i < c1;
       k[u] = v[u] ? A[i].key : 0ull;
     }
 #pragma unroll
@@ -80,50 +92,58 @@ __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A,
     }
   }
   __syncthreads();
+  if (c0 >= c1) return;
   for (uint32_t i = t; i < passes * NB; i += 256) {
     const uint32_t c = h[i];
-    if (c) atomicAdd(&counts[i], c);
+    const uint32_t p = i / NB, b = i % NB;
+    if (c) atomicAdd(&regcount[((size_t)p * kRegions + region) * NB + b], c);
   }
 }
 
 hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
-                            uint32_t passes, uint32_t* counts, uint32_t* zero,
-                            size_t zero_words, hipStream_t s) {
+                            uint32_t passes, uint32_t* regcount, uint32_t region_elems,
+                            uint32_t* zero, size_t zero_words, hipStream_t s) {
   // Enough blocks to keep every CU streaming, few enough that the final
-  // flush of LDS counters to HBM stays small.
-  size_t blocks = ((size_t)n + 256 * 4 * 8 - 1) / (256 * 4 * 8);
-  if (blocks < 1) blocks = 1;
-  if (blocks > 2048) blocks = 2048;
-  const size_t zblocks = (zero_words / 4 + 255) / 256;
-  if (blocks < zblocks && zblocks <= 2048) blocks = zblocks;
-  if (blocks < zblocks) blocks = 2048;
+  // flush of LDS counters stays small.
+  size_t per_region = ((size_t)region_elems + 256 * 4 * 8 - 1) / (256 * 4 * 8);
+  if (per_region < 1) per_region = 1;
+  if (per_region > 256) per_region = 256;
+  const size_t blocks = per_region * kRegions;
   const elem_t* a = static_cast<const elem_t*>(A);
   if (d == 8)
     hipLaunchKernelGGL(k_histogram<8>, dim3((unsigned)blocks), dim3(256), 0, s, a, n, key_bits,
-                       passes, counts, zero, zero_words);
+                       passes, regcount, region_elems, (uint32_t)per_region, zero, zero_words);
   else if (d == 11)
-    hipLaunchKernelGGL(k_histogram<11>, dim3((unsigned)blocks), dim3(256), 0, s, a, n,
-                       key_bits, passes, counts, zero, zero_words);
+    hipLaunchKernelGGL(k_histogram<11>, dim3((unsigned)blocks), dim3(256), 0, s, a, n, key_bits,
+                       passes, regcount, region_elems, (uint32_t)per_region, zero, zero_words);
   else
     return hipErrorInvalidValue;
   return hipSuccess;
 }
 
 // ------------------------------------------------------------ k_scan_plan
+// Global count of bucket b = sum over regions; exclusive scan per digit;
+// single-bucket detection; plan; clears the tickets and the per-region
+// counters the scatter passes fill for their successors.
 template <int NB>
-__global__ __launch_bounds__(1024) void k_scan_plan(const uint32_t* __restrict__ counts,
+__global__ __launch_bounds__(1024) void k_scan_plan(const uint32_t* __restrict__ regcount,
                                                     uint32_t* __restrict__ starts,
                                                     uint32_t* __restrict__ plan,
-                                                    uint32_t* __restrict__ tickets, uint32_t n,
+                                                    uint32_t* __restrict__ tickets,
+                                                    uint32_t* __restrict__ regnext, uint32_t n,
                                                     uint32_t passes, uint32_t flags) {
   __shared__ uint32_t a[NB];
   __shared__ uint32_t wsum[16];
   __shared__ uint32_t trivial[16];
   const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < passes * kRegions * NB; i += 1024) regnext[i] = 0u;
+  for (uint32_t i = t; i < passes * kRegions; i += 1024) tickets[i] = 0u;
   for (uint32_t p = 0; p < passes; ++p) {
     int triv = 0;
     for (uint32_t i = t; i < NB; i += 1024) {
-      const uint32_t c = counts[p * NB + i];
+      uint32_t c = 0;
+#pragma unroll
+      for (int r = 0; r < kRegions; ++r) c += regcount[((size_t)p * kRegions + r) * NB + i];
       a[i] = c;
       triv |= (c == n);
     }
@@ -141,22 +161,21 @@ __global__ __launch_bounds__(1024) void k_scan_plan(const uint32_t* __restrict__
       plan[4 * p + 1] = ord;
       plan[4 * p + 2] = ord & 1u;
       plan[4 * p + 3] = 0u;
-      tickets[p] = 0u;
       ord += run;
     }
     plan[4 * passes] = ord & 1u;
   }
 }
 
-hipError_t launch_scan_plan(uint32_t d, const uint32_t* counts, uint32_t* starts,
-                            uint32_t* plan, uint32_t* tickets, uint32_t n, uint32_t passes,
-                            uint32_t flags, hipStream_t s) {
+hipError_t launch_scan_plan(uint32_t d, const uint32_t* regcount, uint32_t* starts,
+                            uint32_t* plan, uint32_t* tickets, uint32_t* regnext, uint32_t n,
+                            uint32_t passes, uint32_t flags, hipStream_t s) {
   if (d == 8)
-    hipLaunchKernelGGL(k_scan_plan<256>, dim3(1), dim3(1024), 0, s, counts, starts, plan,
-                       tickets, n, passes, flags);
+    hipLaunchKernelGGL(k_scan_plan<256>, dim3(1), dim3(1024), 0, s, regcount, starts, plan,
+                       tickets, regnext, n, passes, flags);
   else if (d == 11)
-    hipLaunchKernelGGL(k_scan_plan<2048>, dim3(1), dim3(1024), 0, s, counts, starts, plan,
-                       tickets, n, passes, flags);
+    hipLaunchKernelGGL(k_scan_plan<2048>, dim3(1), dim3(1024), 0, s, regcount, starts, plan,
+                       tickets, regnext, n, passes, flags);
   else
     return hipErrorInvalidValue;
   return hipSuccess;
@@ -252,6 +271,100 @@ static void launch_onesweep_pf(const PassArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_onesweep_pf<D, BLOCK, ITEMS>), dim3(g), dim3(BLOCK), 0, s, a);
 }
 
+// Region form: a persistent grid of kRegions groups of blocks; group r takes
+// tickets for the tiles of region r only and runs that region's look-back
+// chain, so kRegions chains advance side by side.  Each region's bucket
+// bases come from the per-region counts of the previous executed pass (or of
+// the histogram kernel for the first one); the pass itself counts, per
+// region of its output, the digits of the next executed pass.
+template <int D, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_onesweep_rg(PassArgs args) {
+  constexpr int NB = 1 << D;
+  static_assert(kRegions == kRegionsDev, "region count");
+  __shared__ TileSmem<D, BLOCK, ITEMS> sm;
+  __shared__ uint32_t s_rbase[NB];
+  __shared__ uint32_t s_hnext[kRegions * NB];
+  const uint32_t* plan = args.plan;
+  const uint32_t pass = args.pass;
+  if (!plan[4 * pass]) return;
+  const uint32_t ord = plan[4 * pass + 1];
+  const bool src_b = plan[4 * pass + 2] != 0;
+  const elem_t* src = static_cast<const elem_t*>(src_b ? args.B : args.A);
+  elem_t* dst = static_cast<elem_t*>(src_b ? args.A : args.B);
+  const size_t slice = (size_t)args.ntiles * NB;
+  uint32_t* st_cur = args.status + (size_t)(ord & 1u) * slice;
+  uint32_t* st_next = args.status + (size_t)((ord + 1u) & 1u) * slice;
+  const uint32_t region = blockIdx.x % kRegions;
+  const uint32_t t_begin = region * args.region_tiles;
+  if (t_begin >= args.ntiles) return;
+  const uint32_t t_count =
+      args.ntiles - t_begin < args.region_tiles ? args.ntiles - t_begin : args.region_tiles;
+  const uint32_t region_elems = args.region_tiles * (uint32_t)(BLOCK * ITEMS);
+
+  // the next executed pass, whose per-region counts this pass produces
+  int q = -1;
+  for (uint32_t p2 = pass + 1; p2 < args.passes; ++p2)
+    if (plan[4 * p2]) {
+      q = (int)p2;
+      break;
+    }
+  uint32_t nshift = 0, nmask = 0;
+  if (q >= 0) {
+    nshift = (uint32_t)q * D;
+    const uint32_t w = args.key_bits - nshift < (uint32_t)D ? args.key_bits - nshift : (uint32_t)D;
+    nmask = (1u << w) - 1u;
+  }
+  const uint32_t* C = (ord == 0 ? args.regcount : args.regnext) + (size_t)pass * kRegions * NB;
+  for (uint32_t b = threadIdx.x; b < (uint32_t)NB; b += BLOCK) {
+    uint32_t s = args.starts[b];
+    for (uint32_t r = 0; r < region; ++r) s += C[(size_t)r * NB + b];
+    s_rbase[b] = s;
+  }
+  for (uint32_t i = threadIdx.x; i < (uint32_t)(kRegions * NB); i += BLOCK) s_hnext[i] = 0u;
+  uint32_t* ticket = &args.tickets[pass * kRegions + region];
+  if (threadIdx.x == 0) sm.scalar[0] = atomicAdd(ticket, 1u);
+  __syncthreads();
+  while (true) {
+    const uint32_t local = sm.scalar[0];
+    if (local >= t_count) break;
+    const uint32_t tile = t_begin + local;
+    elem_t e[ITEMS];
+    tile_load<BLOCK, ITEMS>(e, src, args.n, tile);
+    onesweep_tile_region<D, BLOCK, ITEMS>(sm, e, dst, args.n, tile, local == 0, st_cur, st_next,
+                                          s_rbase, args.shift, args.mask, ticket,
+                                          q >= 0 ? s_hnext : nullptr, nshift, nmask,
+                                          region_elems);
+    __syncthreads();
+  }
+  if (q >= 0) {
+    uint32_t* out = args.regnext + (size_t)q * kRegions * NB;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(kRegions * NB); i += BLOCK) {
+      const uint32_t c = s_hnext[i];
+      if (c) atomicAdd(&out[i], c);
+    }
+  }
+}
+
+template <int D, int BLOCK, int ITEMS>
+static void launch_onesweep_rg(const PassArgs& a, hipStream_t s) {
+  static uint32_t per_region = 0;
+  if (!per_region) {
+    int dev = 0, cus = 0, occ = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_onesweep_rg<D, BLOCK, ITEMS>, BLOCK,
+                                                     0) != hipSuccess) {
+      (void)hipGetLastError();
+      cus = 256;
+      occ = 1;
+    }
+    per_region = (uint32_t)((cus * (occ > 0 ? occ : 1)) / kRegions);
+    if (per_region < 1) per_region = 1;
+  }
+  const uint32_t pr = a.region_tiles < per_region ? a.region_tiles : per_region;
+  hipLaunchKernelGGL((k_onesweep_rg<D, BLOCK, ITEMS>), dim3(pr * kRegions), dim3(BLOCK), 0, s, a);
+}
+
 // Split-staging form (keys, then values, through an 8-byte buffer).
 template <int D, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_onesweep_split(PassArgs args) {
@@ -289,6 +402,8 @@ static void launch_onesweep(const PassArgs& a, hipStream_t s) {
 // Variants are (digit width, tile, block, persistent-prefetch?).  The
 // first one per digit width is the default for that width.
 static const SortVariant kVariants[] = {
+    {8, 256 * 16, 256, 3, &launch_onesweep_rg<8, 256, 16>},
+    {8, 256 * 12, 256, 3, &launch_onesweep_rg<8, 256, 12>},
     {8, 256 * 16, 256, 0, &launch_onesweep<8, 256, 16>},
     {8, 256 * 12, 256, 0, &launch_onesweep<8, 256, 12>},
     {8, 512 * 16, 512, 0, &launch_onesweep<8, 512, 16>},

@@ -13,6 +13,9 @@
 
 namespace lsb {
 
+// Regions per scatter pass (kRegions in lsb_internal.hpp).
+constexpr int kRegionsDev = 8;
+
 // ------------------------------------------------------------- wave_rank
 // Ranks this wave's ITEMS rows into its LDS counters wh[0 .. 2^D).  Row i,
 // lane l is valid when i*64 + l < lim.  For every valid element: dg = its
@@ -374,6 +377,116 @@ __device__ __forceinline__ void onesweep_tile_split(TileSmemSplit<D, BLOCK, ITEM
     const uint32_t j = k * BLOCK + t;
     if (j < tile_n) out[2 * (size_t)dest[k] + 1] = sm.half[j];
   }
+}
+
+// ------------------------------------------------------- region form
+// One tile of a pass whose array is cut into kRegions regions, each with its
+// own ticket counter and look-back chain (`chain_first` = the region's first
+// tile, which publishes an inclusive word at once).  rbase[b] (LDS) is the
+// destination of the region's first element of bucket b: the global bucket
+// start plus the bucket's elements in earlier regions.  Two more duties:
+//  * the block's next ticket is requested before the store phase, so its
+//    latency hides under the stores (thread 0 leaves it in sm.scalar[0]);
+//  * when hnext is given, every stored element is counted by the region its
+//    destination falls in and by its digit for the next executed pass
+//    (nshift, nmask) -- the per-region counts that pass needs.
+template <int D, int BLOCK, int ITEMS>
+__device__ __forceinline__ void onesweep_tile_region(
+    TileSmem<D, BLOCK, ITEMS>& sm, const elem_t (&e)[ITEMS], elem_t* __restrict__ dst,
+    uint32_t n, uint32_t tile, bool chain_first, uint32_t* st_cur, uint32_t* st_next,
+    const uint32_t* rbase, uint32_t shift, uint32_t mask, uint32_t* ticket, uint32_t* hnext,
+    uint32_t nshift, uint32_t nmask, uint32_t region_elems) {
+  using S = TileSmem<D, BLOCK, ITEMS>;
+  constexpr int NB = S::NB, W = S::W, T = S::T;
+  constexpr int DPT = (NB + BLOCK - 1) / BLOCK;
+  const uint32_t t = threadIdx.x;
+  const uint32_t lane = t & (kWave - 1);
+  const uint32_t wave = t / kWave;
+  uint32_t* whist = reinterpret_cast<uint32_t*>(sm.elems);
+
+  for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
+  for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) st_next[(size_t)tile * NB + b] = 0u;
+  const uint32_t tile_base = tile * (uint32_t)T;
+  const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
+  const uint32_t wrun = wave * (kWave * ITEMS);
+  const uint32_t lim = tile_n > wrun ? tile_n - wrun : 0u;
+  __syncthreads();
+
+  uint32_t dg[ITEMS], rk[ITEMS];
+  uint32_t* wh = whist + wave * NB;
+  wave_rank<D, ITEMS>(e, lim, shift, mask, wh, dg, rk);
+  __syncthreads();
+
+  uint32_t tot[DPT], gstart[DPT];
+  uint32_t lbw[DPT][kLookback];
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const uint32_t b = t + k * BLOCK;
+    tot[k] = 0;
+    gstart[k] = 0;
+    if (b < (uint32_t)NB) {
+      gstart[k] = rbase[b];
+      uint32_t s = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const uint32_t c = whist[w * NB + b];
+        whist[w * NB + b] = s;
+        s += c;
+      }
+      tot[k] = s;
+      sm.start[b] = s;
+      uint32_t* mine = st_cur + (size_t)tile * NB + b;
+      if (chain_first) {
+        st_status(mine, (s + 1u) | kFlagInclusive);
+      } else {
+        st_status(mine, s + 1u);
+        lookback_issue(st_cur, tile, NB, b, lbw[k]);
+      }
+    }
+  }
+  block_exclusive_scan<NB, BLOCK>(sm.start, sm.wsum);
+
+  uint32_t pos[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) pos[i] = sm.start[dg[i]] + wh[dg[i]] + rk[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if ((uint32_t)i * kWave + lane < lim) sm.elems[pos[i]] = e[i];
+
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const uint32_t b = t + k * BLOCK;
+    if (b < (uint32_t)NB) {
+      uint32_t prefix = 0;
+      if (!chain_first) {
+        prefix = lookback_finish(st_cur, tile, NB, b, lbw[k]);
+        st_status(st_cur + (size_t)tile * NB + b, (prefix + tot[k] + 1u) | kFlagInclusive);
+      }
+      sm.gbase[b] = gstart[k] + prefix - sm.start[b];
+    }
+  }
+  __syncthreads();
+
+  uint32_t next = 0;
+  if (t == 0) next = atomicAdd(ticket, 1u);
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t j = k * BLOCK + t;
+    if (j < tile_n) {
+      const elem_t x = sm.elems[j];
+      const uint32_t d = digit_of(x.key, shift, mask);
+      const uint32_t dest = sm.gbase[d] + j;
+      dst[(size_t)dest] = x;
+      if (hnext) {
+        uint32_t r = 0;
+#pragma unroll
+        for (int q = 1; q < kRegionsDev; ++q) r += dest >= (uint32_t)q * region_elems ? 1u : 0u;
+        atomicAdd(&hnext[r * NB + digit_of(x.key, nshift, nmask)], 1u);
+      }
+    }
+  }
+  if (t == 0) sm.scalar[0] = next;
 }
 
 // Loads and processes one tile (the whole block, after a barrier).
