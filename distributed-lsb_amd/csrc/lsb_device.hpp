@@ -39,6 +39,17 @@ __device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also orders
+// global memory, so the compiler drains every outstanding global load and
+// store of the wave at it; the sort kernels communicate through LDS inside
+// a block and through device-scope atomics (or kernel boundaries) across
+// blocks, so they keep global traffic in flight across their barriers.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
   const uint32_t lane = lane_id();
 #pragma unroll
@@ -57,7 +68,7 @@ __device__ __forceinline__ void block_exclusive_scan(uint32_t* a, uint32_t* wave
   constexpr int PER = (N + BLOCK - 1) / BLOCK;
   constexpr int W = BLOCK / kWave;
   const int t = threadIdx.x;
-  __syncthreads();
+  lds_barrier();
   uint32_t v[PER];
   uint32_t s = 0;
 #pragma unroll
@@ -68,7 +79,7 @@ __device__ __forceinline__ void block_exclusive_scan(uint32_t* a, uint32_t* wave
   }
   const uint32_t inc = wave_inclusive_scan(s);
   if (lane_id() == kWave - 1) wave_sums[t / kWave] = inc;
-  __syncthreads();
+  lds_barrier();
   uint32_t run = inc - s;
   const int w = t / kWave;
 #pragma unroll
@@ -80,7 +91,7 @@ __device__ __forceinline__ void block_exclusive_scan(uint32_t* a, uint32_t* wave
     if (i < N) a[i] = run;
     run += v[k];
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // Digit p of a key: bits [shift, shift + width) where the last digit is cut

@@ -178,12 +178,12 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
   const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
   const uint32_t wrun = wave * (kWave * ITEMS);
   const uint32_t lim = tile_n > wrun ? tile_n - wrun : 0u;
-  __syncthreads();
+  lds_barrier();
 
   uint32_t dg[ITEMS], rk[ITEMS];
   uint32_t* wh = whist + wave * NB;
   wave_rank<D, ITEMS>(e, lim, shift, mask, wh, dg, rk);
-  __syncthreads();
+  lds_barrier();
 
   // Per digit: exclusive prefix over waves (in place) and the tile total,
   // published at once as this tile's aggregate.  The first look-back round
@@ -221,7 +221,7 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
   uint32_t pos[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) pos[i] = sm.start[dg[i]] + wh[dg[i]] + rk[i];
-  __syncthreads();  // counters are dead from here: the buffer takes elements
+  lds_barrier();  // counters are dead from here: the buffer takes elements
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
     if ((uint32_t)i * kWave + lane < lim) sm.elems[pos[i]] = e[i];
@@ -238,7 +238,7 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
       sm.gbase[b] = gstart[k] + prefix - sm.start[b];
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // Write out in digit order: consecutive lanes write consecutive slots of
   // one bucket run.
@@ -296,12 +296,12 @@ __device__ __forceinline__ void onesweep_tile_split(TileSmemSplit<D, BLOCK, ITEM
   const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
   const uint32_t wrun = wave * (kWave * ITEMS);
   const uint32_t lim = tile_n > wrun ? tile_n - wrun : 0u;
-  __syncthreads();
+  lds_barrier();
 
   uint32_t dg[ITEMS], rk[ITEMS];
   uint32_t* wh = whist + wave * NB;
   wave_rank<D, ITEMS>(e, lim, shift, mask, wh, dg, rk);
-  __syncthreads();
+  lds_barrier();
 
   uint32_t tot[DPT], gstart[DPT];
   uint32_t lbw[DPT][kLookback];
@@ -335,7 +335,7 @@ __device__ __forceinline__ void onesweep_tile_split(TileSmemSplit<D, BLOCK, ITEM
   uint32_t pos[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) pos[i] = sm.start[dg[i]] + wh[dg[i]] + rk[i];
-  __syncthreads();  // counters dead: the buffer takes the keys
+  lds_barrier();  // counters dead: the buffer takes the keys
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
     if ((uint32_t)i * kWave + lane < lim) sm.half[pos[i]] = e[i].key;
@@ -352,7 +352,7 @@ __device__ __forceinline__ void onesweep_tile_split(TileSmemSplit<D, BLOCK, ITEM
       sm.gbase[b] = gstart[k] + prefix - sm.start[b];
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // keys out; remember where each of this thread's slots went
   uint32_t dest[ITEMS];
@@ -367,11 +367,11 @@ __device__ __forceinline__ void onesweep_tile_split(TileSmemSplit<D, BLOCK, ITEM
       out[2 * (size_t)dest[k]] = key;
     }
   }
-  __syncthreads();  // keys read: the buffer takes the values
+  lds_barrier();  // keys read: the buffer takes the values
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
     if ((uint32_t)i * kWave + lane < lim) sm.half[pos[i]] = e[i].val;
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const uint32_t j = k * BLOCK + t;
@@ -384,18 +384,21 @@ __device__ __forceinline__ void onesweep_tile_split(TileSmemSplit<D, BLOCK, ITEM
 // own ticket counter and look-back chain (`chain_first` = the region's first
 // tile, which publishes an inclusive word at once).  rbase[b] (LDS) is the
 // destination of the region's first element of bucket b: the global bucket
-// start plus the bucket's elements in earlier regions.  Two more duties:
-//  * the block's next ticket is requested before the store phase, so its
-//    latency hides under the stores (thread 0 leaves it in sm.scalar[0]);
+// start plus the bucket's elements in earlier regions.  More duties:
+//  * the block's next ticket is requested right after ranking, and once the
+//    tile's elements sit in LDS, the NEXT tile's loads are issued into the
+//    freed registers e[], so they are in flight during the store phase;
+//    returns the next tile's index within the region (>= t_count: none);
 //  * when hnext is given, every stored element is counted by the region its
 //    destination falls in and by its digit for the next executed pass
 //    (nshift, nmask) -- the per-region counts that pass needs.
 template <int D, int BLOCK, int ITEMS>
-__device__ __forceinline__ void onesweep_tile_region(
-    TileSmem<D, BLOCK, ITEMS>& sm, const elem_t (&e)[ITEMS], elem_t* __restrict__ dst,
-    uint32_t n, uint32_t tile, bool chain_first, uint32_t* st_cur, uint32_t* st_next,
-    const uint32_t* rbase, uint32_t shift, uint32_t mask, uint32_t* ticket, uint32_t* hnext,
-    uint32_t nshift, uint32_t nmask, uint32_t region_elems) {
+__device__ __forceinline__ uint32_t onesweep_tile_region(
+    TileSmem<D, BLOCK, ITEMS>& sm, elem_t (&e)[ITEMS], const elem_t* __restrict__ src,
+    elem_t* __restrict__ dst, uint32_t n, uint32_t tile, bool chain_first, uint32_t t_begin,
+    uint32_t t_count, uint32_t* st_cur, uint32_t* st_next, const uint32_t* rbase,
+    uint32_t shift, uint32_t mask, uint32_t* ticket, uint32_t* hnext, uint32_t nshift,
+    uint32_t nmask, uint32_t region_elems) {
   using S = TileSmem<D, BLOCK, ITEMS>;
   constexpr int NB = S::NB, W = S::W, T = S::T;
   constexpr int DPT = (NB + BLOCK - 1) / BLOCK;
@@ -410,12 +413,16 @@ __device__ __forceinline__ void onesweep_tile_region(
   const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
   const uint32_t wrun = wave * (kWave * ITEMS);
   const uint32_t lim = tile_n > wrun ? tile_n - wrun : 0u;
-  __syncthreads();
+  lds_barrier();
 
   uint32_t dg[ITEMS], rk[ITEMS];
   uint32_t* wh = whist + wave * NB;
   wave_rank<D, ITEMS>(e, lim, shift, mask, wh, dg, rk);
-  __syncthreads();
+  lds_barrier();
+  // Only thread 0 requests (and later publishes) the next ticket; leaving
+  // the value undefined on other lanes lets its latency run until its use.
+  uint32_t next;
+  if (t == 0) next = atomicAdd(ticket, 1u);
 
   uint32_t tot[DPT], gstart[DPT];
   uint32_t lbw[DPT][kLookback];
@@ -449,7 +456,7 @@ __device__ __forceinline__ void onesweep_tile_region(
   uint32_t pos[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) pos[i] = sm.start[dg[i]] + wh[dg[i]] + rk[i];
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
     if ((uint32_t)i * kWave + lane < lim) sm.elems[pos[i]] = e[i];
@@ -466,10 +473,13 @@ __device__ __forceinline__ void onesweep_tile_region(
       sm.gbase[b] = gstart[k] + prefix - sm.start[b];
     }
   }
-  __syncthreads();
+  if (t == 0) sm.scalar[1] = next;
+  lds_barrier();
 
-  uint32_t next = 0;
-  if (t == 0) next = atomicAdd(ticket, 1u);
+  // e[] is dead (the tile is in LDS): start loading the next tile into it.
+  const uint32_t nlocal = sm.scalar[1];
+  if (nlocal < t_count) tile_load<BLOCK, ITEMS>(e, src, n, t_begin + nlocal);
+
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const uint32_t j = k * BLOCK + t;
@@ -486,7 +496,7 @@ __device__ __forceinline__ void onesweep_tile_region(
       }
     }
   }
-  if (t == 0) sm.scalar[0] = next;
+  return nlocal;
 }
 
 // Loads and processes one tile (the whole block, after a barrier).
@@ -536,14 +546,14 @@ __device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, elem_t
     const uint32_t shift = p * D;
     const uint32_t w = key_bits - shift < (uint32_t)D ? key_bits - shift : (uint32_t)D;
     const uint32_t mask = (1u << w) - 1u;
-    __syncthreads();  // previous pass's reload is done with the buffer
+    lds_barrier();  // previous pass's reload is done with the buffer
     for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
     if (t == 0) sm.scalar[0] = 0u;
-    __syncthreads();
+    lds_barrier();
     uint32_t dg[ITEMS], rk[ITEMS];
     uint32_t* wh = whist + wave * NB;
     wave_rank<D, ITEMS>(e, lim, shift, mask, wh, dg, rk);
-    __syncthreads();
+    lds_barrier();
     for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) {
       uint32_t s = 0;
 #pragma unroll
@@ -560,11 +570,11 @@ __device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, elem_t
     uint32_t pos[ITEMS];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) pos[i] = sm.start[dg[i]] + wh[dg[i]] + rk[i];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i)
       if ((uint32_t)i * kWave + lane < lim) sm.elems[pos[i]] = e[i];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i)
       if ((uint32_t)i * kWave + lane < lim) e[i] = sm.elems[wrun + (uint32_t)i * kWave + lane];
