@@ -452,6 +452,7 @@ __device__ __forceinline__ uint32_t onesweep_tile_region(
     }
   }
   block_exclusive_scan<NB, BLOCK>(sm.start, sm.wsum);
+  if (t == 0) sm.scalar[1] = next;  // returned long ago, during the scan
 
   uint32_t pos[ITEMS];
 #pragma unroll
@@ -473,7 +474,6 @@ __device__ __forceinline__ uint32_t onesweep_tile_region(
       sm.gbase[b] = gstart[k] + prefix - sm.start[b];
     }
   }
-  if (t == 0) sm.scalar[1] = next;
   lds_barrier();
 
   // e[] is dead (the tile is in LDS): start loading the next tile into it.
