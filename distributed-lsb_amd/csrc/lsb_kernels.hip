@@ -324,14 +324,16 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep_rg(PassArgs args) {
   uint32_t* ticket = &args.tickets[pass * kRegions + region];
   if (threadIdx.x == 0) sm.scalar[0] = atomicAdd(ticket, 1u);
   lds_barrier();
-  uint32_t local = sm.scalar[0];
-  elem_t e[ITEMS];
-  if (local < t_count) tile_load<BLOCK, ITEMS>(e, src, args.n, t_begin + local);
-  while (local < t_count) {
+  while (true) {
+    const uint32_t local = sm.scalar[0];
+    if (local >= t_count) break;
     const uint32_t tile = t_begin + local;
-    local = onesweep_tile_region<D, BLOCK, ITEMS>(
-        sm, e, src, dst, args.n, tile, local == 0, t_begin, t_count, st_cur, st_next, s_rbase,
-        args.shift, args.mask, ticket, q >= 0 ? s_hnext : nullptr, nshift, nmask, region_elems);
+    elem_t e[ITEMS];
+    tile_load<BLOCK, ITEMS>(e, src, args.n, tile);
+    onesweep_tile_region<D, BLOCK, ITEMS>(sm, e, dst, args.n, tile, local == 0, st_cur, st_next,
+                                          s_rbase, args.shift, args.mask, ticket,
+                                          q >= 0 ? s_hnext : nullptr, nshift, nmask,
+                                          region_elems);
     lds_barrier();
   }
   if (q >= 0) {

@@ -384,21 +384,20 @@ __device__ __forceinline__ void onesweep_tile_split(TileSmemSplit<D, BLOCK, ITEM
 // own ticket counter and look-back chain (`chain_first` = the region's first
 // tile, which publishes an inclusive word at once).  rbase[b] (LDS) is the
 // destination of the region's first element of bucket b: the global bucket
-// start plus the bucket's elements in earlier regions.  More duties:
-//  * the block's next ticket is requested right after ranking, and once the
-//    tile's elements sit in LDS, the NEXT tile's loads are issued into the
-//    freed registers e[], so they are in flight during the store phase;
-//    returns the next tile's index within the region (>= t_count: none);
+// start plus the bucket's elements in earlier regions.  Two more duties:
+//  * the block's next ticket is requested before the store phase, so its
+//    latency hides under the stores (thread 0 leaves it in sm.scalar[0]).
+//    It is not requested earlier: a ticket held by a block that cannot yet
+//    start its tile delays every later tile's look-back in that region;
 //  * when hnext is given, every stored element is counted by the region its
 //    destination falls in and by its digit for the next executed pass
 //    (nshift, nmask) -- the per-region counts that pass needs.
 template <int D, int BLOCK, int ITEMS>
-__device__ __forceinline__ uint32_t onesweep_tile_region(
-    TileSmem<D, BLOCK, ITEMS>& sm, elem_t (&e)[ITEMS], const elem_t* __restrict__ src,
-    elem_t* __restrict__ dst, uint32_t n, uint32_t tile, bool chain_first, uint32_t t_begin,
-    uint32_t t_count, uint32_t* st_cur, uint32_t* st_next, const uint32_t* rbase,
-    uint32_t shift, uint32_t mask, uint32_t* ticket, uint32_t* hnext, uint32_t nshift,
-    uint32_t nmask, uint32_t region_elems) {
+__device__ __forceinline__ void onesweep_tile_region(
+    TileSmem<D, BLOCK, ITEMS>& sm, const elem_t (&e)[ITEMS], elem_t* __restrict__ dst,
+    uint32_t n, uint32_t tile, bool chain_first, uint32_t* st_cur, uint32_t* st_next,
+    const uint32_t* rbase, uint32_t shift, uint32_t mask, uint32_t* ticket, uint32_t* hnext,
+    uint32_t nshift, uint32_t nmask, uint32_t region_elems) {
   using S = TileSmem<D, BLOCK, ITEMS>;
   constexpr int NB = S::NB, W = S::W, T = S::T;
   constexpr int DPT = (NB + BLOCK - 1) / BLOCK;
@@ -419,10 +418,6 @@ __device__ __forceinline__ uint32_t onesweep_tile_region(
   uint32_t* wh = whist + wave * NB;
   wave_rank<D, ITEMS>(e, lim, shift, mask, wh, dg, rk);
   lds_barrier();
-  // Only thread 0 requests (and later publishes) the next ticket; leaving
-  // the value undefined on other lanes lets its latency run until its use.
-  uint32_t next;
-  if (t == 0) next = atomicAdd(ticket, 1u);
 
   uint32_t tot[DPT], gstart[DPT];
   uint32_t lbw[DPT][kLookback];
@@ -452,7 +447,6 @@ __device__ __forceinline__ uint32_t onesweep_tile_region(
     }
   }
   block_exclusive_scan<NB, BLOCK>(sm.start, sm.wsum);
-  if (t == 0) sm.scalar[1] = next;  // returned long ago, during the scan
 
   uint32_t pos[ITEMS];
 #pragma unroll
@@ -476,10 +470,8 @@ __device__ __forceinline__ uint32_t onesweep_tile_region(
   }
   lds_barrier();
 
-  // e[] is dead (the tile is in LDS): start loading the next tile into it.
-  const uint32_t nlocal = sm.scalar[1];
-  if (nlocal < t_count) tile_load<BLOCK, ITEMS>(e, src, n, t_begin + nlocal);
-
+  uint32_t next;
+  if (t == 0) next = atomicAdd(ticket, 1u);
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const uint32_t j = k * BLOCK + t;
@@ -496,7 +488,7 @@ __device__ __forceinline__ uint32_t onesweep_tile_region(
       }
     }
   }
-  return nlocal;
+  if (t == 0) sm.scalar[0] = next;
 }
 
 // Loads and processes one tile (the whole block, after a barrier).
