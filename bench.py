@@ -79,6 +79,21 @@ def cpu_baseline(sample_n, max_cores, reps):
             "sample": f"oracle C LSD sort (16-bit digits, one thread), n={n}"}
 
 
+def traffic_from_profile(log2n, d, tile):
+    """HBM bytes per scatter launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary, when it was taken on this workload and layout."""
+    path = os.path.join(ROOT, "profiles", "k_onesweep_rg_2p30_uniform_traffic.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch: (\d+)$", text, re.M)
+    c = re.search(r"^traffic_config: log2n=(\d+) digit_bits=(\d+) tile=(\d+)$", text, re.M)
+    if not m or not c or tuple(map(int, c.groups())) != (log2n, d, tile):
+        return None
+    return int(m.group(1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,15 +175,7 @@ def main():
         if not verified:
             log(f"VERIFY FAILED: {bad} bad positions, first {first}")
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "onesweep_traffic.json")
-    if os.path.exists(tpath):
-        try:
-            tj = json.load(open(tpath))
-            if tj.get("log2n") == args.log2n and tj.get("digit_bits") == d and tj.get("tile") == tile:
-                traffic = tj.get("bytes_per_launch")
-        except Exception:
-            traffic = None
+    traffic = traffic_from_profile(args.log2n, d, tile)
 
     value = n / (ms_wall * 1e-3) / 1e6
     out = {
@@ -191,7 +198,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": f"k_onesweep<{d}> scatter pass, 32 B/elem x 2^{args.log2n}",
+                     "kernel": f"onesweep scatter pass (d={d}, tile={tile}), 32 B/elem x 2^{args.log2n}",
                      "avg_launch_ms": round(avg_pass_ms, 4)},
         "sort_roofline": {"model_bytes": model_bytes, "achieved": round(sort_gbs, 1),
                           "frac": round(sort_gbs / HBM_PEAK_GBS, 4),

@@ -31,13 +31,19 @@ __device__ __forceinline__ void wave_rank(const elem_t (&e)[ITEMS], uint32_t lim
   for (int i = 0; i < ITEMS; ++i) {
     const bool valid = (uint32_t)i * kWave + lane < lim;
     const uint32_t d = digit_of(e[i].key, shift, mask);
-    uint64_t peers = __ballot(valid);
+    const uint64_t vmask = __ballot(valid);
+    // Per bit: c = the lane's bit broadcast to 0 / ~0 (v_bfe_i32), the
+    // wave's ballot of it, and peers &= ~(ballot ^ c) on both halves with
+    // one v_bitop3 each (truth table 0x90 = a & (b == c)).
+    uint32_t plo = (uint32_t)vmask, phi = (uint32_t)(vmask >> 32);
 #pragma unroll
     for (int b = 0; b < D; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bal = __ballot(bit);
-      peers &= bit ? bal : ~bal;
+      const uint32_t c = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
+      const uint64_t bal = __ballot(c != 0u);
+      plo = __builtin_amdgcn_bitop3_b32(plo, (uint32_t)bal, c, 0x90);
+      phi = __builtin_amdgcn_bitop3_b32(phi, (uint32_t)(bal >> 32), c, 0x90);
     }
+    const uint64_t peers = ((uint64_t)phi << 32) | plo;
     const uint32_t below = count_below(peers);
     uint32_t old = 0;
     if (valid) old = wh[d];
@@ -120,6 +126,7 @@ struct TileSmem {
   elem_t elems[T];
   uint32_t start[NB];  // tile-local exclusive bucket starts
   uint32_t gbase[NB];  // destination index of tile-local slot 0, per bucket
+  uint32_t rsplit[NB];  // region form: (first slot in the next region << 3) | region
   uint32_t wsum[W];
   uint32_t scalar[4];
 };
@@ -465,7 +472,17 @@ __device__ __forceinline__ void onesweep_tile_region(
         prefix = lookback_finish(st_cur, tile, NB, b, lbw[k]);
         st_status(st_cur + (size_t)tile * NB + b, (prefix + tot[k] + 1u) | kFlagInclusive);
       }
-      sm.gbase[b] = gstart[k] + prefix - sm.start[b];
+      // The bucket's run is contiguous in dst and no longer than a region,
+      // so it crosses at most one region boundary: note its first region and
+      // the tile slot where the next region begins.
+      const uint32_t first = gstart[k] + prefix;
+      sm.gbase[b] = first - sm.start[b];
+      if (hnext) {
+        uint32_t r0 = 0;
+#pragma unroll
+        for (int q = 1; q < kRegionsDev; ++q) r0 += first >= (uint32_t)q * region_elems ? 1u : 0u;
+        sm.rsplit[b] = ((sm.start[b] + (r0 + 1u) * region_elems - first) << 3) | r0;
+      }
     }
   }
   lds_barrier();
@@ -481,9 +498,8 @@ __device__ __forceinline__ void onesweep_tile_region(
       const uint32_t dest = sm.gbase[d] + j;
       dst[(size_t)dest] = x;
       if (hnext) {
-        uint32_t r = 0;
-#pragma unroll
-        for (int q = 1; q < kRegionsDev; ++q) r += dest >= (uint32_t)q * region_elems ? 1u : 0u;
+        const uint32_t info = sm.rsplit[d];
+        const uint32_t r = (info & 7u) + (j >= (info >> 3) ? 1u : 0u);
         atomicAdd(&hnext[r * NB + digit_of(x.key, nshift, nmask)], 1u);
       }
     }
