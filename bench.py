@@ -213,8 +213,11 @@ def main():
         out["batched"] = bench_batched(args.batched_segments)
         torch.cuda.empty_cache()
     if not args.no_serving:
-        log("serving stream ...")
-        out["serving"] = bench_serving(args.serving_jobs)
+        log("serving stream, sizes 2^12..2^24 ...")
+        out["serving"] = bench_serving(args.serving_jobs, span_log2=12)
+        torch.cuda.empty_cache()
+        log("serving stream, sizes 2^12..2^28 ...")
+        out["serving_wide"] = bench_serving(args.serving_jobs, span_log2=16)
         torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
 
@@ -282,16 +285,18 @@ def bench_batched(nseg, seed=2024, reps=3):
             "speedup_vs_plain": round(t_plain / t_batched, 3), "verified_sample": ok}
 
 
-def bench_serving(jobs, seed=77):
-    """Serving stream: jobs of log-uniform size over [2^12, 2^28], 32- or
-    64-bit keys, uniform or Zipf(1.1), resident in HBM.  Three runs on the
-    same jobs: one plain lsb_sort per job in arrival order (each synchronised);
-    the server with every job submitted at once (capacity); and the server fed
-    by Poisson arrivals at 0.8 of that capacity (latency)."""
+def bench_serving(jobs, seed=77, lo_log2=12, span_log2=16):
+    """Serving stream: jobs of log-uniform size over [2^lo, 2^(lo+span)],
+    32- or 64-bit keys, uniform or Zipf(1.1), resident in HBM, admitted under
+    the server's 200 GB budget (32 B per element: A and B).  Three runs on
+    the same jobs: one plain lsb_sort per job in arrival order (each
+    synchronised); the server with every job submitted at once (capacity);
+    and the server fed by Poisson arrivals at 0.8 of that capacity
+    (latency)."""
     import torch
     import lsb_amd
     from lsb_amd import workloads as W
-    trace = W.serving_trace(jobs, seed)
+    trace = W.serving_trace(jobs, seed, lo_log2=lo_log2, span_log2=span_log2)
     dev = torch.device("cuda:0")
     bufs, pris = [], []
     for j, job in enumerate(trace):
@@ -315,8 +320,8 @@ def bench_serving(jobs, seed=77):
         t_plain += _timed(lambda: lsb_amd.sort(a, scratch, key_bits=job["key_bits"]))
     del scratch
     torch.cuda.empty_cache()
-    out = {"workload": f"{jobs} jobs, sizes log-uniform [2^12, 2^28], 32/64-bit keys, "
-                       "uniform/Zipf(1.1), HBM-resident",
+    out = {"workload": f"{jobs} jobs, sizes log-uniform [2^{lo_log2}, 2^{lo_log2 + span_log2}], "
+                       "32/64-bit keys, uniform/Zipf(1.1), HBM-resident",
            "elements": total, "plain_melem_s": round(total / t_plain / 1e6, 1)}
     with lsb_amd.Server() as srv:
         # warm the server's streams, workspaces and graphs

@@ -58,22 +58,83 @@ lsb_config defaults() {
 std::mutex g_tune_mu;
 std::vector<lsb_tune_entry> g_tune;
 
+// Built-in table: lsb_autotune's own output on one MI355X (uniform keys,
+// 2^12 .. 2^30, both key-width classes).  Variants are named by (digit_bits,
+// tile, form) and resolved to ids when used, like a loaded table.  A table
+// made by lsb_autotune or lsb_tune_load takes precedence; lsb_tune_clear
+// returns to this one.
+const lsb_tune_entry kBuiltinTune[] = {
+    {12, 64, 8, 2048, 0, 0, 0.0818f, 0.0865f},
+    {13, 64, 8, 2048, 0, 0, 0.0798f, 0.0869f},
+    {14, 64, 8, 2048, 0, 0, 0.0864f, 0.0971f},
+    {15, 64, 8, 2048, 0, 0, 0.0983f, 0.1107f},
+    {16, 64, 8, 2048, 0, 0, 0.1199f, 0.1256f},
+    {17, 64, 8, 2048, 0, 0, 0.1295f, 0.1345f},
+    {18, 64, 8, 2048, 0, 0, 0.1447f, 0.1458f},
+    {19, 64, 8, 2048, 3, 0, 0.1718f, 0.1755f},
+    {20, 64, 8, 4096, 3, 0, 0.2094f, 0.2098f},
+    {21, 64, 8, 8192, 0, 0, 0.2415f, 0.2483f},
+    {22, 64, 8, 4096, 3, 0, 0.3613f, 0.3703f},
+    {23, 64, 8, 8192, 3, 0, 0.6256f, 0.6406f},
+    {24, 64, 8, 8192, 3, 0, 1.1924f, 1.2601f},
+    {25, 64, 8, 8192, 3, 0, 2.1836f, 2.3507f},
+    {26, 64, 8, 8192, 3, 0, 4.1360f, 4.5398f},
+    {27, 64, 8, 8192, 3, 0, 8.0769f, 8.8606f},
+    {28, 64, 8, 8192, 3, 0, 16.6266f, 17.8641f},
+    {29, 64, 8, 8192, 3, 0, 33.4812f, 35.9129f},
+    {30, 64, 8, 8192, 3, 0, 67.7886f, 71.9106f},
+    {12, 32, 8, 2048, 0, 0, 0.0462f, 0.0490f},
+    {13, 32, 8, 2048, 0, 0, 0.0481f, 0.0499f},
+    {14, 32, 8, 2048, 0, 0, 0.0495f, 0.0538f},
+    {15, 32, 8, 2048, 0, 0, 0.0562f, 0.0618f},
+    {16, 32, 8, 2048, 0, 0, 0.0679f, 0.0712f},
+    {17, 32, 8, 2048, 0, 0, 0.0726f, 0.0744f},
+    {18, 32, 8, 2048, 3, 0, 0.0806f, 0.0813f},
+    {19, 32, 8, 2048, 3, 0, 0.0943f, 0.0959f},
+    {20, 32, 8, 8192, 0, 0, 0.1122f, 0.1129f},
+    {21, 32, 8, 8192, 3, 0, 0.1284f, 0.1305f},
+    {22, 32, 8, 4096, 3, 0, 0.1863f, 0.1907f},
+    {23, 32, 8, 8192, 3, 0, 0.3064f, 0.3104f},
+    {24, 32, 8, 8192, 3, 0, 0.5948f, 0.6286f},
+    {25, 32, 8, 8192, 3, 0, 1.1196f, 1.1948f},
+    {26, 32, 8, 8192, 3, 0, 2.1239f, 2.3035f},
+    {27, 32, 8, 8192, 3, 0, 4.1112f, 4.4716f},
+    {28, 32, 8, 8192, 3, 0, 8.4561f, 8.9874f},
+    {29, 32, 8, 8192, 3, 0, 16.9845f, 18.0351f},
+    {30, 32, 8, 8192, 3, 0, 34.0582f, 35.8162f},
+};
+
+uint32_t variant_id(uint32_t d, uint32_t tile, uint32_t persistent) {
+  int nv = 0;
+  const lsb::SortVariant* vs = lsb::sort_variants(&nv);
+  for (int i = 0; i < nv; ++i)
+    if (vs[i].digit_bits == d && vs[i].tile == tile && vs[i].persistent == persistent)
+      return (uint32_t)i + 1;
+  return 0;
+}
+
 // Automatic variant: the tuned entry of the nearest size class with the
-// same key-width class (<= 32 bits or wider); without a table, 8-bit digits
-// in 4096-element tiles (long per-bucket runs, two blocks per CU).
+// same key-width class (<= 32 bits or wider), from the current table or
+// else the built-in one; 0 if neither names a compiled variant.
 uint32_t auto_variant(size_t n, uint32_t key_bits) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   const bool narrow = key_bits <= 32;
   uint32_t lg = 0;
   while (lg < 63 && ((size_t)1 << (lg + 1)) <= n) ++lg;
+  const lsb_tune_entry* table = g_tune.empty() ? kBuiltinTune : g_tune.data();
+  const size_t count =
+      g_tune.empty() ? sizeof(kBuiltinTune) / sizeof(kBuiltinTune[0]) : g_tune.size();
   const lsb_tune_entry* best = nullptr;
-  for (const lsb_tune_entry& e : g_tune) {
+  for (size_t i = 0; i < count; ++i) {
+    const lsb_tune_entry& e = table[i];
     if ((e.key_bits <= 32) != narrow) continue;
     const uint32_t dist = e.log2n > lg ? e.log2n - lg : lg - e.log2n;
     const uint32_t bd = best ? (best->log2n > lg ? best->log2n - lg : lg - best->log2n) : ~0u;
     if (dist < bd) best = &e;
   }
-  return best ? best->variant : 0u;
+  if (!best) return 0u;
+  return g_tune.empty() ? variant_id(best->digit_bits, best->tile, best->persistent)
+                        : best->variant;
 }
 
 void auto_choice(size_t n, uint32_t key_bits, uint32_t* d, uint32_t* tile, uint32_t* variant) {
@@ -186,7 +247,9 @@ int get_workspace(void* stream, size_t need, void** out) {
       w.bytes = 0;
     }
     size_t bytes = need + need / 4;
-    if (hipMalloc(&w.ptr, bytes) != hipSuccess) {
+    const hipError_t he = hipMalloc(&w.ptr, bytes);
+    if (he != hipSuccess) {
+      g_last_hip = (int)he;
       (void)hipGetLastError();
       w.ptr = nullptr;
       return LSB_ENOMEM;
@@ -344,7 +407,7 @@ int make_seg_layout(size_t cap_n, uint32_t cap_segs, const lsb_config& c, SegLay
   uint32_t d = c.digit_bits ? c.digit_bits : 8;
   uint32_t want_tile = c.tile;
   if (!c.digit_bits && !c.tile && typical_n) {
-    // follow the tuned plain-sort choice for segments of the typical size
+    // follow the tuned plain-sort choice for an array of typical_n elements
     const uint32_t v = auto_variant(typical_n, L->key_bits);
     if (v) {
       int nsv = 0;
@@ -571,7 +634,10 @@ int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, ui
   if (kb > 64) return LSB_EINVAL;
   const uint64_t total_n = seg_offsets[nseg];
   uint32_t n_large = 0, n_tiny = 0, n_small = 0;
-  std::vector<uint64_t> large_sizes;
+  // The tile pipeline runs the tiles of every large segment in one
+  // persistent launch per pass, so its variant follows the tuned choice for
+  // one array of all their elements together (typical_n).
+  size_t typical_n = 0;
   for (uint32_t i = 0; i < nseg; ++i) {
     if (seg_offsets[i + 1] < seg_offsets[i]) return LSB_EINVAL;
     const uint64_t n = seg_offsets[i + 1] - seg_offsets[i];
@@ -579,15 +645,7 @@ int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, ui
     if (n <= 1) continue;
     if (n <= lsb::seg_tiny_capacity()) ++n_tiny;
     else if (n <= lsb::seg_small_capacity()) ++n_small;
-    else { ++n_large; large_sizes.push_back(n); }
-  }
-  // The tile pipeline's variant follows the tuned choice for the median
-  // large segment.
-  size_t typical_n = 0;
-  if (!large_sizes.empty()) {
-    std::nth_element(large_sizes.begin(), large_sizes.begin() + large_sizes.size() / 2,
-                     large_sizes.end());
-    typical_n = large_sizes[large_sizes.size() / 2];
+    else { ++n_large; typical_n += n; }
   }
   if (n_large + n_tiny + n_small == 0) return LSB_OK;
   if (!A || !B || (((uintptr_t)A | (uintptr_t)B) & 15u)) return LSB_EINVAL;
@@ -712,12 +770,20 @@ int lsb_autotune(uint32_t lo_log2, uint32_t hi_log2, uint32_t key_bits, void* st
     if (e1) (void)hipEventDestroy(e1);
     (void)hipGetLastError();
   };
-  if (hipMalloc(&pristine, nmax * sizeof(lsb_elem)) != hipSuccess ||
-      hipMalloc(&A, nmax * sizeof(lsb_elem)) != hipSuccess ||
-      hipMalloc(&B, nmax * sizeof(lsb_elem)) != hipSuccess ||
-      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+  hipError_t he = hipMalloc(&pristine, nmax * sizeof(lsb_elem));
+  if (he == hipSuccess) he = hipMalloc(&A, nmax * sizeof(lsb_elem));
+  if (he == hipSuccess) he = hipMalloc(&B, nmax * sizeof(lsb_elem));
+  if (he != hipSuccess) {
+    g_last_hip = (int)he;
     cleanup();
     return LSB_ENOMEM;
+  }
+  he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    g_last_hip = (int)he;
+    cleanup();
+    return LSB_EHIP;
   }
   lsb_gen_spec g;
   memset(&g, 0, sizeof g);

@@ -43,7 +43,7 @@ struct SortVariant {
   uint32_t digit_bits;
   uint32_t tile;  // elements per tile (block * items)
   uint32_t block;
-  uint32_t persistent;  // 1 = persistent prefetching, 2 = split staging, 3 = regions
+  uint32_t persistent;  // 0 = one tile per block, 3 = region form (persistent)
   ScatterLaunch scatter;
 };
 

@@ -207,70 +207,6 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(PassArgs args) {
                                  args.shift, args.mask);
 }
 
-// Persistent form: a grid that fits the device at once; each block keeps
-// taking tickets, and takes its NEXT ticket and issues that tile's loads
-// before it ranks, looks back and stores the current tile, so the loads of
-// one tile are in flight under the work of the previous one.  A block waits
-// only on tiles with smaller tickets, which running blocks hold and finish
-// without waiting on later tiles, so the loop always drains.
-template <int D, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void k_onesweep_pf(PassArgs args) {
-  constexpr int NB = 1 << D;
-  __shared__ TileSmem<D, BLOCK, ITEMS> sm;
-  const uint32_t* plan = args.plan;
-  const uint32_t pass = args.pass;
-  if (!plan[4 * pass]) return;
-  const uint32_t ord = plan[4 * pass + 1];
-  const bool src_b = plan[4 * pass + 2] != 0;
-  const elem_t* src = static_cast<const elem_t*>(src_b ? args.B : args.A);
-  elem_t* dst = static_cast<elem_t*>(src_b ? args.A : args.B);
-  const size_t slice = (size_t)args.ntiles * NB;
-  uint32_t* st_cur = args.status + (size_t)(ord & 1u) * slice;
-  uint32_t* st_next = args.status + (size_t)((ord + 1u) & 1u) * slice;
-  const uint32_t ntiles = args.ntiles;
-  uint32_t* ticket = &args.tickets[pass];
-
-  if (threadIdx.x == 0) sm.scalar[0] = atomicAdd(ticket, 1u);
-  __syncthreads();
-  uint32_t tile = sm.scalar[0];
-  if (tile >= ntiles) return;
-  elem_t e[ITEMS];
-  tile_load<BLOCK, ITEMS>(e, src, args.n, tile);
-  while (true) {
-    __syncthreads();  // every thread has read scalar[] and finished with sm
-    if (threadIdx.x == 0) sm.scalar[1] = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t next = sm.scalar[1];
-    elem_t en[ITEMS];
-    if (next < ntiles) tile_load<BLOCK, ITEMS>(en, src, args.n, next);
-    onesweep_tile_regs<D, BLOCK, ITEMS>(sm, e, dst, args.n, tile, st_cur, st_next, args.starts,
-                                        args.shift, args.mask);
-    if (next >= ntiles) break;
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) e[i] = en[i];
-    tile = next;
-  }
-}
-
-template <int D, int BLOCK, int ITEMS>
-static void launch_onesweep_pf(const PassArgs& a, hipStream_t s) {
-  static uint32_t grid = 0;
-  if (!grid) {
-    int dev = 0, cus = 0, per = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_onesweep_pf<D, BLOCK, ITEMS>, BLOCK,
-                                                     0) != hipSuccess) {
-      (void)hipGetLastError();
-      cus = 256;
-      per = 1;
-    }
-    grid = (uint32_t)(cus * (per > 0 ? per : 1));
-  }
-  const uint32_t g = a.ntiles < grid ? a.ntiles : grid;
-  hipLaunchKernelGGL((k_onesweep_pf<D, BLOCK, ITEMS>), dim3(g), dim3(BLOCK), 0, s, a);
-}
-
 // Region form: a persistent grid of kRegions groups of blocks; group r takes
 // tickets for the tiles of region r only and runs that region's look-back
 // chain, so kRegions chains advance side by side.  Each region's bucket
@@ -278,7 +214,8 @@ static void launch_onesweep_pf(const PassArgs& a, hipStream_t s) {
 // the histogram kernel for the first one); the pass itself counts, per
 // region of its output, the digits of the next executed pass.
 template <int D, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void k_onesweep_rg(PassArgs args) {
+__global__ __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(PassArgs args) {
   constexpr int NB = 1 << D;
   static_assert(kRegions == kRegionsDev, "region count");
   __shared__ TileSmem<D, BLOCK, ITEMS> sm;
@@ -365,56 +302,22 @@ static void launch_onesweep_rg(const PassArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_onesweep_rg<D, BLOCK, ITEMS>), dim3(pr * kRegions), dim3(BLOCK), 0, s, a);
 }
 
-// Split-staging form (keys, then values, through an 8-byte buffer).
-template <int D, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void k_onesweep_split(PassArgs args) {
-  constexpr int NB = 1 << D;
-  __shared__ TileSmemSplit<D, BLOCK, ITEMS> sm;
-  const uint32_t* plan = args.plan;
-  const uint32_t pass = args.pass;
-  if (!plan[4 * pass]) return;
-  const uint32_t ord = plan[4 * pass + 1];
-  const bool src_b = plan[4 * pass + 2] != 0;
-  const elem_t* src = static_cast<const elem_t*>(src_b ? args.B : args.A);
-  elem_t* dst = static_cast<elem_t*>(src_b ? args.A : args.B);
-  const size_t slice = (size_t)args.ntiles * NB;
-  uint32_t* st_cur = args.status + (size_t)(ord & 1u) * slice;
-  uint32_t* st_next = args.status + (size_t)((ord + 1u) & 1u) * slice;
-  if (threadIdx.x == 0) sm.scalar[0] = atomicAdd(&args.tickets[pass], 1u);
-  __syncthreads();
-  const uint32_t tile = sm.scalar[0];
-  elem_t e[ITEMS];
-  tile_load<BLOCK, ITEMS>(e, src, args.n, tile);
-  onesweep_tile_split<D, BLOCK, ITEMS>(sm, e, dst, args.n, tile, st_cur, st_next, args.starts,
-                                       args.shift, args.mask);
-}
-
-template <int D, int BLOCK, int ITEMS>
-static void launch_onesweep_split(const PassArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_onesweep_split<D, BLOCK, ITEMS>), dim3(a.ntiles), dim3(BLOCK), 0, s, a);
-}
-
 template <int D, int BLOCK, int ITEMS>
 static void launch_onesweep(const PassArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_onesweep<D, BLOCK, ITEMS>), dim3(a.ntiles), dim3(BLOCK), 0, s, a);
 }
 
-// Variants are (digit width, tile, block, persistent-prefetch?).  The
-// first one per digit width is the default for that width.
+// Variants are (digit width, tile, block, form, launcher); form 3 is the
+// region form, 0 the one-tile-per-block form.  The first one per digit
+// width is the fallback for that width when no tuned choice applies.
 static const SortVariant kVariants[] = {
+    {8, 512 * 16, 512, 3, &launch_onesweep_rg<8, 512, 16>},
     {8, 256 * 16, 256, 3, &launch_onesweep_rg<8, 256, 16>},
-    {8, 256 * 12, 256, 3, &launch_onesweep_rg<8, 256, 12>},
+    {8, 256 * 8, 256, 3, &launch_onesweep_rg<8, 256, 8>},
     {8, 256 * 16, 256, 0, &launch_onesweep<8, 256, 16>},
-    {8, 256 * 12, 256, 0, &launch_onesweep<8, 256, 12>},
     {8, 512 * 16, 512, 0, &launch_onesweep<8, 512, 16>},
     {8, 256 * 8, 256, 0, &launch_onesweep<8, 256, 8>},
     {11, 512 * 12, 512, 0, &launch_onesweep<11, 512, 12>},
-    {8, 256 * 16, 256, 2, &launch_onesweep_split<8, 256, 16>},
-    {8, 512 * 12, 512, 2, &launch_onesweep_split<8, 512, 12>},
-    {8, 256 * 12, 256, 2, &launch_onesweep_split<8, 256, 12>},
-    {8, 256 * 12, 256, 1, &launch_onesweep_pf<8, 256, 12>},
-    {8, 256 * 8, 256, 1, &launch_onesweep_pf<8, 256, 8>},
-    {11, 512 * 8, 512, 1, &launch_onesweep_pf<11, 512, 8>},
 };
 
 const SortVariant* sort_variants(int* count) {

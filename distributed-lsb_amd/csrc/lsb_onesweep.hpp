@@ -20,13 +20,18 @@ constexpr int kRegionsDev = 8;
 // Ranks this wave's ITEMS rows into its LDS counters wh[0 .. 2^D).  Row i,
 // lane l is valid when i*64 + l < lim.  For every valid element: dg = its
 // digit, rk = number of elements of the same digit earlier in the wave's
-// run.  Peers (lanes holding the same digit) are found with D ballots; the
-// lowest peer bumps the counter, the others read the old value.
+// run.  Peers (lanes holding the same digit) are found with D ballots.  The
+// lowest peer of each group adds the group's size to the counter with one
+// returning LDS atomic; the rows' atomics go out back to back (LDS
+// operations of one wave complete in order, so row i+1 sees row i's adds),
+// and each lane then fetches its group leader's old count with
+// ds_bpermute.  No row waits on the previous row's LDS round trip.
 template <int D, int ITEMS>
 __device__ __forceinline__ void wave_rank(const elem_t (&e)[ITEMS], uint32_t lim, uint32_t shift,
                                           uint32_t mask, uint32_t* wh, uint32_t (&dg)[ITEMS],
                                           uint32_t (&rk)[ITEMS]) {
   const uint32_t lane = lane_id();
+  uint32_t old[ITEMS], lead[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const bool valid = (uint32_t)i * kWave + lane < lim;
@@ -45,12 +50,15 @@ __device__ __forceinline__ void wave_rank(const elem_t (&e)[ITEMS], uint32_t lim
     }
     const uint64_t peers = ((uint64_t)phi << 32) | plo;
     const uint32_t below = count_below(peers);
-    uint32_t old = 0;
-    if (valid) old = wh[d];
-    if (valid && below == 0) wh[d] = old + (uint32_t)__popcll(peers);
+    old[i] = 0;
+    if (valid && below == 0) old[i] = atomicAdd(&wh[d], (uint32_t)__popcll(peers));
+    lead[i] = peers ? (uint32_t)__builtin_ctzll(peers) : lane;
     dg[i] = d;
-    rk[i] = old + below;
+    rk[i] = below;
   }
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    rk[i] += (uint32_t)__builtin_amdgcn_ds_bpermute((int)(lead[i] << 2), (int)old[i]);
 }
 
 // ------------------------------------------------------------- look-back
@@ -260,129 +268,57 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
   }
 }
 
-// ------------------------------------------------- split-staging variant
-// Same pass with half the LDS: the tile is reordered through an 8-byte-per-
-// element buffer twice, first the keys, then the values.  Each thread keeps
-// the destination of the slots it stores in registers between the halves,
-// so the value half needs no digit.  Halving the buffer lets more tiles be
-// resident per CU (each tile spends most of its life loading, ranking and
-// looking back, when the buffer is idle); the two 8-byte stores of an
-// element meet again in L2 before the line goes to HBM.
-template <int D, int BLOCK, int ITEMS>
-struct TileSmemSplit {
-  static constexpr int NB = 1 << D;
-  static constexpr int W = BLOCK / kWave;
-  static constexpr int T = BLOCK * ITEMS;
-  static_assert(W * NB * 4 <= T * 8, "per-wave counters must fit in the tile buffer");
-  uint64_t half[T];
-  uint32_t start[NB];
-  uint32_t gbase[NB];
-  uint32_t wsum[W];
-  uint32_t scalar[4];
-};
-
-template <int D, int BLOCK, int ITEMS>
-__device__ __forceinline__ void onesweep_tile_split(TileSmemSplit<D, BLOCK, ITEMS>& sm,
-                                                    const elem_t (&e)[ITEMS],
-                                                    elem_t* __restrict__ dst, uint32_t n,
-                                                    uint32_t tile, uint32_t* st_cur,
-                                                    uint32_t* st_next,
-                                                    const uint32_t* __restrict__ starts,
-                                                    uint32_t shift, uint32_t mask) {
-  using S = TileSmemSplit<D, BLOCK, ITEMS>;
-  constexpr int NB = S::NB, W = S::W, T = S::T;
-  constexpr int DPT = (NB + BLOCK - 1) / BLOCK;
+// ------------------------------------------------------------ store_run
+// Writes a tile held in LDS in digit order (slot j of bucket d goes to
+// dst[gbase[d] + j]).  With COUNT, every element is also counted in hnext
+// by the region its destination falls in (rsplit[d] = (first slot of the
+// next region << 3) | region) and by its digit for the next executed pass.
+// A full tile takes a branch-free path, so the LDS reads of all ITEMS rows
+// are in flight together instead of one row's round trips at a time.
+template <int BLOCK, int ITEMS, int NB, bool COUNT>
+__device__ __forceinline__ void store_run(const elem_t* elems, const uint32_t* gbase,
+                                          const uint32_t* rsplit, uint32_t* hnext,
+                                          uint32_t tile_n, elem_t* __restrict__ dst,
+                                          uint32_t shift, uint32_t mask, uint32_t nshift,
+                                          uint32_t nmask) {
   const uint32_t t = threadIdx.x;
-  const uint32_t lane = t & (kWave - 1);
-  const uint32_t wave = t / kWave;
-  uint32_t* whist = reinterpret_cast<uint32_t*>(sm.half);
-
-  for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
-  for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) st_next[(size_t)tile * NB + b] = 0u;
-  const uint32_t tile_base = tile * (uint32_t)T;
-  const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
-  const uint32_t wrun = wave * (kWave * ITEMS);
-  const uint32_t lim = tile_n > wrun ? tile_n - wrun : 0u;
-  lds_barrier();
-
-  uint32_t dg[ITEMS], rk[ITEMS];
-  uint32_t* wh = whist + wave * NB;
-  wave_rank<D, ITEMS>(e, lim, shift, mask, wh, dg, rk);
-  lds_barrier();
-
-  uint32_t tot[DPT], gstart[DPT];
-  uint32_t lbw[DPT][kLookback];
+  if (tile_n == (uint32_t)(BLOCK * ITEMS)) {
+    elem_t x[ITEMS];
+    uint32_t d[ITEMS], g[ITEMS];
 #pragma unroll
-  for (int k = 0; k < DPT; ++k) {
-    const uint32_t b = t + k * BLOCK;
-    tot[k] = 0;
-    gstart[k] = 0;
-    if (b < (uint32_t)NB) {
-      gstart[k] = starts[b];
-      uint32_t s = 0;
+    for (int k = 0; k < ITEMS; ++k) x[k] = elems[k * BLOCK + t];
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        const uint32_t c = whist[w * NB + b];
-        whist[w * NB + b] = s;
-        s += c;
-      }
-      tot[k] = s;
-      sm.start[b] = s;
-      uint32_t* mine = st_cur + (size_t)tile * NB + b;
-      if (tile == 0) {
-        st_status(mine, (s + 1u) | kFlagInclusive);
-      } else {
-        st_status(mine, s + 1u);
-        lookback_issue(st_cur, tile, NB, b, lbw[k]);
+    for (int k = 0; k < ITEMS; ++k) d[k] = digit_of(x[k].key, shift, mask);
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) g[k] = gbase[d[k]];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) dst[(size_t)(g[k] + (uint32_t)(k * BLOCK) + t)] = x[k];
+    if (COUNT) {
+      uint32_t info[ITEMS];
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) info[k] = rsplit[d[k]];
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t j = k * BLOCK + t;
+        const uint32_t r = (info[k] & 7u) + (j >= (info[k] >> 3) ? 1u : 0u);
+        atomicAdd(&hnext[r * NB + digit_of(x[k].key, nshift, nmask)], 1u);
       }
     }
+    return;
   }
-  block_exclusive_scan<NB, BLOCK>(sm.start, sm.wsum);
-
-  uint32_t pos[ITEMS];
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) pos[i] = sm.start[dg[i]] + wh[dg[i]] + rk[i];
-  lds_barrier();  // counters dead: the buffer takes the keys
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i)
-    if ((uint32_t)i * kWave + lane < lim) sm.half[pos[i]] = e[i].key;
-
-#pragma unroll
-  for (int k = 0; k < DPT; ++k) {
-    const uint32_t b = t + k * BLOCK;
-    if (b < (uint32_t)NB) {
-      uint32_t prefix = 0;
-      if (tile > 0) {
-        prefix = lookback_finish(st_cur, tile, NB, b, lbw[k]);
-        st_status(st_cur + (size_t)tile * NB + b, (prefix + tot[k] + 1u) | kFlagInclusive);
-      }
-      sm.gbase[b] = gstart[k] + prefix - sm.start[b];
-    }
-  }
-  lds_barrier();
-
-  // keys out; remember where each of this thread's slots went
-  uint32_t dest[ITEMS];
-  uint64_t* out = reinterpret_cast<uint64_t*>(dst);
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const uint32_t j = k * BLOCK + t;
-    dest[k] = 0;
     if (j < tile_n) {
-      const uint64_t key = sm.half[j];
-      dest[k] = sm.gbase[digit_of(key, shift, mask)] + j;
-      out[2 * (size_t)dest[k]] = key;
+      const elem_t x = elems[j];
+      const uint32_t d = digit_of(x.key, shift, mask);
+      dst[(size_t)(gbase[d] + j)] = x;
+      if (COUNT) {
+        const uint32_t info = rsplit[d];
+        const uint32_t r = (info & 7u) + (j >= (info >> 3) ? 1u : 0u);
+        atomicAdd(&hnext[r * NB + digit_of(x.key, nshift, nmask)], 1u);
+      }
     }
-  }
-  lds_barrier();  // keys read: the buffer takes the values
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i)
-    if ((uint32_t)i * kWave + lane < lim) sm.half[pos[i]] = e[i].val;
-  lds_barrier();
-#pragma unroll
-  for (int k = 0; k < ITEMS; ++k) {
-    const uint32_t j = k * BLOCK + t;
-    if (j < tile_n) out[2 * (size_t)dest[k] + 1] = sm.half[j];
   }
 }
 
@@ -489,21 +425,12 @@ __device__ __forceinline__ void onesweep_tile_region(
 
   uint32_t next;
   if (t == 0) next = atomicAdd(ticket, 1u);
-#pragma unroll
-  for (int k = 0; k < ITEMS; ++k) {
-    const uint32_t j = k * BLOCK + t;
-    if (j < tile_n) {
-      const elem_t x = sm.elems[j];
-      const uint32_t d = digit_of(x.key, shift, mask);
-      const uint32_t dest = sm.gbase[d] + j;
-      dst[(size_t)dest] = x;
-      if (hnext) {
-        const uint32_t info = sm.rsplit[d];
-        const uint32_t r = (info & 7u) + (j >= (info >> 3) ? 1u : 0u);
-        atomicAdd(&hnext[r * NB + digit_of(x.key, nshift, nmask)], 1u);
-      }
-    }
-  }
+  if (hnext)
+    store_run<BLOCK, ITEMS, NB, true>(sm.elems, sm.gbase, sm.rsplit, hnext, tile_n, dst, shift,
+                                      mask, nshift, nmask);
+  else
+    store_run<BLOCK, ITEMS, NB, false>(sm.elems, sm.gbase, sm.rsplit, hnext, tile_n, dst, shift,
+                                       mask, nshift, nmask);
   if (t == 0) sm.scalar[0] = next;
 }
 

@@ -135,6 +135,15 @@ def lib():
             raise RuntimeError(
                 f"liblsb_amd.so not built at {LIB_PATH}: run `make -C distributed-lsb_amd` "
                 "or __graft_entry__.build(); there is no fallback path")
+        # PyTorch-ROCm ships its own libamdhip64 and links it by file name,
+        # so a process that maps /opt/rocm's copy first (our NEEDED entry)
+        # and imports torch afterwards ends up with two HIP runtimes, and
+        # allocations fail.  Loading torch first makes this library bind to
+        # torch's runtime through the shared soname.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -116,7 +116,9 @@ typedef struct lsb_tune_entry {
  * lsb_sort_segmented (digit_bits = tile = variant = 0).  Needs 48 bytes x
  * 2^hi_log2 of free device memory while it runs.  Synchronous. */
 int lsb_autotune(uint32_t lo_log2, uint32_t hi_log2, uint32_t key_bits, void* stream);
-/* Copies up to cap entries of the current table; returns the entry count. */
+/* Copies up to cap entries of the current table; returns the entry count.
+ * While it is empty (never tuned, or after lsb_tune_clear) the automatic
+ * choice follows a built-in table: this autotuner's output on one MI355X. */
 int lsb_tune_table(lsb_tune_entry* out, int cap);
 /* Saves / loads / clears the table (text file, one size class per line). */
 int lsb_tune_save(const char* path);

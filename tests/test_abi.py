@@ -140,22 +140,29 @@ def test_server_rejects_bad_config_without_gpu():
 
 def test_tune_table_drives_auto_choice(tmp_path):
     """A loaded tune table picks the automatic variant per size class and
-    key width (nearest class wins); clearing it restores the default."""
+    key width (nearest class wins); clearing it restores the built-in table
+    (the autotuner's MI355X output: 2048-element tiles for small arrays,
+    8192 for large ones)."""
     lsb_amd.tune_clear()
+    assert lsb_amd.tune_table() == []
     default = lsb_amd.plan(1 << 20)
+    assert lsb_amd.plan(1 << 16)[:2] == (8, 2048)
+    assert lsb_amd.plan(1 << 22)[:2] == (8, 4096)
+    assert lsb_amd.plan(1 << 30)[:2] == (8, 8192)
+    assert lsb_amd.plan(1 << 30, key_bits=32)[:2] == (8, 8192)
     p = tmp_path / "tune.txt"
     p.write_text("# comment\n"
                  "20 64 8 2048 0 1.0 2.0\n"
-                 "26 64 8 8192 0 9.0 9.5\n"
-                 "20 32 8 3072 1 0.5 0.6\n"
+                 "26 64 8 4096 3 9.0 9.5\n"
+                 "20 32 8 8192 3 0.5 0.6\n"
                  "22 64 13 999 0 1.0 1.0\n")        # no such variant: dropped
     lsb_amd.tune_load(str(p))
     t = lsb_amd.tune_table()
     assert [e["log2n"] for e in t] == [20, 26, 20]
     assert lsb_amd.plan(1 << 20)[:2] == (8, 2048)
     assert lsb_amd.plan(1 << 19)[:2] == (8, 2048)
-    assert lsb_amd.plan(1 << 27)[:2] == (8, 8192)
-    assert lsb_amd.plan(1 << 20, key_bits=32)[:2] == (8, 3072)
+    assert lsb_amd.plan(1 << 27)[:2] == (8, 4096)
+    assert lsb_amd.plan(1 << 20, key_bits=32)[:2] == (8, 8192)
     # explicit settings still win over the table
     assert lsb_amd.plan(1 << 20, digit_bits=8, tile=4096)[:2] == (8, 4096)
     out = tmp_path / "saved.txt"
