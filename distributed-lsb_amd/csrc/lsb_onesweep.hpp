@@ -43,7 +43,10 @@ __device__ __forceinline__ void wave_rank(const elem_t (&e)[ITEMS], uint32_t lim
     uint32_t plo = (uint32_t)vmask, phi = (uint32_t)(vmask >> 32);
 #pragma unroll
     for (int b = 0; b < D; ++b) {
-      const uint32_t c = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
+      // A pure VALU op in asm: the compiler would otherwise rewrite the
+      // field extract as two shifts and test the sign of the first.
+      uint32_t c;
+      asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(c) : "v"(d), "i"(b));
       const uint64_t bal = __ballot(c != 0u);
       plo = __builtin_amdgcn_bitop3_b32(plo, (uint32_t)bal, c, 0x90);
       phi = __builtin_amdgcn_bitop3_b32(phi, (uint32_t)(bal >> 32), c, 0x90);
