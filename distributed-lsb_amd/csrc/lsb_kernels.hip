@@ -40,21 +40,32 @@ void launch_zero(uint32_t* p, size_t words, hipStream_t s) {
 // The array is cut into kRegions equal regions of `region_elems` elements
 // (whole tiles); the scatter passes give each region its own look-back
 // chain.  Each block owns one contiguous chunk inside one region and adds
-// its counts for every digit into regcount[p][region][b].
-template <int D>
+// its counts for every digit into regcount[p][region][b].  P (the digit
+// count) is a template parameter so the per-key digit loop unrolls; full
+// rounds of 256 x U keys take hist_add_full, the chunk's tail the checked
+// hist_add.
+template <int D, int P>
 __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A, uint32_t n,
-                                                   uint32_t key_bits, uint32_t passes,
+                                                   uint32_t key_bits,
                                                    uint32_t* __restrict__ regcount,
                                                    uint32_t region_elems,
                                                    uint32_t blocks_per_region,
                                                    uint32_t* __restrict__ zero,
                                                    size_t zero_words) {
   constexpr int NB = 1 << D;
-  constexpr int MAXP = (64 + D - 1) / D;
-  constexpr int U = 4;  // keys in flight per thread
-  __shared__ uint32_t h[MAXP * NB];
+  constexpr int U = 8;  // keys in flight per thread
+  __shared__ uint32_t h[P * NB];
+  __shared__ uint32_t spare[kWave];
   const uint32_t t = threadIdx.x;
-  for (uint32_t i = t; i < passes * NB; i += 256) h[i] = 0u;
+  for (uint32_t i = t; i < (uint32_t)(P * NB); i += 256) h[i] = 0u;
+  uint32_t* spare_lane = spare + lane_id();
+  const uint32_t lane0_64 = lane_id() == 0 ? (uint32_t)kWave : 0u;
+  uint32_t mask[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const uint32_t w = key_bits - p * D < (uint32_t)D ? key_bits - p * D : (uint32_t)D;
+    mask[p] = (1u << w) - 1u;
+  }
 
   // Clear the first look-back slice of the scatter passes (16-byte stores).
   {
@@ -72,32 +83,40 @@ __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A,
   const size_t per = (region_elems + blocks_per_region - 1) / blocks_per_region;
   const size_t c0 = r0 + (size_t)chunk * per;
   const size_t c1 = c0 + per < r1 ? c0 + per : r1;
-  for (size_t base = c0; base < c1; base += 256 * U) {
+  size_t base = c0;
+  for (; base + 256 * U <= c1; base += 256 * U) {
     uint64_t k[U];
-    bool v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) k[u] = A[base + (size_t)u * 256 + t].key;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const size_t i = base + (size_t)u * 256 + t;
-      v[u] = i < c1;
-      k[u] = v[u] ? A[i].key : 0ull;
-    }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      for (uint32_t p = 0; p < passes; ++p) {
-        const uint32_t shift = p * D;
-        const uint32_t w = key_bits - shift < (uint32_t)D ? key_bits - shift : (uint32_t)D;
-        const uint32_t dg = digit_of(k[u], shift, (1u << w) - 1u);
-        hist_add(h + p * NB, dg, v[u]);
-      }
+      for (int p = 0; p < P; ++p)
+        hist_add_full(h + p * NB, digit_of(k[u], p * D, mask[p]), spare_lane, lane0_64);
     }
+  }
+  for (; base < c1; base += 256) {
+    const size_t i = base + t;
+    const bool v = i < c1;
+    const uint64_t k = v ? A[i].key : 0ull;
+#pragma unroll
+    for (int p = 0; p < P; ++p) hist_add(h + p * NB, digit_of(k, p * D, mask[p]), v);
   }
   __syncthreads();
   if (c0 >= c1) return;
-  for (uint32_t i = t; i < passes * NB; i += 256) {
+  for (uint32_t i = t; i < (uint32_t)(P * NB); i += 256) {
     const uint32_t c = h[i];
     const uint32_t p = i / NB, b = i % NB;
     if (c) atomicAdd(&regcount[((size_t)p * kRegions + region) * NB + b], c);
   }
+}
+
+template <int D, int P>
+static void launch_hist(const elem_t* a, uint32_t n, uint32_t key_bits, uint32_t* regcount,
+                        uint32_t region_elems, uint32_t per_region, uint32_t* zero,
+                        size_t zero_words, hipStream_t s) {
+  hipLaunchKernelGGL((k_histogram<D, P>), dim3(per_region * kRegions), dim3(256), 0, s, a, n,
+                     key_bits, regcount, region_elems, per_region, zero, zero_words);
 }
 
 hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
@@ -105,20 +124,30 @@ hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_
                             uint32_t* zero, size_t zero_words, hipStream_t s) {
   // Enough blocks to keep every CU streaming, few enough that the final
   // flush of LDS counters stays small.
-  size_t per_region = ((size_t)region_elems + 256 * 4 * 8 - 1) / (256 * 4 * 8);
+  size_t per_region = ((size_t)region_elems + 256 * 8 * 8 - 1) / (256 * 8 * 8);
   if (per_region < 1) per_region = 1;
   if (per_region > 256) per_region = 256;
-  const size_t blocks = per_region * kRegions;
+  const uint32_t pr = (uint32_t)per_region;
   const elem_t* a = static_cast<const elem_t*>(A);
-  if (d == 8)
-    hipLaunchKernelGGL(k_histogram<8>, dim3((unsigned)blocks), dim3(256), 0, s, a, n, key_bits,
-                       passes, regcount, region_elems, (uint32_t)per_region, zero, zero_words);
-  else if (d == 11)
-    hipLaunchKernelGGL(k_histogram<11>, dim3((unsigned)blocks), dim3(256), 0, s, a, n, key_bits,
-                       passes, regcount, region_elems, (uint32_t)per_region, zero, zero_words);
-  else
+#define LSB_HIST_CASE(DD, PP) \
+  case PP: launch_hist<DD, PP>(a, n, key_bits, regcount, region_elems, pr, zero, zero_words, s); break;
+  if (d == 8) {
+    switch (passes) {
+      LSB_HIST_CASE(8, 1) LSB_HIST_CASE(8, 2) LSB_HIST_CASE(8, 3) LSB_HIST_CASE(8, 4)
+      LSB_HIST_CASE(8, 5) LSB_HIST_CASE(8, 6) LSB_HIST_CASE(8, 7) LSB_HIST_CASE(8, 8)
+      default: return hipErrorInvalidValue;
+    }
+  } else if (d == 11) {
+    switch (passes) {
+      LSB_HIST_CASE(11, 1) LSB_HIST_CASE(11, 2) LSB_HIST_CASE(11, 3)
+      LSB_HIST_CASE(11, 4) LSB_HIST_CASE(11, 5) LSB_HIST_CASE(11, 6)
+      default: return hipErrorInvalidValue;
+    }
+  } else {
     return hipErrorInvalidValue;
-  return hipSuccess;
+  }
+#undef LSB_HIST_CASE
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------ k_scan_plan
