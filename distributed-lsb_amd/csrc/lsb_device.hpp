@@ -66,9 +66,12 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
 
 // In-place exclusive scan of a[0..N) in LDS by a BLOCK-thread workgroup.
 // Every thread of the block must call it; it begins and ends with a barrier
-// so callers may write a[] right before and read it right after.
-template <int N, int BLOCK>
-__device__ __forceinline__ void block_exclusive_scan(uint32_t* a, uint32_t* wave_sums) {
+// so callers may write a[] right before and read it right after.  With
+// ROWS > 0 the result is also added into ROWS rows of N words at `rows`
+// (rows[r*N + i] += scan[i]) before the closing barrier.
+template <int N, int BLOCK, int ROWS = 0>
+__device__ __forceinline__ void block_exclusive_scan(uint32_t* a, uint32_t* wave_sums,
+                                                     uint32_t* rows = nullptr) {
   constexpr int PER = (N + BLOCK - 1) / BLOCK;
   constexpr int W = BLOCK / kWave;
   const int t = threadIdx.x;
@@ -92,7 +95,11 @@ __device__ __forceinline__ void block_exclusive_scan(uint32_t* a, uint32_t* wave
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int i = t * PER + k;
-    if (i < N) a[i] = run;
+    if (i < N) {
+      a[i] = run;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) rows[r * N + i] += run;
+    }
     run += v[k];
   }
   lds_barrier();

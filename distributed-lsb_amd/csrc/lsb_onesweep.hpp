@@ -136,8 +136,10 @@ struct TileSmem {
   static_assert(W * NB * 4 <= T * 16, "per-wave counters must fit in the tile buffer");
   elem_t elems[T];
   uint32_t start[NB];  // tile-local exclusive bucket starts
-  uint32_t gbase[NB];  // destination index of tile-local slot 0, per bucket
-  uint32_t rsplit[NB];  // region form: (first slot in the next region << 3) | region
+  // per bucket: x = destination index of tile-local slot 0; y (region form
+  // only) = (first slot in the next region << 3) | region.  Interleaved so
+  // the store phase reads both with one 8-byte LDS load.
+  uint2 grs[NB];
   uint32_t wsum[W];
   uint32_t scalar[4];
 };
@@ -253,7 +255,7 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
         prefix = lookback_finish(st_cur, tile, NB, b, lbw[k]);
         st_status(st_cur + (size_t)tile * NB + b, (prefix + tot[k] + 1u) | kFlagInclusive);
       }
-      sm.gbase[b] = gstart[k] + prefix - sm.start[b];
+      sm.grs[b].x = gstart[k] + prefix - sm.start[b];
     }
   }
   lds_barrier();
@@ -266,44 +268,41 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
     if (j < tile_n) {
       const elem_t x = sm.elems[j];
       const uint32_t d = digit_of(x.key, shift, mask);
-      dst[(size_t)(uint32_t)(sm.gbase[d] + j)] = x;
+      dst[(size_t)(uint32_t)(sm.grs[d].x + j)] = x;
     }
   }
 }
 
 // ------------------------------------------------------------ store_run
 // Writes a tile held in LDS in digit order (slot j of bucket d goes to
-// dst[gbase[d] + j]).  With COUNT, every element is also counted in hnext
-// by the region its destination falls in (rsplit[d] = (first slot of the
+// dst[grs[d].x + j]).  With COUNT, every element is also counted in hnext
+// by the region its destination falls in (grs[d].y = (first slot of the
 // next region << 3) | region) and by its digit for the next executed pass.
 // A full tile takes a branch-free path, so the LDS reads of all ITEMS rows
 // are in flight together instead of one row's round trips at a time.
 template <int BLOCK, int ITEMS, int NB, bool COUNT>
-__device__ __forceinline__ void store_run(const elem_t* elems, const uint32_t* gbase,
-                                          const uint32_t* rsplit, uint32_t* hnext,
+__device__ __forceinline__ void store_run(const elem_t* elems, const uint2* grs, uint32_t* hnext,
                                           uint32_t tile_n, elem_t* __restrict__ dst,
                                           uint32_t shift, uint32_t mask, uint32_t nshift,
                                           uint32_t nmask) {
   const uint32_t t = threadIdx.x;
   if (tile_n == (uint32_t)(BLOCK * ITEMS)) {
     elem_t x[ITEMS];
-    uint32_t d[ITEMS], g[ITEMS];
+    uint32_t d[ITEMS];
+    uint2 g[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) x[k] = elems[k * BLOCK + t];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) d[k] = digit_of(x[k].key, shift, mask);
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) g[k] = gbase[d[k]];
+    for (int k = 0; k < ITEMS; ++k) g[k] = grs[d[k]];
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) dst[(size_t)(g[k] + (uint32_t)(k * BLOCK) + t)] = x[k];
+    for (int k = 0; k < ITEMS; ++k) dst[(size_t)(g[k].x + (uint32_t)(k * BLOCK) + t)] = x[k];
     if (COUNT) {
-      uint32_t info[ITEMS];
-#pragma unroll
-      for (int k = 0; k < ITEMS; ++k) info[k] = rsplit[d[k]];
 #pragma unroll
       for (int k = 0; k < ITEMS; ++k) {
         const uint32_t j = k * BLOCK + t;
-        const uint32_t r = (info[k] & 7u) + (j >= (info[k] >> 3) ? 1u : 0u);
+        const uint32_t r = (g[k].y & 7u) + (j >= (g[k].y >> 3) ? 1u : 0u);
         atomicAdd(&hnext[r * NB + digit_of(x[k].key, nshift, nmask)], 1u);
       }
     }
@@ -315,10 +314,10 @@ __device__ __forceinline__ void store_run(const elem_t* elems, const uint32_t* g
     if (j < tile_n) {
       const elem_t x = elems[j];
       const uint32_t d = digit_of(x.key, shift, mask);
-      dst[(size_t)(gbase[d] + j)] = x;
+      const uint2 g = grs[d];
+      dst[(size_t)(g.x + j)] = x;
       if (COUNT) {
-        const uint32_t info = rsplit[d];
-        const uint32_t r = (info & 7u) + (j >= (info >> 3) ? 1u : 0u);
+        const uint32_t r = (g.y & 7u) + (j >= (g.y >> 3) ? 1u : 0u);
         atomicAdd(&hnext[r * NB + digit_of(x.key, nshift, nmask)], 1u);
       }
     }
@@ -392,11 +391,13 @@ __device__ __forceinline__ void onesweep_tile_region(
       }
     }
   }
-  block_exclusive_scan<NB, BLOCK>(sm.start, sm.wsum);
+  // The scan also adds each bucket's tile start into every wave's row, so
+  // an element's slot is one LDS read: wh[digit] + rank.
+  block_exclusive_scan<NB, BLOCK, W>(sm.start, sm.wsum, whist);
 
   uint32_t pos[ITEMS];
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) pos[i] = sm.start[dg[i]] + wh[dg[i]] + rk[i];
+  for (int i = 0; i < ITEMS; ++i) pos[i] = wh[dg[i]] + rk[i];
   lds_barrier();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
@@ -415,13 +416,15 @@ __device__ __forceinline__ void onesweep_tile_region(
       // so it crosses at most one region boundary: note its first region and
       // the tile slot where the next region begins.
       const uint32_t first = gstart[k] + prefix;
-      sm.gbase[b] = first - sm.start[b];
+      const uint32_t st = sm.start[b];
+      uint32_t info = 0;
       if (hnext) {
         uint32_t r0 = 0;
 #pragma unroll
         for (int q = 1; q < kRegionsDev; ++q) r0 += first >= (uint32_t)q * region_elems ? 1u : 0u;
-        sm.rsplit[b] = ((sm.start[b] + (r0 + 1u) * region_elems - first) << 3) | r0;
+        info = ((st + (r0 + 1u) * region_elems - first) << 3) | r0;
       }
+      sm.grs[b] = make_uint2(first - st, info);
     }
   }
   lds_barrier();
@@ -429,11 +432,11 @@ __device__ __forceinline__ void onesweep_tile_region(
   uint32_t next;
   if (t == 0) next = atomicAdd(ticket, 1u);
   if (hnext)
-    store_run<BLOCK, ITEMS, NB, true>(sm.elems, sm.gbase, sm.rsplit, hnext, tile_n, dst, shift,
-                                      mask, nshift, nmask);
+    store_run<BLOCK, ITEMS, NB, true>(sm.elems, sm.grs, hnext, tile_n, dst, shift, mask, nshift,
+                                      nmask);
   else
-    store_run<BLOCK, ITEMS, NB, false>(sm.elems, sm.gbase, sm.rsplit, hnext, tile_n, dst, shift,
-                                       mask, nshift, nmask);
+    store_run<BLOCK, ITEMS, NB, false>(sm.elems, sm.grs, hnext, tile_n, dst, shift, mask, nshift,
+                                       nmask);
   if (t == 0) sm.scalar[0] = next;
 }
 
