@@ -42,6 +42,7 @@ inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 struct Plan {
   uint32_t key_bits, d, tile, passes, nb, ntiles, region_tiles;
+  lsb::KeyOrder ko;
   const lsb::SortVariant* v;
   // workspace offsets
   size_t counts, regnext, starts, plan, tickets, status, total;
@@ -52,6 +53,20 @@ lsb_config defaults() {
   memset(&c, 0, sizeof c);
   c.key_bits = 64;
   return c;
+}
+
+// Key order from LSB_KEYS_SIGNED / LSB_KEYS_FLOAT (at most one; floats are
+// binary32 or binary64, so key_bits 32 or 64).
+int make_key_order(uint32_t flags, uint32_t kb, lsb::KeyOrder* ko) {
+  const bool sgn = (flags & LSB_KEYS_SIGNED) != 0, flt = (flags & LSB_KEYS_FLOAT) != 0;
+  ko->sign = ko->fsign = ko->full = 0;
+  if (sgn && flt) return LSB_EINVAL;
+  if (flt && kb != 32 && kb != 64) return LSB_EINVAL;
+  if (!sgn && !flt) return LSB_OK;
+  ko->sign = 1ull << (kb - 1);
+  ko->full = kb >= 64 ? ~0ull : ((1ull << kb) - 1);
+  ko->fsign = flt ? ko->sign : 0;
+  return LSB_OK;
 }
 
 // Tuned table (lsb_autotune / lsb_tune_load), guarded by g_tune_mu.
@@ -146,6 +161,7 @@ void auto_choice(size_t n, uint32_t key_bits, uint32_t* d, uint32_t* tile, uint3
 int make_plan(size_t n, const lsb_config& c, Plan* p) {
   p->key_bits = c.key_bits ? c.key_bits : 64;
   if (p->key_bits > 64) return LSB_EINVAL;
+  if (make_key_order(c.flags, p->key_bits, &p->ko)) return LSB_EINVAL;
   uint32_t d = c.digit_bits, tile = c.tile, variant = c.variant;
   if (d == 0 && tile == 0 && variant == 0) {
     auto_choice(n, p->key_bits, &d, &tile, &variant);
@@ -301,7 +317,7 @@ int enqueue_sort(lsb_elem* A, lsb_elem* B, uint32_t n, const Plan& p, char* ws,
   }
   if ((rc = prof_mark(prof, 0, s))) return rc;
   lsb::launch_zero(counts, (size_t)p.passes * lsb::kRegions * p.nb, s);
-  LSB_HIP(lsb::launch_histogram(p.d, A, n, p.key_bits, p.passes, counts,
+  LSB_HIP(lsb::launch_histogram(p.d, A, n, p.key_bits, p.ko, p.passes, counts,
                                 p.region_tiles * p.tile, status, slice, s));
   if ((rc = prof_mark(prof, 1, s))) return rc;
   LSB_HIP(lsb::launch_scan_plan(p.d, counts, starts, plan, tickets, regnext, n, p.passes, flags,
@@ -323,6 +339,7 @@ int enqueue_sort(lsb_elem* A, lsb_elem* B, uint32_t n, const Plan& p, char* ws,
     a.ntiles = p.ntiles;
     a.passes = p.passes;
     a.key_bits = p.key_bits;
+    a.ko = p.ko;
     a.region_tiles = p.region_tiles;
     a.regcount = counts;
     a.regnext = regnext;
@@ -395,6 +412,7 @@ int launch_graph(const std::vector<uint64_t>& key, hipStream_t s, Enqueue enqueu
 struct SegLayout {
   const lsb::SegVariant* v;
   uint32_t key_bits, d, tile, passes, maxp, nb;
+  lsb::KeyOrder ko;
   size_t cap_n;
   uint32_t cap_segs, max_tiles;
   size_t hdr, meta, counts, plan, tickets, status, total;
@@ -404,6 +422,7 @@ int make_seg_layout(size_t cap_n, uint32_t cap_segs, const lsb_config& c, SegLay
                     size_t typical_n = 0) {
   L->key_bits = c.key_bits ? c.key_bits : 64;
   if (L->key_bits > 64) return LSB_EINVAL;
+  if (make_key_order(c.flags, L->key_bits, &L->ko)) return LSB_EINVAL;
   uint32_t d = c.digit_bits ? c.digit_bits : 8;
   uint32_t want_tile = c.tile;
   if (!c.digit_bits && !c.tile && typical_n) {
@@ -489,9 +508,9 @@ int enqueue_segmented(lsb_elem* A, lsb_elem* B, const SegLayout& L, char* ws, ui
   uint32_t* tickets = reinterpret_cast<uint32_t*>(ws + L.tickets);
   uint32_t* status = reinterpret_cast<uint32_t*>(ws + L.status);
   const uint32_t cus = device_cus();
-  LSB_HIP(lsb::launch_seg_prepare(L.d, A, hdr, large, L.tile, L.key_bits, L.passes, counts, plan,
-                                  tickets, status, flags, 4 * cus, s));
-  LSB_HIP(lsb::launch_seg_small(A, hdr, small, L.key_bits, 4 * cus, s));
+  LSB_HIP(lsb::launch_seg_prepare(L.d, A, hdr, large, L.tile, L.key_bits, L.ko, L.passes, counts,
+                                  plan, tickets, status, flags, 4 * cus, s));
+  LSB_HIP(lsb::launch_seg_small(A, hdr, small, L.key_bits, L.ko, 4 * cus, s));
   const uint32_t grid = L.v->blocks_per_cu() * cus;
   for (uint32_t q = 0; q < L.passes; ++q) {
     lsb::SegPassArgs a;
@@ -508,6 +527,7 @@ int enqueue_segmented(lsb_elem* A, lsb_elem* B, const SegLayout& L, char* ws, ui
     a.shift = q * L.d;
     const uint32_t w = L.key_bits - a.shift < L.d ? L.key_bits - a.shift : L.d;
     a.mask = (1u << w) - 1u;
+    a.ko = L.ko;
     LSB_HIP(L.v->scatter(a, grid, s));
   }
   LSB_HIP(lsb::launch_seg_copy(A, B, hdr, large, plan, L.tile, 4 * cus, s));
@@ -587,7 +607,7 @@ int lsb_sort(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg) {
   // profiling, runs the pass pipeline).
   if (c.digit_bits == 0 && c.tile == 0 && c.variant == 0 && !(c.flags & LSB_PROFILE) &&
       n <= lsb::small_sort_capacity()) {
-    LSB_HIP(lsb::launch_small_sort(A, (uint32_t)n, p.key_bits, s));
+    LSB_HIP(lsb::launch_small_sort(A, (uint32_t)n, p.key_bits, p.ko, s));
     if (c.flags & LSB_SYNC) LSB_HIP(hipStreamSynchronize(s));
     return LSB_OK;
   }
@@ -949,7 +969,7 @@ int lsb_debug_histogram(const lsb_elem* A, size_t n, const lsb_config* cfg, uint
   const size_t words = (size_t)p.passes * lsb::kRegions * p.nb;
   LSB_HIP(hipMallocAsync(&tmp, words * 4, s));
   lsb::launch_zero(tmp, words, s);
-  if (n) LSB_HIP(lsb::launch_histogram(p.d, A, (uint32_t)n, p.key_bits, p.passes, tmp,
+  if (n) LSB_HIP(lsb::launch_histogram(p.d, A, (uint32_t)n, p.key_bits, p.ko, p.passes, tmp,
                                        (uint32_t)n, nullptr, 0, s));
   for (uint32_t q = 0; q < p.passes; ++q)
     LSB_HIP(hipMemcpyAsync(counts + (size_t)q * p.nb, tmp + (size_t)q * lsb::kRegions * p.nb,
@@ -1009,13 +1029,17 @@ int lsb_generate(lsb_elem* A, size_t n, const lsb_gen_spec* spec, void* stream) 
 }
 
 int lsb_verify(const lsb_elem* out, const uint64_t* in_keys, size_t n, uint64_t val_base,
-               uint32_t key_bits, uint64_t* bad_count, int64_t* first_bad, void* stream) {
+               uint32_t key_bits, uint32_t key_flags, uint64_t* bad_count, int64_t* first_bad,
+               void* stream) {
   if (!bad_count || !first_bad) return LSB_EINVAL;
   *bad_count = 0;
   *first_bad = -1;
-  if (n == 0) return LSB_OK;
-  if (!out || !in_keys || key_bits > 64) return LSB_EINVAL;
+  if (key_bits > 64 || (key_flags & ~(LSB_KEYS_SIGNED | LSB_KEYS_FLOAT))) return LSB_EINVAL;
   const uint32_t kb = key_bits ? key_bits : 64;
+  lsb::KeyOrder ko;
+  if (make_key_order(key_flags, kb, &ko)) return LSB_EINVAL;
+  if (n == 0) return LSB_OK;
+  if (!out || !in_keys) return LSB_EINVAL;
   const uint64_t mask = kb >= 64 ? ~0ull : ((1ull << kb) - 1);
   hipStream_t s = (hipStream_t)stream;
   const size_t words = (n + 31) / 32;
@@ -1023,7 +1047,7 @@ int lsb_verify(const lsb_elem* out, const uint64_t* in_keys, size_t n, uint64_t 
   LSB_HIP(hipMalloc(&buf, align_up(words * 4) + 16));
   uint32_t* bitmap = reinterpret_cast<uint32_t*>(buf);
   unsigned long long* res = reinterpret_cast<unsigned long long*>(buf + align_up(words * 4));
-  hipError_t e = lsb::launch_verify(out, in_keys, n, val_base, mask, bitmap, res, s);
+  hipError_t e = lsb::launch_verify(out, in_keys, n, val_base, mask, ko, bitmap, res, s);
   unsigned long long host[2] = {0, 0};
   if (e == hipSuccess) e = hipMemcpyAsync(host, res, sizeof host, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "lsb_internal.hpp"
+
 namespace lsb {
 
 // One element, 16 bytes, key at offset 0 (SortElement, mpi/mpi_lsbsort.cpp:30-33).
@@ -109,6 +111,32 @@ __device__ __forceinline__ void block_exclusive_scan(uint32_t* a, uint32_t* wave
 // to the key width (Chapel's nBits, chpl/arkouda-radix-sort.chpl:69-78).
 __device__ __forceinline__ uint32_t digit_of(uint64_t key, uint32_t shift, uint32_t mask) {
   return (uint32_t)(key >> shift) & mask;
+}
+
+// Key order.  Keys are unsigned integers of key_bits bits by default; a
+// caller may declare them two's complement signed or IEEE-754 floats
+// (binary32 in the low 32 bits, or binary64).  key_fwd maps a key to an
+// unsigned integer of the same order, which is what every digit is taken
+// from; key_inv maps it back before the element is written, so elements
+// leave the sort unchanged.  sign = bit key_bits-1; fsign = sign for floats,
+// else 0; full = the low key_bits bits.  Floats: a set sign bit flips every
+// bit (negatives reverse), a clear one flips the sign bit alone, which puts
+// -0 before +0 and orders NaNs by their bits at both ends.  All three are 0
+// for unsigned keys, so both maps are the identity.  The reference carries
+// an unused `negs` argument for this (chpl/arkouda-radix-sort.chpl:37-39).
+__device__ __host__ __forceinline__ uint64_t key_fwd(uint64_t k, const KeyOrder& o) {
+  return k ^ ((k & o.fsign) ? o.full : o.sign);
+}
+__device__ __host__ __forceinline__ uint64_t key_inv(uint64_t t, const KeyOrder& o) {
+  return t ^ ((o.fsign && !(t & o.fsign)) ? o.full : o.sign);
+}
+// key_fwd on every element of a register tile (a no-op branch for unsigned).
+template <int ITEMS>
+__device__ __forceinline__ void keys_fwd(elem_t (&e)[ITEMS], const KeyOrder& o) {
+  if (o.sign) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) e[i].key = key_fwd(e[i].key, o);
+  }
 }
 
 // Adds one count for digit `d` of every valid lane.  When every active lane

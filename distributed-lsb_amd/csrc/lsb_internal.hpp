@@ -8,6 +8,15 @@
 
 namespace lsb {
 
+// Key order (key_fwd / key_inv in lsb_device.hpp): all zero for unsigned
+// keys; sign = bit key_bits-1 for signed and float keys; fsign = sign for
+// float keys; full = the low key_bits bits.
+struct KeyOrder {
+  uint64_t sign;
+  uint64_t fsign;
+  uint64_t full;
+};
+
 // Each scatter pass cuts the array into this many regions of whole tiles;
 // every region has its own tile tickets and look-back chain, so eight
 // chains advance side by side instead of one.
@@ -35,6 +44,7 @@ struct PassArgs {
   uint32_t region_tiles;        // tiles per region (the last region may hold fewer)
   const uint32_t* regcount;     // histogram kernel's per-region counts [P][R][2^d]
   uint32_t* regnext;            // per-region counts filled by the pass before [P][R][2^d]
+  KeyOrder ko;                  // key order (identity for unsigned keys)
 };
 
 typedef void (*ScatterLaunch)(const PassArgs&, hipStream_t);
@@ -55,8 +65,9 @@ void launch_zero(uint32_t* p, size_t words, hipStream_t s);
 // All-digit histogram per region: regcount[p][region][b] += ..., with
 // regions of region_elems elements; zeroes zero[0..zero_words).
 hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
-                            uint32_t passes, uint32_t* regcount, uint32_t region_elems,
-                            uint32_t* zero, size_t zero_words, hipStream_t s);
+                            const KeyOrder& ko, uint32_t passes, uint32_t* regcount,
+                            uint32_t region_elems, uint32_t* zero, size_t zero_words,
+                            hipStream_t s);
 // Global counts (sum of regions), exclusive scans, skip detection, plan;
 // clears tickets[P][R] and regnext[P][R][2^d].
 hipError_t launch_scan_plan(uint32_t d, const uint32_t* regcount, uint32_t* starts,
@@ -65,7 +76,8 @@ hipError_t launch_scan_plan(uint32_t d, const uint32_t* regcount, uint32_t* star
 // Whole sort of one array of n <= small_sort_capacity() elements inside one
 // workgroup (one HBM read and write per element).
 uint32_t small_sort_capacity();
-hipError_t launch_small_sort(void* A, uint32_t n, uint32_t key_bits, hipStream_t s);
+hipError_t launch_small_sort(void* A, uint32_t n, uint32_t key_bits, const KeyOrder& ko,
+                             hipStream_t s);
 // Copies B to A when plan says the result is in B.
 void launch_copy_back(void* A, const void* B, uint32_t n, const uint32_t* flag,
                       hipStream_t s);
@@ -102,6 +114,7 @@ struct SegPassArgs {
   uint32_t pass;
   uint32_t shift;
   uint32_t mask;
+  KeyOrder ko;
 };
 typedef hipError_t (*SegScatterLaunch)(const SegPassArgs&, uint32_t grid, hipStream_t);
 struct SegVariant {
@@ -115,10 +128,10 @@ const SegVariant* seg_variants(int* count);
 uint32_t seg_tiny_capacity();
 uint32_t seg_small_capacity();
 hipError_t launch_seg_small(void* A, const SegHeader* hdr, const SegSmall* list,
-                            uint32_t key_bits, uint32_t grid, hipStream_t s);
+                            uint32_t key_bits, const KeyOrder& ko, uint32_t grid, hipStream_t s);
 hipError_t launch_seg_prepare(uint32_t d, const void* A, const SegHeader* hdr,
                               const SegLarge* large, uint32_t tile, uint32_t key_bits,
-                              uint32_t passes, uint32_t* counts, uint32_t* seg_plan,
+                              const KeyOrder& ko, uint32_t passes, uint32_t* counts, uint32_t* seg_plan,
                               uint32_t* tickets, uint32_t* status0, uint32_t flags,
                               uint32_t grid, hipStream_t s);
 hipError_t launch_seg_copy(void* A, const void* B, const SegHeader* hdr, const SegLarge* large,
@@ -131,7 +144,8 @@ hipError_t launch_generate(void* A, uint64_t n, uint32_t kind, uint64_t per,
                            const double* cdf, const uint64_t* table, uint32_t m,
                            hipStream_t s);
 hipError_t launch_verify(const void* out, const uint64_t* in_keys, uint64_t n,
-                         uint64_t val_base, uint64_t key_mask, uint32_t* bitmap,
+                         uint64_t val_base, uint64_t key_mask, const KeyOrder& ko,
+                         uint32_t* bitmap,
                          unsigned long long* result, hipStream_t s);
 hipError_t launch_extract_keys(const void* A, uint64_t n, uint64_t* keys, hipStream_t s);
 

@@ -46,7 +46,7 @@ void launch_zero(uint32_t* p, size_t words, hipStream_t s) {
 // hist_add.
 template <int D, int P>
 __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A, uint32_t n,
-                                                   uint32_t key_bits,
+                                                   uint32_t key_bits, KeyOrder ko,
                                                    uint32_t* __restrict__ regcount,
                                                    uint32_t region_elems,
                                                    uint32_t blocks_per_region,
@@ -88,6 +88,10 @@ __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A,
     uint64_t k[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) k[u] = A[base + (size_t)u * 256 + t].key;
+    if (ko.sign) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) k[u] = key_fwd(k[u], ko);
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
 #pragma unroll
@@ -98,7 +102,7 @@ __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A,
   for (; base < c1; base += 256) {
     const size_t i = base + t;
     const bool v = i < c1;
-    const uint64_t k = v ? A[i].key : 0ull;
+    const uint64_t k = v ? key_fwd(A[i].key, ko) : 0ull;
 #pragma unroll
     for (int p = 0; p < P; ++p) hist_add(h + p * NB, digit_of(k, p * D, mask[p]), v);
   }
@@ -112,16 +116,17 @@ __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A,
 }
 
 template <int D, int P>
-static void launch_hist(const elem_t* a, uint32_t n, uint32_t key_bits, uint32_t* regcount,
-                        uint32_t region_elems, uint32_t per_region, uint32_t* zero,
-                        size_t zero_words, hipStream_t s) {
+static void launch_hist(const elem_t* a, uint32_t n, uint32_t key_bits, const KeyOrder& ko,
+                        uint32_t* regcount, uint32_t region_elems, uint32_t per_region,
+                        uint32_t* zero, size_t zero_words, hipStream_t s) {
   hipLaunchKernelGGL((k_histogram<D, P>), dim3(per_region * kRegions), dim3(256), 0, s, a, n,
-                     key_bits, regcount, region_elems, per_region, zero, zero_words);
+                     key_bits, ko, regcount, region_elems, per_region, zero, zero_words);
 }
 
 hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
-                            uint32_t passes, uint32_t* regcount, uint32_t region_elems,
-                            uint32_t* zero, size_t zero_words, hipStream_t s) {
+                            const KeyOrder& ko, uint32_t passes, uint32_t* regcount,
+                            uint32_t region_elems, uint32_t* zero, size_t zero_words,
+                            hipStream_t s) {
   // Enough blocks to keep every CU streaming, few enough that the final
   // flush of LDS counters stays small.
   size_t per_region = ((size_t)region_elems + 256 * 8 * 8 - 1) / (256 * 8 * 8);
@@ -130,7 +135,9 @@ hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_
   const uint32_t pr = (uint32_t)per_region;
   const elem_t* a = static_cast<const elem_t*>(A);
 #define LSB_HIST_CASE(DD, PP) \
-  case PP: launch_hist<DD, PP>(a, n, key_bits, regcount, region_elems, pr, zero, zero_words, s); break;
+  case PP:                   \
+    launch_hist<DD, PP>(a, n, key_bits, ko, regcount, region_elems, pr, zero, zero_words, s); \
+    break;
   if (d == 8) {
     switch (passes) {
       LSB_HIST_CASE(8, 1) LSB_HIST_CASE(8, 2) LSB_HIST_CASE(8, 3) LSB_HIST_CASE(8, 4)
@@ -233,7 +240,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(PassArgs args) {
   __syncthreads();
   const uint32_t tile = sm.scalar[0];
   onesweep_tile<D, BLOCK, ITEMS>(sm, src, dst, args.n, tile, st_cur, st_next, args.starts,
-                                 args.shift, args.mask);
+                                 args.shift, args.mask, args.ko);
 }
 
 // Region form: a persistent grid of kRegions groups of blocks; group r takes
@@ -296,10 +303,11 @@ __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(Pass
     const uint32_t tile = t_begin + local;
     elem_t e[ITEMS];
     tile_load<BLOCK, ITEMS>(e, src, args.n, tile);
+    keys_fwd<ITEMS>(e, args.ko);
     onesweep_tile_region<D, BLOCK, ITEMS>(sm, e, dst, args.n, tile, local == 0, st_cur, st_next,
                                           s_rbase, args.shift, args.mask, ticket,
                                           q >= 0 ? s_hnext : nullptr, nshift, nmask,
-                                          region_elems);
+                                          region_elems, args.ko);
     lds_barrier();
   }
   if (q >= 0) {
@@ -357,21 +365,23 @@ const SortVariant* sort_variants(int* count) {
 // ----------------------------------------------------------- k_small_sort
 // Whole sort of one array of n <= BLOCK*ITEMS elements in one block (LDS).
 template <int D, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void k_small_sort(elem_t* A, uint32_t n, uint32_t key_bits) {
+__global__ __launch_bounds__(BLOCK) void k_small_sort(elem_t* A, uint32_t n, uint32_t key_bits,
+                                                      KeyOrder ko) {
   __shared__ TileSmem<D, BLOCK, ITEMS> sm;
-  small_sort<D, BLOCK, ITEMS>(sm, A, n, key_bits);
+  small_sort<D, BLOCK, ITEMS>(sm, A, n, key_bits, ko);
 }
 
 uint32_t small_sort_capacity() { return 512 * 16; }
 
-hipError_t launch_small_sort(void* A, uint32_t n, uint32_t key_bits, hipStream_t s) {
+hipError_t launch_small_sort(void* A, uint32_t n, uint32_t key_bits, const KeyOrder& ko,
+                             hipStream_t s) {
   if (n > small_sort_capacity()) return hipErrorInvalidValue;
   if (n <= 256 * 8)
     hipLaunchKernelGGL((k_small_sort<8, 256, 8>), dim3(1), dim3(256), 0, s,
-                       static_cast<elem_t*>(A), n, key_bits);
+                       static_cast<elem_t*>(A), n, key_bits, ko);
   else
     hipLaunchKernelGGL((k_small_sort<8, 512, 16>), dim3(1), dim3(512), 0, s,
-                       static_cast<elem_t*>(A), n, key_bits);
+                       static_cast<elem_t*>(A), n, key_bits, ko);
   return hipGetLastError();
 }
 

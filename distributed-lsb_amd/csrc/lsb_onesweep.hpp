@@ -182,7 +182,8 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
                                                    uint32_t tile, uint32_t* st_cur,
                                                    uint32_t* st_next,
                                                    const uint32_t* __restrict__ starts,
-                                                   uint32_t shift, uint32_t mask) {
+                                                   uint32_t shift, uint32_t mask,
+                                                   const KeyOrder& ko) {
   using S = TileSmem<D, BLOCK, ITEMS>;
   constexpr int NB = S::NB, W = S::W, T = S::T;
   constexpr int DPT = (NB + BLOCK - 1) / BLOCK;  // digits per thread
@@ -266,8 +267,9 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
   for (int k = 0; k < ITEMS; ++k) {
     const uint32_t j = k * BLOCK + t;
     if (j < tile_n) {
-      const elem_t x = sm.elems[j];
+      elem_t x = sm.elems[j];
       const uint32_t d = digit_of(x.key, shift, mask);
+      x.key = key_inv(x.key, ko);
       dst[(size_t)(uint32_t)(sm.grs[d].x + j)] = x;
     }
   }
@@ -284,7 +286,7 @@ template <int BLOCK, int ITEMS, int NB, bool COUNT>
 __device__ __forceinline__ void store_run(const elem_t* elems, const uint2* grs, uint32_t* hnext,
                                           uint32_t tile_n, elem_t* __restrict__ dst,
                                           uint32_t shift, uint32_t mask, uint32_t nshift,
-                                          uint32_t nmask) {
+                                          uint32_t nmask, const KeyOrder& ko) {
   const uint32_t t = threadIdx.x;
   if (tile_n == (uint32_t)(BLOCK * ITEMS)) {
     elem_t x[ITEMS];
@@ -296,8 +298,6 @@ __device__ __forceinline__ void store_run(const elem_t* elems, const uint2* grs,
     for (int k = 0; k < ITEMS; ++k) d[k] = digit_of(x[k].key, shift, mask);
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) g[k] = grs[d[k]];
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k) dst[(size_t)(g[k].x + (uint32_t)(k * BLOCK) + t)] = x[k];
     if (COUNT) {
 #pragma unroll
       for (int k = 0; k < ITEMS; ++k) {
@@ -306,20 +306,27 @@ __device__ __forceinline__ void store_run(const elem_t* elems, const uint2* grs,
         atomicAdd(&hnext[r * NB + digit_of(x[k].key, nshift, nmask)], 1u);
       }
     }
+    if (ko.sign) {
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) x[k].key = key_inv(x[k].key, ko);
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) dst[(size_t)(g[k].x + (uint32_t)(k * BLOCK) + t)] = x[k];
     return;
   }
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const uint32_t j = k * BLOCK + t;
     if (j < tile_n) {
-      const elem_t x = elems[j];
+      elem_t x = elems[j];
       const uint32_t d = digit_of(x.key, shift, mask);
       const uint2 g = grs[d];
-      dst[(size_t)(g.x + j)] = x;
       if (COUNT) {
         const uint32_t r = (g.y & 7u) + (j >= (g.y >> 3) ? 1u : 0u);
         atomicAdd(&hnext[r * NB + digit_of(x.key, nshift, nmask)], 1u);
       }
+      x.key = key_inv(x.key, ko);
+      dst[(size_t)(g.x + j)] = x;
     }
   }
 }
@@ -342,7 +349,7 @@ __device__ __forceinline__ void onesweep_tile_region(
     TileSmem<D, BLOCK, ITEMS>& sm, const elem_t (&e)[ITEMS], elem_t* __restrict__ dst,
     uint32_t n, uint32_t tile, bool chain_first, uint32_t* st_cur, uint32_t* st_next,
     const uint32_t* rbase, uint32_t shift, uint32_t mask, uint32_t* ticket, uint32_t* hnext,
-    uint32_t nshift, uint32_t nmask, uint32_t region_elems) {
+    uint32_t nshift, uint32_t nmask, uint32_t region_elems, const KeyOrder& ko) {
   using S = TileSmem<D, BLOCK, ITEMS>;
   constexpr int NB = S::NB, W = S::W, T = S::T;
   constexpr int DPT = (NB + BLOCK - 1) / BLOCK;
@@ -433,10 +440,10 @@ __device__ __forceinline__ void onesweep_tile_region(
   if (t == 0) next = atomicAdd(ticket, 1u);
   if (hnext)
     store_run<BLOCK, ITEMS, NB, true>(sm.elems, sm.grs, hnext, tile_n, dst, shift, mask, nshift,
-                                      nmask);
+                                      nmask, ko);
   else
     store_run<BLOCK, ITEMS, NB, false>(sm.elems, sm.grs, hnext, tile_n, dst, shift, mask, nshift,
-                                       nmask);
+                                       nmask, ko);
   if (t == 0) sm.scalar[0] = next;
 }
 
@@ -448,10 +455,13 @@ __device__ __forceinline__ void onesweep_tile(TileSmem<D, BLOCK, ITEMS>& sm,
                                               uint32_t tile, uint32_t* st_cur,
                                               uint32_t* st_next,
                                               const uint32_t* __restrict__ starts,
-                                              uint32_t shift, uint32_t mask) {
+                                              uint32_t shift, uint32_t mask,
+                                              const KeyOrder& ko) {
   elem_t e[ITEMS];
   tile_load<BLOCK, ITEMS>(e, src, n, tile);
-  onesweep_tile_regs<D, BLOCK, ITEMS>(sm, e, dst, n, tile, st_cur, st_next, starts, shift, mask);
+  keys_fwd<ITEMS>(e, ko);
+  onesweep_tile_regs<D, BLOCK, ITEMS>(sm, e, dst, n, tile, st_cur, st_next, starts, shift, mask,
+                                      ko);
 }
 
 // ----------------------------------------------------------- small_sort
@@ -462,7 +472,7 @@ __device__ __forceinline__ void onesweep_tile(TileSmem<D, BLOCK, ITEMS>& sm,
 // Digits whose elements all share one bucket are skipped.
 template <int D, int BLOCK, int ITEMS>
 __device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, elem_t* __restrict__ a,
-                                           uint32_t n, uint32_t key_bits) {
+                                           uint32_t n, uint32_t key_bits, const KeyOrder& ko) {
   using S = TileSmem<D, BLOCK, ITEMS>;
   constexpr int NB = S::NB, W = S::W;
   const uint32_t t = threadIdx.x;
@@ -482,6 +492,7 @@ __device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, elem_t
       e[i].val = 0;
     }
   }
+  keys_fwd<ITEMS>(e, ko);
   const uint32_t passes = (key_bits + D - 1) / D;
   for (uint32_t p = 0; p < passes; ++p) {
     const uint32_t shift = p * D;
@@ -519,6 +530,10 @@ __device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, elem_t
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i)
       if ((uint32_t)i * kWave + lane < lim) e[i] = sm.elems[wrun + (uint32_t)i * kWave + lane];
+  }
+  if (ko.sign) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) e[i].key = key_inv(e[i].key, ko);
   }
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)

@@ -39,7 +39,7 @@ __device__ __forceinline__ uint32_t find_segment(const SegLarge* __restrict__ la
 template <int D, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_seg_small(elem_t* A, const SegHeader* __restrict__ hdr,
                                                      const SegSmall* __restrict__ list,
-                                                     int which, uint32_t key_bits) {
+                                                     int which, uint32_t key_bits, KeyOrder ko) {
   __shared__ TileSmem<D, BLOCK, ITEMS> sm;
   // The metadata array holds the large segments, then the tiny, then the
   // small ones (16 bytes each).
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_small(elem_t* A, const SegHeader*
   const SegSmall* l = list + hdr->n_large + (which == 0 ? 0u : hdr->n_tiny);
   for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
     const SegSmall s = l[i];
-    small_sort<D, BLOCK, ITEMS>(sm, A + s.base, s.n, key_bits);
+    small_sort<D, BLOCK, ITEMS>(sm, A + s.base, s.n, key_bits, ko);
     __syncthreads();
   }
 }
@@ -68,7 +68,7 @@ template <int D>
 __global__ __launch_bounds__(256) void k_seg_hist(const elem_t* __restrict__ A,
                                                   const SegHeader* __restrict__ hdr,
                                                   const SegLarge* __restrict__ large,
-                                                  uint32_t tile, uint32_t key_bits,
+                                                  uint32_t tile, uint32_t key_bits, KeyOrder ko,
                                                   uint32_t passes, uint32_t* __restrict__ counts,
                                                   uint32_t* __restrict__ status0) {
   constexpr int NB = 1 << D;
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_seg_hist(const elem_t* __restrict__ A,
       for (int u = 0; u < 4; ++u) {
         const uint32_t i = i0 + u * 256 + t;
         v[u] = i < cnt;
-        k[u] = v[u] ? a[i].key : 0ull;
+        k[u] = v[u] ? key_fwd(a[i].key, ko) : 0ull;
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(SegPassArgs args) {
           args.status + (size_t)((ord + 1u) & 1u) * slice + (size_t)L.tile_begin * NB;
       const uint32_t* starts = args.counts + (size_t)seg * MAXP * NB + (size_t)pass * NB;
       onesweep_tile<D, BLOCK, ITEMS>(sm, src, dst, L.n, tile - L.tile_begin, st_cur, st_next,
-                                     starts, args.shift, args.mask);
+                                     starts, args.shift, args.mask, args.ko);
       (void)T;
     }
     __syncthreads();
@@ -275,17 +275,17 @@ uint32_t seg_tiny_capacity() { return 256 * 8; }
 uint32_t seg_small_capacity() { return 512 * 16; }
 
 hipError_t launch_seg_small(void* A, const SegHeader* hdr, const SegSmall* list,
-                            uint32_t key_bits, uint32_t grid, hipStream_t s) {
+                            uint32_t key_bits, const KeyOrder& ko, uint32_t grid, hipStream_t s) {
   hipLaunchKernelGGL((k_seg_small<8, 256, 8>), dim3(grid), dim3(256), 0, s,
-                     static_cast<elem_t*>(A), hdr, list, 0, key_bits);
+                     static_cast<elem_t*>(A), hdr, list, 0, key_bits, ko);
   hipLaunchKernelGGL((k_seg_small<8, 512, 16>), dim3(grid), dim3(512), 0, s,
-                     static_cast<elem_t*>(A), hdr, list, 1, key_bits);
+                     static_cast<elem_t*>(A), hdr, list, 1, key_bits, ko);
   return hipGetLastError();
 }
 
 hipError_t launch_seg_prepare(uint32_t d, const void* A, const SegHeader* hdr,
                               const SegLarge* large, uint32_t tile, uint32_t key_bits,
-                              uint32_t passes, uint32_t* counts, uint32_t* seg_plan,
+                              const KeyOrder& ko, uint32_t passes, uint32_t* counts, uint32_t* seg_plan,
                               uint32_t* tickets, uint32_t* status0, uint32_t flags,
                               uint32_t grid, hipStream_t s) {
   const elem_t* a = static_cast<const elem_t*>(A);
@@ -293,14 +293,14 @@ hipError_t launch_seg_prepare(uint32_t d, const void* A, const SegHeader* hdr,
     constexpr int MAXP = 8;
     hipLaunchKernelGGL(k_seg_zero, dim3(grid), dim3(256), 0, s, hdr, counts, MAXP * 256u);
     hipLaunchKernelGGL(k_seg_hist<8>, dim3(grid), dim3(256), 0, s, a, hdr, large, tile,
-                       key_bits, passes, counts, status0);
+                       key_bits, ko, passes, counts, status0);
     hipLaunchKernelGGL(k_seg_scan<8>, dim3(grid), dim3(1024), 0, s, hdr, large, counts,
                        seg_plan, tickets, passes, flags);
   } else if (d == 11) {
     constexpr int MAXP = 6;
     hipLaunchKernelGGL(k_seg_zero, dim3(grid), dim3(256), 0, s, hdr, counts, MAXP * 2048u);
     hipLaunchKernelGGL(k_seg_hist<11>, dim3(grid), dim3(256), 0, s, a, hdr, large, tile,
-                       key_bits, passes, counts, status0);
+                       key_bits, ko, passes, counts, status0);
     hipLaunchKernelGGL(k_seg_scan<11>, dim3(grid), dim3(1024), 0, s, hdr, large, counts,
                        seg_plan, tickets, passes, flags);
   } else {

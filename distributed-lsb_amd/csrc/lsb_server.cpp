@@ -44,6 +44,7 @@ struct Job {
   lsb_elem* A;
   size_t n;
   uint32_t key_bits;
+  uint32_t key_flags;  // LSB_KEYS_SIGNED / LSB_KEYS_FLOAT or 0
   double submitted, done_at = 0;
   int state = 0;  // 0 pending, 1 admitted/running, 2 done, <0 error code
   int rc = 0;
@@ -76,7 +77,8 @@ struct lsb_server {
   std::unordered_map<uint64_t, Job> jobs;
   std::vector<Launch> inflight;
   std::vector<StreamSlot> slots;
-  std::map<uint32_t, std::vector<uint64_t>> open_batch;  // key_bits -> admitted small jobs
+  // key class (key_bits | key_flags << 8) -> admitted small jobs
+  std::map<uint32_t, std::vector<uint64_t>> open_batch;
   std::map<uint32_t, uint64_t> open_elems;
   std::map<uint32_t, double> batch_opened;
   uint64_t next_id = 1;
@@ -96,7 +98,7 @@ struct lsb_server {
   bool admit();
   bool flush_batches();
   int launch_plain(Job& j);
-  int launch_batch(uint32_t key_bits, std::vector<uint64_t>& ids);
+  int launch_batch(uint32_t key_class, std::vector<uint64_t>& ids);
   int pick_slot(bool need_slab);
 };
 
@@ -137,6 +139,7 @@ int lsb_server::launch_plain(Job& j) {
       lsb_config c;
       lsb_config_init(&c);
       c.key_bits = j.key_bits;
+      c.flags = j.key_flags;
       c.stream = sl.s;
       rc = lsb_sort(j.A, static_cast<lsb_elem*>(L.scratch), j.n, &c);
       (void)hipFreeAsync(L.scratch, sl.s);
@@ -158,7 +161,7 @@ int lsb_server::launch_plain(Job& j) {
   return LSB_OK;
 }
 
-int lsb_server::launch_batch(uint32_t key_bits, std::vector<uint64_t>& ids) {
+int lsb_server::launch_batch(uint32_t key_class, std::vector<uint64_t>& ids) {
   const int si = pick_slot(true);
   if (si < 0) return LSB_EBUSY;  // every staging slab still busy: retry later
   StreamSlot& sl = slots[si];
@@ -173,7 +176,8 @@ int lsb_server::launch_batch(uint32_t key_bits, std::vector<uint64_t>& ids) {
   if (rc == LSB_OK) {
     lsb_config c;
     lsb_config_init(&c);
-    c.key_bits = key_bits;
+    c.key_bits = key_class & 0xFFu;
+    c.flags = key_class >> 8;
     c.stream = sl.s;
     if (cfg.use_graphs) c.flags |= LSB_GRAPH;
     rc = lsb_sort_segmented(sl.slab_a, sl.slab_b, offsets.data(), (uint32_t)ids.size(), &c);
@@ -249,17 +253,19 @@ bool lsb_server::admit() {
     const uint64_t charge = 32ull * j.n;
     if (used + charge > cfg.memory_budget && used > 0) break;  // FIFO: wait for memory
     if (j.n <= cfg.batch_max_elems) {
-      auto& b = open_batch[j.key_bits];
-      uint64_t& tot = open_elems[j.key_bits];
+      // jobs are batched with others of the same key width and key order
+      const uint32_t cls = j.key_bits | (j.key_flags << 8);
+      auto& b = open_batch[cls];
+      uint64_t& tot = open_elems[cls];
       if (!b.empty() && tot + j.n > cfg.batch_max_total) {
-        if (launch_batch(j.key_bits, b) == LSB_EBUSY) break;  // every slab busy
+        if (launch_batch(cls, b) == LSB_EBUSY) break;  // every slab busy
         b.clear();
         tot = 0;
       }
       pending.pop_front();
       used += charge;
       j.state = 1;
-      if (b.empty()) batch_opened[j.key_bits] = now_s();
+      if (b.empty()) batch_opened[cls] = now_s();
       b.push_back(j.id);
       tot += j.n;
     } else {
@@ -395,10 +401,14 @@ int lsb_server_create(const lsb_server_config* cfg, lsb_server** out) {
 }
 
 int lsb_server_submit(lsb_server* s, lsb_elem* A, size_t n, uint32_t key_bits,
-                      uint64_t* job_id) {
+                      uint32_t key_flags, uint64_t* job_id) {
   if (!s || !job_id) return LSB_EINVAL;
   if (n > LSB_MAX_N) return LSB_ERANGE;
   if (key_bits > 64 || (n > 1 && (!A || ((uintptr_t)A & 15u)))) return LSB_EINVAL;
+  if (key_flags != 0 && key_flags != LSB_KEYS_SIGNED && key_flags != LSB_KEYS_FLOAT)
+    return LSB_EINVAL;
+  if (key_flags == LSB_KEYS_FLOAT && key_bits != 32 && key_bits != 64 && key_bits != 0)
+    return LSB_EINVAL;
   std::lock_guard<std::mutex> lock(s->mu);
   if (s->stop) return LSB_EINVAL;
   if (s->pending.size() >= s->cfg.max_pending) return LSB_EBUSY;
@@ -408,6 +418,7 @@ int lsb_server_submit(lsb_server* s, lsb_elem* A, size_t n, uint32_t key_bits,
   j.A = A;
   j.n = n;
   j.key_bits = key_bits ? key_bits : 64;
+  j.key_flags = key_flags;
   j.submitted = now_s();
   if (s->first_submit == 0) s->first_submit = j.submitted;
   s->jobs[j.id] = j;

@@ -4,7 +4,8 @@
 //                   a block distribution over virtual ranks: element g takes
 //                   output number (g mod per) of pcg64(seed_base + g/per).
 //                   Each thread jumps its stream ahead once and then steps.
-//   k_verify        O(n) proof of out == stable_sort(in) given val = index
+//   k_verify        O(n) proof of out == stable_sort(in) given val = index,
+//                   under the caller's key order (key_fwd)
 //                   (replaces the gather + std::stable_sort comparison of
 //                   mpi/mpi_lsbsort.cpp:755-784, which is O(n log n) on one
 //                   host core).
@@ -84,7 +85,7 @@ hipError_t launch_generate(void* A, uint64_t n, uint32_t kind, uint64_t per,
 __global__ __launch_bounds__(256) void k_verify(const elem_t* __restrict__ out,
                                                 const uint64_t* __restrict__ in_keys,
                                                 uint64_t n, uint64_t val_base,
-                                                uint64_t key_mask,
+                                                uint64_t key_mask, KeyOrder ko,
                                                 uint32_t* __restrict__ bitmap,
                                                 unsigned long long* __restrict__ result) {
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(256) void k_verify(const elem_t* __restrict__ out,
     }
     if (ok && i > 0) {
       const elem_t p = out[i - 1];
-      const uint64_t pk = p.key & key_mask, k = e.key & key_mask;
+      const uint64_t pk = key_fwd(p.key, ko) & key_mask, k = key_fwd(e.key, ko) & key_mask;
       ok = pk < k || (pk == k && p.val < e.val);
     }
     if (!ok) {
@@ -121,7 +122,8 @@ __global__ void k_init_result(unsigned long long* r) {
 }
 
 hipError_t launch_verify(const void* out, const uint64_t* in_keys, uint64_t n,
-                         uint64_t val_base, uint64_t key_mask, uint32_t* bitmap,
+                         uint64_t val_base, uint64_t key_mask, const KeyOrder& ko,
+                         uint32_t* bitmap,
                          unsigned long long* result, hipStream_t s) {
   const uint64_t words = (n + 31) / 32;
   hipLaunchKernelGGL(k_zero_words, dim3(2048), dim3(256), 0, s, bitmap, words);
@@ -130,7 +132,7 @@ hipError_t launch_verify(const void* out, const uint64_t* in_keys, uint64_t n,
   if (blocks > 8192) blocks = 8192;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_verify, dim3((unsigned)blocks), dim3(256), 0, s,
-                     static_cast<const elem_t*>(out), in_keys, n, val_base, key_mask, bitmap,
+                     static_cast<const elem_t*>(out), in_keys, n, val_base, key_mask, ko, bitmap,
                      result);
   return hipGetLastError();
 }

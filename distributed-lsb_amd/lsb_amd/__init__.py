@@ -21,6 +21,9 @@ LIB_PATH = os.path.join(PKG, "liblsb_amd.so")
 LSB_OK, LSB_EINVAL, LSB_ENOMEM, LSB_EHIP, LSB_ERANGE, LSB_EBUSY, LSB_ENOTFOUND = (
     0, -1, -2, -3, -4, -5, -6)
 LSB_SYNC, LSB_NO_SKIP, LSB_PROFILE, LSB_GRAPH = 1, 2, 4, 8
+LSB_KEYS_SIGNED, LSB_KEYS_FLOAT = 16, 32
+# key_order names accepted by the sort, verify and server calls
+KEY_ORDERS = {"unsigned": 0, "signed": LSB_KEYS_SIGNED, "float": LSB_KEYS_FLOAT}
 LSB_MAX_N = 0x7FFFFFFE
 GEN_UNIFORM, GEN_SORTED, GEN_REVERSE, GEN_ZIPF = 0, 1, 2, 3
 
@@ -106,7 +109,7 @@ SIGNATURES = {
     "lsb_server_config_init": (None, [ctypes.POINTER(ServerConfig)]),
     "lsb_server_create": (_I, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
     "lsb_server_destroy": (_I, [_P]),
-    "lsb_server_submit": (_I, [_P, _P, _SZ, _U32, ctypes.POINTER(_U64)]),
+    "lsb_server_submit": (_I, [_P, _P, _SZ, _U32, _U32, ctypes.POINTER(_U64)]),
     "lsb_server_poll": (_I, [_P, _U64]),
     "lsb_server_wait": (_I, [_P, _U64, ctypes.c_double]),
     "lsb_server_drain": (_I, [_P]),
@@ -121,7 +124,7 @@ SIGNATURES = {
     "lsb_profile_read": (_I, [ctypes.POINTER(Profile)]),
     "lsb_debug_histogram": (_I, [_P, _SZ, ctypes.POINTER(Config), _P]),
     "lsb_generate": (_I, [_P, _SZ, ctypes.POINTER(GenSpec), _P]),
-    "lsb_verify": (_I, [_P, _P, _SZ, _U64, _U32, ctypes.POINTER(_U64),
+    "lsb_verify": (_I, [_P, _P, _SZ, _U64, _U32, _U32, ctypes.POINTER(_U64),
                         ctypes.POINTER(ctypes.c_int64), _P]),
     "lsb_extract_keys": (_I, [_P, _SZ, _P, _P]),
 }
@@ -235,20 +238,29 @@ def workspace_size(n, key_bits=64, digit_bits=0, tile=0, variant=0):
     return lib().lsb_workspace_size(n, ctypes.byref(c))
 
 
+def _key_flags(key_order):
+    if key_order not in KEY_ORDERS:
+        raise ValueError(f"key_order must be one of {sorted(KEY_ORDERS)}")
+    return KEY_ORDERS[key_order]
+
+
 def sort(A, B, n=None, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,
          no_skip=False, workspace=None, workspace_bytes=0, profile=False, graph=False,
-         variant=0):
-    """mySort(A, B): stable sort of A[0:n] by key, B is scratch, result in A."""
+         variant=0, key_order="unsigned"):
+    """mySort(A, B): stable sort of A[0:n] by key, B is scratch, result in A.
+    key_order: "unsigned" (default), "signed" or "float" keys."""
     n = _nelem(A) if n is None else n
     flags = ((LSB_SYNC if sync else 0) | (LSB_NO_SKIP if no_skip else 0)
-             | (LSB_PROFILE if profile else 0) | (LSB_GRAPH if graph else 0))
+             | (LSB_PROFILE if profile else 0) | (LSB_GRAPH if graph else 0)
+             | _key_flags(key_order))
     c = make_config(key_bits, digit_bits, tile, stream, flags, workspace, workspace_bytes,
                     variant)
     _check(lib().lsb_sort(_ptr(A), _ptr(B), n, ctypes.byref(c)), "lsb_sort")
 
 
 def sort_segmented(A, B, offsets, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,
-                   no_skip=False, graph=False, workspace=None, workspace_bytes=0):
+                   no_skip=False, graph=False, workspace=None, workspace_bytes=0,
+                   key_order="unsigned"):
     """Sorts every segment [offsets[s], offsets[s+1]) of A independently and
     stably by key; B is scratch; results in A.  offsets: host sequence of
     nseg + 1 non-decreasing element offsets."""
@@ -256,7 +268,7 @@ def sort_segmented(A, B, offsets, key_bits=64, digit_bits=0, tile=0, stream=None
     off = _np.ascontiguousarray(_np.asarray(offsets, dtype=_np.uint64))
     nseg = len(off) - 1
     flags = ((LSB_SYNC if sync else 0) | (LSB_NO_SKIP if no_skip else 0)
-             | (LSB_GRAPH if graph else 0))
+             | (LSB_GRAPH if graph else 0) | _key_flags(key_order))
     c = make_config(key_bits, digit_bits, tile, stream, flags, workspace, workspace_bytes)
     _check(lib().lsb_sort_segmented(_ptr(A), _ptr(B),
                                     off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
@@ -305,9 +317,11 @@ def profile_read():
             "pass_ms": [p.pass_ms[q] for q in range(P)]}
 
 
-def debug_histogram(A, counts, n=None, key_bits=64, digit_bits=0, stream=None, sync=True):
+def debug_histogram(A, counts, n=None, key_bits=64, digit_bits=0, stream=None, sync=True,
+                    key_order="unsigned"):
     n = _nelem(A) if n is None else n
-    c = make_config(key_bits, digit_bits, 0, stream, LSB_SYNC if sync else 0)
+    c = make_config(key_bits, digit_bits, 0, stream,
+                    (LSB_SYNC if sync else 0) | _key_flags(key_order))
     _check(lib().lsb_debug_histogram(_ptr(A), n, ctypes.byref(c), _ptr(counts)),
            "lsb_debug_histogram")
 
@@ -321,12 +335,13 @@ def generate(A, n=None, kind=GEN_UNIFORM, ranks=1, key_bits=64, seed_base=0, val
            "lsb_generate")
 
 
-def verify(out, in_keys, n=None, val_base=0, key_bits=64, stream=None):
+def verify(out, in_keys, n=None, val_base=0, key_bits=64, stream=None, key_order="unsigned"):
     """Returns (bad_count, first_bad) -- (0, -1) proves out is the stable sort."""
     n = _nelem(out) if n is None else n
     bad, first = ctypes.c_uint64(), ctypes.c_int64()
     _check(lib().lsb_verify(_ptr(out), _ptr(in_keys), n, val_base, key_bits,
-                            ctypes.byref(bad), ctypes.byref(first), _stream_ptr(stream)),
+                            _key_flags(key_order), ctypes.byref(bad), ctypes.byref(first),
+                            _stream_ptr(stream)),
            "lsb_verify")
     return bad.value, first.value
 
@@ -363,10 +378,11 @@ class Server:
         _check(lib().lsb_server_create(ctypes.byref(c), ctypes.byref(h)), "lsb_server_create")
         self._h = h
 
-    def submit(self, A, n=None, key_bits=64):
+    def submit(self, A, n=None, key_bits=64, key_order="unsigned"):
         n = _nelem(A) if n is None else n
         jid = ctypes.c_uint64()
-        _check(lib().lsb_server_submit(self._h, _ptr(A), n, key_bits, ctypes.byref(jid)),
+        _check(lib().lsb_server_submit(self._h, _ptr(A), n, key_bits, _key_flags(key_order),
+                                       ctypes.byref(jid)),
                "lsb_server_submit")
         return jid.value
 

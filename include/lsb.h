@@ -54,6 +54,15 @@ typedef struct lsb_elem {
 #define LSB_NO_SKIP 2u  /* run every digit pass, even single-bucket ones */
 #define LSB_PROFILE 4u  /* record HIP events around each kernel of this call (lsb_profile_read) */
 #define LSB_GRAPH 8u    /* run the launch sequence as a captured hipGraph, cached per buffer set */
+/* Key order (at most one; default: unsigned integers of key_bits bits).
+ * SIGNED: two's complement of key_bits bits.  FLOAT: IEEE-754 binary64
+ * (key_bits 64) or binary32 in the low 32 bits (key_bits 32), ordered by
+ * value with -0 before +0 and NaNs at the ends by their bits (negative NaNs
+ * first).  Elements are moved unchanged; only the order changes.  Replaces
+ * the `negs` argument the reference passes to getDigit and never uses
+ * (chpl/arkouda-radix-sort.chpl:37-39). */
+#define LSB_KEYS_SIGNED 16u
+#define LSB_KEYS_FLOAT 32u
 
 typedef struct lsb_config {
   uint32_t key_bits;          /* sort by the low key_bits bits of key (1..64); 0 = 64.
@@ -68,7 +77,8 @@ typedef struct lsb_config {
   void* workspace;            /* device scratch of >= lsb_workspace_size() bytes;
                                  NULL = library-managed, cached per (device, stream) */
   size_t workspace_bytes;
-  uint32_t flags;             /* LSB_SYNC | LSB_NO_SKIP */
+  uint32_t flags;             /* LSB_SYNC | LSB_NO_SKIP | LSB_PROFILE | LSB_GRAPH |
+                                 LSB_KEYS_SIGNED | LSB_KEYS_FLOAT */
   uint32_t reserved1;
 } lsb_config;
 
@@ -186,11 +196,12 @@ int lsb_server_create(const lsb_server_config* cfg, lsb_server** out);
 /* Finishes every submitted job, then frees the server. */
 int lsb_server_destroy(lsb_server* srv);
 /* Queues a sort of A[0..n) (device memory, sorted in place by the low
- * key_bits of key; 0 = 64).  A must stay valid until the job completes.
- * LSB_EBUSY when max_pending jobs are waiting; LSB_ENOMEM when the job alone
- * exceeds the budget. */
+ * key_bits of key; 0 = 64) in the key order key_flags (0, LSB_KEYS_SIGNED or
+ * LSB_KEYS_FLOAT).  A must stay valid until the job completes.  LSB_EBUSY
+ * when max_pending jobs are waiting; LSB_ENOMEM when the job alone exceeds
+ * the budget. */
 int lsb_server_submit(lsb_server* srv, lsb_elem* A, size_t n, uint32_t key_bits,
-                      uint64_t* job_id);
+                      uint32_t key_flags, uint64_t* job_id);
 /* 1 = done, 0 = not yet, negative = the job failed with that code. */
 int lsb_server_poll(lsb_server* srv, uint64_t job_id);
 /* Waits up to timeout_s (< 0: forever); LSB_EBUSY on timeout. */
@@ -264,8 +275,8 @@ int lsb_generate(lsb_elem* A, size_t n, const lsb_gen_spec* spec, void* stream);
  * Synchronous.  *bad_count = failing positions, *first_bad = lowest one
  * (or -1). */
 int lsb_verify(const lsb_elem* out, const uint64_t* in_keys, size_t n,
-               uint64_t val_base, uint32_t key_bits, uint64_t* bad_count,
-               int64_t* first_bad, void* stream);
+               uint64_t val_base, uint32_t key_bits, uint32_t key_flags,
+               uint64_t* bad_count, int64_t* first_bad, void* stream);
 
 /* Copies the keys of A[0..n) into keys[0..n) (device), stream-ordered. */
 int lsb_extract_keys(const lsb_elem* A, size_t n, uint64_t* keys, void* stream);

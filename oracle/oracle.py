@@ -127,6 +127,34 @@ def stable_sort(a, key_bits=64):
     return A
 
 
+def key_order_fwd(keys, key_bits=64, order="unsigned"):
+    """Maps keys (uint64 array) to unsigned integers of the same order:
+    "signed" = two's complement of key_bits bits (flip the sign bit),
+    "float" = IEEE-754 binary64 / binary32-in-the-low-32-bits (negative:
+    flip all key_bits bits; positive: flip the sign bit).  The reference's
+    getDigit carries an unused `negs` argument for this
+    (chpl/arkouda-radix-sort.chpl:37-39).  Bits above key_bits pass through."""
+    k = np.asarray(keys, dtype=np.uint64)
+    if order == "unsigned":
+        return k.copy()
+    sign = np.uint64(1 << (key_bits - 1))
+    full = np.uint64((1 << key_bits) - 1) if key_bits < 64 else np.uint64(~0 & (2**64 - 1))
+    if order == "signed":
+        return k ^ sign
+    if order == "float":
+        neg = (k & sign) != 0
+        return np.where(neg, k ^ full, k ^ sign)
+    raise ValueError(order)
+
+
+def stable_sort_order(a, key_bits=64, order="unsigned"):
+    """Stable sort of elements a by the low key_bits of key_order_fwd(key)."""
+    a = np.asarray(a, dtype=ELEM)
+    mask = np.uint64((1 << key_bits) - 1) if key_bits < 64 else np.uint64(2**64 - 1)
+    t = key_order_fwd(a["key"], key_bits, order) & mask
+    return a[np.argsort(t, kind="stable")]
+
+
 CHECK_REASONS = {1: "val out of range", 2: "duplicate val (not a permutation)",
                  3: "key does not match its source", 4: "keys out of order",
                  5: "equal keys out of index order (unstable)"}

@@ -79,6 +79,14 @@ def test_plan_and_workspace():
         assert lsb_amd.plan(1 << 20, variant=v)[:2] == (d, t)
     c = lsb_amd.make_config(stream=0, variant=len(vs) + 1)
     assert lsb_amd.lib().lsb_plan(10, ctypes.byref(c), None, None, None) == lsb_amd.LSB_EINVAL
+    # key orders: signed any width, float 32 or 64 bits, never both
+    F, S = lsb_amd.LSB_KEYS_FLOAT, lsb_amd.LSB_KEYS_SIGNED
+    for kb, flags, ok in [(64, S, True), (13, S, True), (64, F, True), (32, F, True),
+                          (48, F, False), (64, F | S, False)]:
+        c = lsb_amd.make_config(key_bits=kb, stream=0, flags=flags)
+        rc = lsb_amd.lib().lsb_plan(1 << 20, ctypes.byref(c), None, None, None)
+        assert (rc == 0) == ok, (kb, flags, rc)
+        assert (lsb_amd.lib().lsb_workspace_size(1 << 20, ctypes.byref(c)) > 0) == ok
 
 
 def test_generate_and_verify_reject_bad_args():
@@ -88,9 +96,18 @@ def test_generate_and_verify_reject_bad_args():
     spec = lsb_amd.GenSpec(kind=lsb_amd.GEN_ZIPF, zipf_m=0, zipf_s=1.1)
     assert L.lsb_generate(0x10000, 10, ctypes.byref(spec), None) == lsb_amd.LSB_EINVAL
     bad, first = ctypes.c_uint64(), ctypes.c_int64()
-    assert L.lsb_verify(None, None, 0, 0, 64, ctypes.byref(bad), ctypes.byref(first), None) == 0
+    assert L.lsb_verify(None, None, 0, 0, 64, 0, ctypes.byref(bad), ctypes.byref(first),
+                        None) == 0
     assert first.value == -1
-    assert L.lsb_verify(None, None, 5, 0, 64, ctypes.byref(bad), ctypes.byref(first),
+    assert L.lsb_verify(None, None, 5, 0, 64, 0, ctypes.byref(bad), ctypes.byref(first),
+                        None) == lsb_amd.LSB_EINVAL
+    # key orders: at most one; floats are 32 or 64 bits wide
+    both = lsb_amd.LSB_KEYS_SIGNED | lsb_amd.LSB_KEYS_FLOAT
+    assert L.lsb_verify(None, None, 0, 0, 64, both, ctypes.byref(bad), ctypes.byref(first),
+                        None) == lsb_amd.LSB_EINVAL
+    assert L.lsb_verify(None, None, 0, 0, 48, lsb_amd.LSB_KEYS_FLOAT, ctypes.byref(bad),
+                        ctypes.byref(first), None) == lsb_amd.LSB_EINVAL
+    assert L.lsb_verify(None, None, 0, 0, 64, 64, ctypes.byref(bad), ctypes.byref(first),
                         None) == lsb_amd.LSB_EINVAL
 
 
@@ -135,7 +152,7 @@ def test_server_rejects_bad_config_without_gpu():
     assert L.lsb_server_create(ctypes.byref(bad), ctypes.byref(h)) == lsb_amd.LSB_EINVAL
     assert L.lsb_server_create(ctypes.byref(c), None) == lsb_amd.LSB_EINVAL
     jid = ctypes.c_uint64()
-    assert L.lsb_server_submit(None, None, 5, 64, ctypes.byref(jid)) == lsb_amd.LSB_EINVAL
+    assert L.lsb_server_submit(None, None, 5, 64, 0, ctypes.byref(jid)) == lsb_amd.LSB_EINVAL
 
 
 def test_tune_table_drives_auto_choice(tmp_path):

@@ -280,3 +280,76 @@ def test_autotune_table_and_sorts_after_tuning():
             assert np.array_equal(gpu_sort(a), stable(a))
     finally:
         lsb_amd.tune_clear()
+
+
+# ----------------------------------------------------------- key orders
+
+def _signed_input(n, key_bits, seed):
+    rng = np.random.default_rng(seed)
+    a = np.zeros(n, dtype=O.ELEM)
+    lo, hi = -(1 << (key_bits - 1)), (1 << (key_bits - 1))
+    v = rng.integers(lo, hi, size=n, dtype=np.int64)
+    q = n // 4
+    v[:q] = v[q:2 * q]  # duplicates: stability matters
+    mask = (1 << key_bits) - 1 if key_bits < 64 else None
+    a["key"] = v.view(np.uint64) if mask is None else (v & mask).astype(np.uint64)
+    a["val"] = np.arange(n, dtype=np.uint64)
+    return a
+
+
+def _float_input(n, key_bits, seed):
+    rng = np.random.default_rng(seed)
+    a = np.zeros(n, dtype=O.ELEM)
+    f = rng.standard_normal(n) * 10.0 ** rng.integers(-30, 30, n)
+    f[: n // 8] = -0.0
+    f[n // 8: n // 4] = 0.0
+    f[n // 4: n // 4 + 3] = [np.inf, -np.inf, np.nan]
+    if key_bits == 32:
+        a["key"] = f.astype(np.float32).view(np.uint32).astype(np.uint64)
+    else:
+        a["key"] = f.view(np.uint64)
+    a["val"] = np.arange(n, dtype=np.uint64)
+    return a
+
+
+@pytest.mark.parametrize("order,key_bits", [("signed", 64), ("signed", 13), ("float", 64),
+                                            ("float", 32)])
+@pytest.mark.parametrize("n", [1000, 8192, 100003])
+def test_key_orders_all_variants(order, key_bits, n):
+    """Signed and float keys sort bit-exactly as the oracle's order, through
+    the one-launch LDS sort and every compiled scatter variant."""
+    make = _signed_input if order == "signed" else _float_input
+    a = make(n, key_bits, seed=n + key_bits)
+    want = O.stable_sort_order(a, key_bits, order)
+    assert np.array_equal(gpu_sort(a, key_bits=key_bits, key_order=order), want)
+    for v, d, tile, _ in variants():
+        got = gpu_sort(a, key_bits=key_bits, key_order=order, variant=v)
+        assert np.array_equal(got, want), (order, key_bits, n, v, d, tile)
+
+
+def test_key_order_verifier_and_skip():
+    """The device verifier checks the declared order (an unsigned-sorted
+    signed array fails it), and pass skipping works on the mapped keys
+    (non-negative signed values below 2^16: six of eight mapped digits are
+    constant)."""
+    n = 300_000
+    rng = np.random.default_rng(11)
+    a = np.zeros(n, dtype=O.ELEM)
+    a["val"] = np.arange(n, dtype=np.uint64)
+    B = torch.empty((n, 2), dtype=torch.int64, device=DEV)
+    a["key"] = rng.integers(0, 60000, size=n, dtype=np.int64).view(np.uint64)
+    A = to_dev(a)
+    lsb_amd.sort(A, B, sync=True, key_order="signed", profile=True)
+    assert sum(lsb_amd.profile_read()["executed"]) == 2
+    assert np.array_equal(from_dev(A), O.stable_sort_order(a, 64, "signed"))
+
+    a["key"] = rng.integers(-1000, 1000, size=n, dtype=np.int64).view(np.uint64)
+    keys = torch.from_numpy(a["key"].view(np.int64).copy()).to(DEV)
+    A = to_dev(a)
+    lsb_amd.sort(A, B, sync=True, key_order="signed")
+    assert np.array_equal(from_dev(A), O.stable_sort_order(a, 64, "signed"))
+    assert lsb_amd.verify(A, keys, n=n, key_order="signed")[0] == 0
+    assert lsb_amd.verify(A, keys, n=n)[0] > 0  # not the unsigned order
+    U = to_dev(a)
+    lsb_amd.sort(U, B, sync=True)
+    assert lsb_amd.verify(U, keys, n=n, key_order="signed")[0] > 0

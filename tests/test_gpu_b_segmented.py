@@ -89,6 +89,32 @@ def test_segmented_key_bits(key_bits):
     assert np.array_equal(from_dev(A), want)
 
 
+@pytest.mark.parametrize("order", ["signed", "float"])
+def test_segmented_key_orders(order):
+    """Signed / float keys per segment (LDS-sorted small segments and the
+    tile pipeline alike) equal the oracle's order of each segment alone."""
+    sizes = [3000, 40000, 10000, 1, 8192, 70001, 5]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    n = int(off[-1])
+    rng = np.random.default_rng(5)
+    a = np.zeros(n, dtype=O.ELEM)
+    if order == "signed":
+        a["key"] = rng.integers(-(1 << 40), 1 << 40, size=n, dtype=np.int64).view(np.uint64)
+    else:
+        f = rng.standard_normal(n) * 1e6
+        f[::7] = -0.0
+        a["key"] = f.view(np.uint64)
+    a["val"] = np.arange(n, dtype=np.uint64)
+    want = a.copy()
+    for s_ in range(len(sizes)):
+        lo, hi = int(off[s_]), int(off[s_ + 1])
+        want[lo:hi] = O.stable_sort_order(a[lo:hi], 64, order)
+    A = to_dev(a)
+    B = torch.empty_like(A)
+    lsb_amd.sort_segmented(A, B, off, key_order=order, sync=True)
+    assert np.array_equal(from_dev(A), want)
+
+
 def test_segment_alone_equals_segment_in_batch():
     """A segment's result does not depend on its neighbours."""
     a, off = batch([20000, 5000, 90000, 3], seed=5)

@@ -72,6 +72,33 @@ def test_each_job_same_alone_and_in_the_mix():
         assert np.array_equal(got, alone(a, kb))
 
 
+def test_key_orders_in_the_mix():
+    """Jobs of different key orders are never batched together: signed,
+    float and unsigned jobs of equal width submitted at once each come back
+    in their own order."""
+    rng = np.random.default_rng(21)
+    jobs = []
+    for i in range(36):
+        n = int(2 ** rng.uniform(4, 17))
+        order = ("unsigned", "signed", "float")[i % 3]
+        a = np.zeros(n, dtype=O.ELEM)
+        if order == "float":
+            a["key"] = (rng.standard_normal(n) * 100).view(np.uint64)
+        else:
+            a["key"] = rng.integers(-(1 << 62), 1 << 62, size=n, dtype=np.int64).view(np.uint64)
+        a["val"] = np.arange(n, dtype=np.uint64)
+        jobs.append((a, order))
+    with lsb_amd.Server(streams=2) as srv:
+        bufs = [to_dev(a) for a, _ in jobs]
+        torch.cuda.synchronize()
+        for buf, (_, order) in zip(bufs, jobs):
+            srv.submit(buf, key_order=order)
+        srv.drain()
+        assert srv.stats()["batched_jobs"] > 0
+    for (a, order), buf in zip(jobs, bufs):
+        assert np.array_equal(from_dev(buf), O.stable_sort_order(a, 64, order)), order
+
+
 def test_coalescing_happens_under_load():
     """Many small jobs submitted at once: most must run inside batches."""
     rng = np.random.default_rng(5)

@@ -206,3 +206,58 @@ def test_product_workload_definitions_match_oracle():
     assert min(sizes) >= 1 << 12 and max(sizes) < 1 << 28
     assert 60 < sum(j["key_bits"] == 32 for j in t) < 140
     assert 60 < sum(j["zipf"] for j in t) < 140
+
+
+# ----------------------------------------------------------- key orders
+# key_order_fwd is pinned against numpy's own signed-integer and float
+# comparisons (a different implementation of the same order), plus the
+# IEEE special values whose place numpy does not define (-0 vs +0, NaNs).
+
+def test_key_order_signed_matches_numpy_int_order():
+    rng = np.random.default_rng(7)
+    v = rng.integers(-2**63, 2**63 - 1, size=20000, dtype=np.int64)
+    v[:50] = v[50:100]  # ties keep index order
+    k = v.view(np.uint64)
+    assert np.array_equal(np.argsort(O.key_order_fwd(k, 64, "signed"), kind="stable"),
+                          np.argsort(v, kind="stable"))
+    # 13-bit two's complement held in the low bits
+    w = rng.integers(-4096, 4096, size=5000)
+    k13 = (w & 0x1FFF).astype(np.uint64)
+    t = O.key_order_fwd(k13, 13, "signed") & np.uint64(0x1FFF)
+    assert np.array_equal(np.argsort(t, kind="stable"), np.argsort(w, kind="stable"))
+
+
+def test_key_order_float_matches_numpy_float_order():
+    rng = np.random.default_rng(8)
+    f = np.concatenate([rng.standard_normal(10000) * 10.0 ** rng.integers(-300, 300, 10000),
+                        [np.inf, -np.inf, 5e-324, -5e-324, 1.0, -1.0, 0.0]])
+    rng.shuffle(f)
+    assert np.array_equal(np.argsort(O.key_order_fwd(f.view(np.uint64), 64, "float"),
+                                     kind="stable"), np.argsort(f, kind="stable"))
+    g = (rng.standard_normal(8000) * 1e30).astype(np.float32)
+    k32 = g.view(np.uint32).astype(np.uint64) | np.uint64(0xABCD << 40)  # junk above bit 31
+    t = O.key_order_fwd(k32, 32, "float") & np.uint64(0xFFFFFFFF)
+    assert np.array_equal(np.argsort(t, kind="stable"), np.argsort(g, kind="stable"))
+
+
+def test_key_order_float_special_values():
+    neg_nan = np.array([0xFFF8000000000000], dtype=np.uint64).view(np.float64)[0]
+    want = [neg_nan, -np.inf, -1.5, -0.0, 0.0, 1e-310, 2.0, np.inf, np.nan]
+    vals = np.array(want, dtype=np.float64)
+    perm = np.array([5, 0, 8, 3, 1, 7, 2, 6, 4])
+    got = vals[perm][np.argsort(O.key_order_fwd(vals[perm].view(np.uint64), 64, "float"),
+                                kind="stable")]
+    assert np.array_equal(got.view(np.uint64), vals.view(np.uint64))
+
+
+def test_key_order_inverse_and_unsigned_identity():
+    rng = np.random.default_rng(9)
+    k = rng.integers(0, 2**64 - 1, size=4096, dtype=np.uint64)
+    assert np.array_equal(O.key_order_fwd(k, 64, "unsigned"), k)
+    for order in ("signed", "float"):
+        t = O.key_order_fwd(k, 64, order)
+        assert len(np.unique(t)) == len(np.unique(k))  # a bijection on 64-bit keys
+    a = np.zeros(6, dtype=O.ELEM)
+    a["key"] = np.array([3, 1, 2, 1, 0, 5], dtype=np.uint64)
+    a["val"] = np.arange(6)
+    assert np.array_equal(O.stable_sort_order(a), O.stable_sort(a))
