@@ -498,8 +498,9 @@ uint32_t device_cus() {
   return (uint32_t)cus;
 }
 
-int enqueue_segmented(lsb_elem* A, lsb_elem* B, const SegLayout& L, char* ws, uint32_t flags,
-                      hipStream_t s) {
+// The kernels reach every segment through the table in the workspace, so
+// the sequence depends on the workspace and its layout only.
+int enqueue_segmented(const SegLayout& L, char* ws, uint32_t flags, hipStream_t s) {
   const lsb::SegHeader* hdr = reinterpret_cast<const lsb::SegHeader*>(ws + L.hdr);
   const lsb::SegLarge* large = reinterpret_cast<const lsb::SegLarge*>(ws + L.meta);
   const lsb::SegSmall* small = reinterpret_cast<const lsb::SegSmall*>(ws + L.meta);
@@ -508,14 +509,12 @@ int enqueue_segmented(lsb_elem* A, lsb_elem* B, const SegLayout& L, char* ws, ui
   uint32_t* tickets = reinterpret_cast<uint32_t*>(ws + L.tickets);
   uint32_t* status = reinterpret_cast<uint32_t*>(ws + L.status);
   const uint32_t cus = device_cus();
-  LSB_HIP(lsb::launch_seg_prepare(L.d, A, hdr, large, L.tile, L.key_bits, L.ko, L.passes, counts,
+  LSB_HIP(lsb::launch_seg_prepare(L.d, hdr, large, L.tile, L.key_bits, L.ko, L.passes, counts,
                                   plan, tickets, status, flags, 4 * cus, s));
-  LSB_HIP(lsb::launch_seg_small(A, hdr, small, L.key_bits, L.ko, 4 * cus, s));
+  LSB_HIP(lsb::launch_seg_small(hdr, small, L.key_bits, L.ko, 4 * cus, s));
   const uint32_t grid = L.v->blocks_per_cu() * cus;
   for (uint32_t q = 0; q < L.passes; ++q) {
     lsb::SegPassArgs a;
-    a.A = A;
-    a.B = B;
     a.hdr = hdr;
     a.large = large;
     a.seg_plan = plan;
@@ -530,7 +529,7 @@ int enqueue_segmented(lsb_elem* A, lsb_elem* B, const SegLayout& L, char* ws, ui
     a.ko = L.ko;
     LSB_HIP(L.v->scatter(a, grid, s));
   }
-  LSB_HIP(lsb::launch_seg_copy(A, B, hdr, large, plan, L.tile, 4 * cus, s));
+  LSB_HIP(lsb::launch_seg_copy(hdr, large, plan, L.tile, 4 * cus, s));
   return LSB_OK;
 }
 
@@ -644,37 +643,36 @@ size_t lsb_segmented_workspace_size(size_t total_n, uint32_t nseg, const lsb_con
   return L.total;
 }
 
-int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, uint32_t nseg,
-                       const lsb_config* cfg) {
-  const lsb_config c = cfg ? *cfg : defaults();
-  if (nseg == 0) return LSB_OK;
-  if (!seg_offsets) return LSB_EINVAL;
-  if (c.flags & LSB_PROFILE) return LSB_EINVAL;
-  const uint32_t kb = c.key_bits ? c.key_bits : 64;
-  if (kb > 64) return LSB_EINVAL;
-  const uint64_t total_n = seg_offsets[nseg];
+}  // extern "C"
+
+namespace {
+
+// One segment of a batch: sorted in place in a, with b as its scratch.
+struct SegEntry {
+  lsb_elem* a;
+  lsb_elem* b;
+  uint64_t n;
+};
+
+// Sorts every entry on its own (shared body of lsb_sort_segmented and
+// lsb_sort_batch).  total_n = sum of the entries' sizes.
+int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_config& c) {
+  const uint32_t nseg = (uint32_t)ent.size();
   uint32_t n_large = 0, n_tiny = 0, n_small = 0;
   // The tile pipeline runs the tiles of every large segment in one
   // persistent launch per pass, so its variant follows the tuned choice for
   // one array of all their elements together (typical_n).
   size_t typical_n = 0;
-  for (uint32_t i = 0; i < nseg; ++i) {
-    if (seg_offsets[i + 1] < seg_offsets[i]) return LSB_EINVAL;
-    const uint64_t n = seg_offsets[i + 1] - seg_offsets[i];
-    if (n > LSB_MAX_N) return LSB_ERANGE;
-    if (n <= 1) continue;
-    if (n <= lsb::seg_tiny_capacity()) ++n_tiny;
-    else if (n <= lsb::seg_small_capacity()) ++n_small;
-    else { ++n_large; typical_n += n; }
+  for (const SegEntry& e : ent) {
+    if (e.n <= 1) continue;
+    if (e.n <= lsb::seg_tiny_capacity()) ++n_tiny;
+    else if (e.n <= lsb::seg_small_capacity()) ++n_small;
+    else { ++n_large; typical_n += e.n; }
   }
   if (n_large + n_tiny + n_small == 0) return LSB_OK;
-  if (!A || !B || (((uintptr_t)A | (uintptr_t)B) & 15u)) return LSB_EINVAL;
-  if (overlaps(A, B, total_n * sizeof(lsb_elem))) return LSB_EINVAL;
   int rc;
   SegLayout L;
   if ((rc = make_seg_layout(total_n, nseg, c, &L, typical_n))) return rc;
-  const size_t bytes = total_n * sizeof(lsb_elem);
-  if ((rc = check_device_range(A, bytes)) || (rc = check_device_range(B, bytes))) return rc;
   hipStream_t s = (hipStream_t)c.stream;
   int dev = 0;
   LSB_HIP(hipGetDevice(&dev));
@@ -703,7 +701,9 @@ int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, ui
         W.ptr = nullptr;
         W.bytes = 0;
       }
-      if (hipMalloc(&W.ptr, L.total) != hipSuccess) {
+      const hipError_t he = hipMalloc(&W.ptr, L.total);
+      if (he != hipSuccess) {
+        g_last_hip = (int)he;
         (void)hipGetLastError();
         W.ptr = nullptr;
         W.cap_n = 0;
@@ -717,8 +717,10 @@ int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, ui
     ws = static_cast<char*>(W.ptr);
   }
 
-  // Stage the batch shape in pinned memory and upload it on the stream.
-  const size_t meta_bytes = L.meta - L.hdr + (size_t)(n_large + n_tiny + n_small) * 16;
+  // Stage the batch's segment table in pinned memory and upload it on the
+  // stream (outside any captured graph: replays read the new table).
+  const size_t meta_bytes = L.meta - L.hdr + (size_t)n_large * sizeof(lsb::SegLarge) +
+                            (size_t)(n_tiny + n_small) * sizeof(lsb::SegSmall);
   if (W.stage_bytes < meta_bytes) {
     if (W.stage) {
       if (W.staged) LSB_HIP(hipEventSynchronize(W.staged));
@@ -736,16 +738,16 @@ int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, ui
   lsb::SegSmall* tiny = reinterpret_cast<lsb::SegSmall*>(lg + n_large);
   lsb::SegSmall* sml = tiny + n_tiny;
   uint32_t il = 0, it = 0, is = 0, tiles = 0;
-  for (uint32_t i = 0; i < nseg; ++i) {
-    const uint64_t base = seg_offsets[i], n = seg_offsets[i + 1] - seg_offsets[i];
-    if (n <= 1) continue;
-    if (n <= lsb::seg_tiny_capacity()) {
-      tiny[it++] = lsb::SegSmall{base, (uint32_t)n, 0u};
-    } else if (n <= lsb::seg_small_capacity()) {
-      sml[is++] = lsb::SegSmall{base, (uint32_t)n, 0u};
+  for (const SegEntry& e : ent) {
+    if (e.n <= 1) continue;
+    const uint64_t a = (uint64_t)(uintptr_t)e.a, b = (uint64_t)(uintptr_t)e.b;
+    if (e.n <= lsb::seg_tiny_capacity()) {
+      tiny[it++] = lsb::SegSmall{a, (uint32_t)e.n, 0u};
+    } else if (e.n <= lsb::seg_small_capacity()) {
+      sml[is++] = lsb::SegSmall{a, (uint32_t)e.n, 0u};
     } else {
-      lg[il++] = lsb::SegLarge{base, (uint32_t)n, tiles};
-      tiles += (uint32_t)((n + L.tile - 1) / L.tile);
+      lg[il++] = lsb::SegLarge{a, b, (uint32_t)e.n, tiles};
+      tiles += (uint32_t)((e.n + L.tile - 1) / L.tile);
     }
   }
   hdr->n_large = n_large;
@@ -757,19 +759,73 @@ int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, ui
   LSB_HIP(hipEventRecord(W.staged, s));
 
   if (c.flags & LSB_GRAPH) {
-    const std::vector<uint64_t> key = {1u, (uint64_t)(uintptr_t)A, (uint64_t)(uintptr_t)B,
-                                       (uint64_t)(uintptr_t)ws, L.cap_n, L.cap_segs, L.d, L.tile,
-                                       L.key_bits, (uint64_t)(c.flags & ~LSB_SYNC),
+    const std::vector<uint64_t> key = {1u, (uint64_t)(uintptr_t)ws, L.cap_n, L.cap_segs, L.d,
+                                       L.tile, L.key_bits, (uint64_t)(c.flags & ~LSB_SYNC),
                                        (uint64_t)dev};
     rc = launch_graph(key, s, [&](hipStream_t cs) {
-      return enqueue_segmented(A, B, L, ws, c.flags, cs);
+      return enqueue_segmented(L, ws, c.flags, cs);
     });
   } else {
-    rc = enqueue_segmented(A, B, L, ws, c.flags, s);
+    rc = enqueue_segmented(L, ws, c.flags, s);
   }
   if (rc) return rc;
   if (c.flags & LSB_SYNC) LSB_HIP(hipStreamSynchronize(s));
   return LSB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, uint32_t nseg,
+                       const lsb_config* cfg) {
+  const lsb_config c = cfg ? *cfg : defaults();
+  if (nseg == 0) return LSB_OK;
+  if (!seg_offsets) return LSB_EINVAL;
+  if (c.flags & LSB_PROFILE) return LSB_EINVAL;
+  if ((c.key_bits ? c.key_bits : 64) > 64) return LSB_EINVAL;
+  const uint64_t total_n = seg_offsets[nseg];
+  bool any = false;
+  for (uint32_t i = 0; i < nseg; ++i) {
+    if (seg_offsets[i + 1] < seg_offsets[i]) return LSB_EINVAL;
+    const uint64_t n = seg_offsets[i + 1] - seg_offsets[i];
+    if (n > LSB_MAX_N) return LSB_ERANGE;
+    any |= n > 1;
+  }
+  if (!any) return LSB_OK;
+  if (!A || !B || (((uintptr_t)A | (uintptr_t)B) & 15u)) return LSB_EINVAL;
+  if (overlaps(A, B, total_n * sizeof(lsb_elem))) return LSB_EINVAL;
+  int rc;
+  const size_t bytes = total_n * sizeof(lsb_elem);
+  if ((rc = check_device_range(A, bytes)) || (rc = check_device_range(B, bytes))) return rc;
+  std::vector<SegEntry> ent(nseg);
+  for (uint32_t i = 0; i < nseg; ++i)
+    ent[i] = SegEntry{A + seg_offsets[i], B + seg_offsets[i], seg_offsets[i + 1] - seg_offsets[i]};
+  return sort_entries(ent, total_n, c);
+}
+
+int lsb_sort_batch(lsb_elem* const* As, lsb_elem* const* Bs, const uint64_t* ns, uint32_t nseg,
+                   const lsb_config* cfg) {
+  const lsb_config c = cfg ? *cfg : defaults();
+  if (nseg == 0) return LSB_OK;
+  if (!As || !ns) return LSB_EINVAL;
+  if (c.flags & LSB_PROFILE) return LSB_EINVAL;
+  if ((c.key_bits ? c.key_bits : 64) > 64) return LSB_EINVAL;
+  uint64_t total_n = 0;
+  std::vector<SegEntry> ent(nseg);
+  for (uint32_t i = 0; i < nseg; ++i) {
+    const uint64_t n = ns[i];
+    if (n > LSB_MAX_N) return LSB_ERANGE;
+    lsb_elem* b = Bs ? Bs[i] : nullptr;
+    if (n > 1 && (!As[i] || ((uintptr_t)As[i] & 15u))) return LSB_EINVAL;
+    if (n > lsb::seg_small_capacity()) {
+      if (!b || ((uintptr_t)b & 15u)) return LSB_EINVAL;
+      if (As[i] < b + n && b < As[i] + n) return LSB_EINVAL;  // A and its scratch overlap
+    }
+    ent[i] = SegEntry{As[i], b, n};
+    total_n += n;
+  }
+  return sort_entries(ent, total_n, c);
 }
 
 // ------------------------------------------------------------- autotuner

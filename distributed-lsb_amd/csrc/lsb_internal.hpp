@@ -91,19 +91,22 @@ struct SegHeader {
   uint32_t n_small;      // segments in (tiny, seg_small_capacity()]
   uint32_t total_tiles;  // tiles over all large segments
 };
+// Segment tables hold device addresses, so one launch sequence serves
+// segments carved out of one array (lsb_sort_segmented) or scattered over
+// separate buffers (lsb_sort_batch) alike.  Large entries come first, then
+// the tiny ones, then the small ones.
 struct SegLarge {
-  uint64_t base;        // first element of the segment in A and B
+  uint64_t a;           // the segment's elements (sorted in place)
+  uint64_t b;           // its scratch of n elements
   uint32_t n;
   uint32_t tile_begin;  // global index of the segment's first tile
 };
 struct SegSmall {
-  uint64_t base;
+  uint64_t a;
   uint32_t n;
   uint32_t pad;
 };
 struct SegPassArgs {
-  void* A;
-  void* B;
   const SegHeader* hdr;
   const SegLarge* large;
   const uint32_t* seg_plan;  // [2*s] executed-pass mask, [2*s+1] result-in-B flag
@@ -127,14 +130,14 @@ struct SegVariant {
 const SegVariant* seg_variants(int* count);
 uint32_t seg_tiny_capacity();
 uint32_t seg_small_capacity();
-hipError_t launch_seg_small(void* A, const SegHeader* hdr, const SegSmall* list,
+hipError_t launch_seg_small(const SegHeader* hdr, const SegSmall* list,
                             uint32_t key_bits, const KeyOrder& ko, uint32_t grid, hipStream_t s);
-hipError_t launch_seg_prepare(uint32_t d, const void* A, const SegHeader* hdr,
+hipError_t launch_seg_prepare(uint32_t d, const SegHeader* hdr,
                               const SegLarge* large, uint32_t tile, uint32_t key_bits,
                               const KeyOrder& ko, uint32_t passes, uint32_t* counts, uint32_t* seg_plan,
                               uint32_t* tickets, uint32_t* status0, uint32_t flags,
                               uint32_t grid, hipStream_t s);
-hipError_t launch_seg_copy(void* A, const void* B, const SegHeader* hdr, const SegLarge* large,
+hipError_t launch_seg_copy(const SegHeader* hdr, const SegLarge* large,
                            const uint32_t* seg_plan, uint32_t tile, uint32_t grid,
                            hipStream_t s);
 

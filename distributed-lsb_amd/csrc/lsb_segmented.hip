@@ -37,17 +37,19 @@ __device__ __forceinline__ uint32_t find_segment(const SegLarge* __restrict__ la
 
 // ----------------------------------------------------------- small segments
 template <int D, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void k_seg_small(elem_t* A, const SegHeader* __restrict__ hdr,
+__global__ __launch_bounds__(BLOCK) void k_seg_small(const SegHeader* __restrict__ hdr,
                                                      const SegSmall* __restrict__ list,
                                                      int which, uint32_t key_bits, KeyOrder ko) {
   __shared__ TileSmem<D, BLOCK, ITEMS> sm;
   // The metadata array holds the large segments, then the tiny, then the
-  // small ones (16 bytes each).
+  // small ones.
   const uint32_t count = which == 0 ? hdr->n_tiny : hdr->n_small;
-  const SegSmall* l = list + hdr->n_large + (which == 0 ? 0u : hdr->n_tiny);
+  const SegSmall* l = reinterpret_cast<const SegSmall*>(
+                          reinterpret_cast<const SegLarge*>(list) + hdr->n_large) +
+                      (which == 0 ? 0u : hdr->n_tiny);
   for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
     const SegSmall s = l[i];
-    small_sort<D, BLOCK, ITEMS>(sm, A + s.base, s.n, key_bits, ko);
+    small_sort<D, BLOCK, ITEMS>(sm, reinterpret_cast<elem_t*>(s.a), s.n, key_bits, ko);
     __syncthreads();
   }
 }
@@ -65,8 +67,7 @@ __global__ __launch_bounds__(256) void k_seg_zero(const SegHeader* __restrict__ 
 // of consecutive tiles of one segment in LDS and flushes them when the
 // segment changes.
 template <int D>
-__global__ __launch_bounds__(256) void k_seg_hist(const elem_t* __restrict__ A,
-                                                  const SegHeader* __restrict__ hdr,
+__global__ __launch_bounds__(256) void k_seg_hist(const SegHeader* __restrict__ hdr,
                                                   const SegLarge* __restrict__ large,
                                                   uint32_t tile, uint32_t key_bits, KeyOrder ko,
                                                   uint32_t passes, uint32_t* __restrict__ counts,
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(256) void k_seg_hist(const elem_t* __restrict__ A,
     const SegLarge L = large[seg];
     const uint32_t first = (tl - L.tile_begin) * tile;
     const uint32_t cnt = L.n - first < tile ? L.n - first : tile;
-    const elem_t* a = A + L.base + first;
+    const elem_t* a = reinterpret_cast<const elem_t*>(L.a) + first;
     for (uint32_t i0 = 0; i0 < cnt; i0 += 256 * 4) {
       uint64_t k[4];
       bool v[4];
@@ -205,8 +206,8 @@ __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(SegPassArgs args) {
     if (mask & (1u << pass)) {
       const SegLarge L = args.large[seg];
       const uint32_t ord = (uint32_t)__popc(mask & ((1u << pass) - 1u));
-      elem_t* a = static_cast<elem_t*>(args.A) + L.base;
-      elem_t* b = static_cast<elem_t*>(args.B) + L.base;
+      elem_t* a = reinterpret_cast<elem_t*>(L.a);
+      elem_t* b = reinterpret_cast<elem_t*>(L.b);
       const elem_t* src = (ord & 1u) ? b : a;
       elem_t* dst = (ord & 1u) ? a : b;
       uint32_t* st_cur = args.status + (size_t)(ord & 1u) * slice + (size_t)L.tile_begin * NB;
@@ -222,9 +223,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(SegPassArgs args) {
 }
 
 // -------------------------------------------------------------- k_seg_copy
-__global__ __launch_bounds__(256) void k_seg_copy(elem_t* __restrict__ A,
-                                                  const elem_t* __restrict__ B,
-                                                  const SegHeader* __restrict__ hdr,
+__global__ __launch_bounds__(256) void k_seg_copy(const SegHeader* __restrict__ hdr,
                                                   const SegLarge* __restrict__ large,
                                                   const uint32_t* __restrict__ seg_plan,
                                                   uint32_t tile) {
@@ -237,7 +236,7 @@ __global__ __launch_bounds__(256) void k_seg_copy(elem_t* __restrict__ A,
     const uint32_t first = (tl - L.tile_begin) * tile;
     const uint32_t cnt = L.n - first < tile ? L.n - first : tile;
     for (uint32_t i = threadIdx.x; i < cnt; i += 256)
-      A[L.base + first + i] = B[L.base + first + i];
+      reinterpret_cast<elem_t*>(L.a)[first + i] = reinterpret_cast<const elem_t*>(L.b)[first + i];
   }
 }
 
@@ -274,32 +273,31 @@ const SegVariant* seg_variants(int* count) {
 uint32_t seg_tiny_capacity() { return 256 * 8; }
 uint32_t seg_small_capacity() { return 512 * 16; }
 
-hipError_t launch_seg_small(void* A, const SegHeader* hdr, const SegSmall* list,
+hipError_t launch_seg_small(const SegHeader* hdr, const SegSmall* list,
                             uint32_t key_bits, const KeyOrder& ko, uint32_t grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_seg_small<8, 256, 8>), dim3(grid), dim3(256), 0, s,
-                     static_cast<elem_t*>(A), hdr, list, 0, key_bits, ko);
-  hipLaunchKernelGGL((k_seg_small<8, 512, 16>), dim3(grid), dim3(512), 0, s,
-                     static_cast<elem_t*>(A), hdr, list, 1, key_bits, ko);
+  hipLaunchKernelGGL((k_seg_small<8, 256, 8>), dim3(grid), dim3(256), 0, s, hdr, list, 0,
+                     key_bits, ko);
+  hipLaunchKernelGGL((k_seg_small<8, 512, 16>), dim3(grid), dim3(512), 0, s, hdr, list, 1,
+                     key_bits, ko);
   return hipGetLastError();
 }
 
-hipError_t launch_seg_prepare(uint32_t d, const void* A, const SegHeader* hdr,
+hipError_t launch_seg_prepare(uint32_t d, const SegHeader* hdr,
                               const SegLarge* large, uint32_t tile, uint32_t key_bits,
                               const KeyOrder& ko, uint32_t passes, uint32_t* counts, uint32_t* seg_plan,
                               uint32_t* tickets, uint32_t* status0, uint32_t flags,
                               uint32_t grid, hipStream_t s) {
-  const elem_t* a = static_cast<const elem_t*>(A);
   if (d == 8) {
     constexpr int MAXP = 8;
     hipLaunchKernelGGL(k_seg_zero, dim3(grid), dim3(256), 0, s, hdr, counts, MAXP * 256u);
-    hipLaunchKernelGGL(k_seg_hist<8>, dim3(grid), dim3(256), 0, s, a, hdr, large, tile,
+    hipLaunchKernelGGL(k_seg_hist<8>, dim3(grid), dim3(256), 0, s, hdr, large, tile,
                        key_bits, ko, passes, counts, status0);
     hipLaunchKernelGGL(k_seg_scan<8>, dim3(grid), dim3(1024), 0, s, hdr, large, counts,
                        seg_plan, tickets, passes, flags);
   } else if (d == 11) {
     constexpr int MAXP = 6;
     hipLaunchKernelGGL(k_seg_zero, dim3(grid), dim3(256), 0, s, hdr, counts, MAXP * 2048u);
-    hipLaunchKernelGGL(k_seg_hist<11>, dim3(grid), dim3(256), 0, s, a, hdr, large, tile,
+    hipLaunchKernelGGL(k_seg_hist<11>, dim3(grid), dim3(256), 0, s, hdr, large, tile,
                        key_bits, ko, passes, counts, status0);
     hipLaunchKernelGGL(k_seg_scan<11>, dim3(grid), dim3(1024), 0, s, hdr, large, counts,
                        seg_plan, tickets, passes, flags);
@@ -309,11 +307,10 @@ hipError_t launch_seg_prepare(uint32_t d, const void* A, const SegHeader* hdr,
   return hipGetLastError();
 }
 
-hipError_t launch_seg_copy(void* A, const void* B, const SegHeader* hdr, const SegLarge* large,
+hipError_t launch_seg_copy(const SegHeader* hdr, const SegLarge* large,
                            const uint32_t* seg_plan, uint32_t tile, uint32_t grid,
                            hipStream_t s) {
-  hipLaunchKernelGGL(k_seg_copy, dim3(grid), dim3(256), 0, s, static_cast<elem_t*>(A),
-                     static_cast<const elem_t*>(B), hdr, large, seg_plan, tile);
+  hipLaunchKernelGGL(k_seg_copy, dim3(grid), dim3(256), 0, s, hdr, large, seg_plan, tile);
   return hipGetLastError();
 }
 

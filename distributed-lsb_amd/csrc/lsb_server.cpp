@@ -5,9 +5,10 @@
 // scheduler thread
 //   * admits jobs in arrival order against a memory budget that charges each
 //     job its doubled buffers (A + B, 32 bytes per element);
-//   * coalesces admitted small jobs of one key width into a batch: their
-//     elements are copied into a per-stream staging slab, sorted by one
-//     segmented launch (replayed from a captured hipGraph), and copied back;
+//   * coalesces admitted small jobs of one key width and key order into a
+//     batch sorted in place by one batched launch sequence (lsb_sort_batch,
+//     replayed from a captured hipGraph) with a per-stream scratch slab --
+//     no staging copies;
 //   * launches larger jobs as plain sorts with server-allocated scratch;
 //   * spreads launches over several streams so jobs that arrive while others
 //     run overlap on the device;
@@ -61,8 +62,7 @@ struct Launch {
 
 struct StreamSlot {
   hipStream_t s = nullptr;
-  lsb_elem* slab_a = nullptr;  // staging for coalesced batches
-  lsb_elem* slab_b = nullptr;
+  lsb_elem* slab_b = nullptr;  // scratch of coalesced batches
   hipEvent_t slab_free = nullptr;  // recorded after the last batch on this slot
   int inflight = 0;
 };
@@ -113,7 +113,7 @@ int hip_rc(hipError_t e) {
 }  // namespace
 
 int lsb_server::pick_slot(bool need_slab) {
-  // Least-loaded stream; a batch also needs the slot's staging slab free.
+  // Least-loaded stream; a batch also needs the slot's scratch slab free.
   int best = -1;
   for (int k = 0; k < (int)slots.size(); ++k) {
     const int i = (next_slot + k) % (int)slots.size();
@@ -163,30 +163,27 @@ int lsb_server::launch_plain(Job& j) {
 
 int lsb_server::launch_batch(uint32_t key_class, std::vector<uint64_t>& ids) {
   const int si = pick_slot(true);
-  if (si < 0) return LSB_EBUSY;  // every staging slab still busy: retry later
+  if (si < 0) return LSB_EBUSY;  // every scratch slab still busy: retry later
   StreamSlot& sl = slots[si];
-  std::vector<uint64_t> offsets(1, 0);
-  for (uint64_t id : ids) offsets.push_back(offsets.back() + jobs[id].n);
-  int rc = LSB_OK;
-  for (size_t k = 0; k < ids.size() && rc == LSB_OK; ++k) {
+  // Each job is sorted in place in its own buffer; its scratch is a slice of
+  // the slot's slab.
+  std::vector<lsb_elem*> as(ids.size()), bs(ids.size());
+  std::vector<uint64_t> ns(ids.size());
+  uint64_t off = 0;
+  for (size_t k = 0; k < ids.size(); ++k) {
     const Job& j = jobs[ids[k]];
-    rc = hip_rc(hipMemcpyAsync(sl.slab_a + offsets[k], j.A, j.n * sizeof(lsb_elem),
-                               hipMemcpyDeviceToDevice, sl.s));
+    as[k] = j.A;
+    bs[k] = sl.slab_b + off;
+    ns[k] = j.n;
+    off += j.n;
   }
-  if (rc == LSB_OK) {
-    lsb_config c;
-    lsb_config_init(&c);
-    c.key_bits = key_class & 0xFFu;
-    c.flags = key_class >> 8;
-    c.stream = sl.s;
-    if (cfg.use_graphs) c.flags |= LSB_GRAPH;
-    rc = lsb_sort_segmented(sl.slab_a, sl.slab_b, offsets.data(), (uint32_t)ids.size(), &c);
-  }
-  for (size_t k = 0; k < ids.size() && rc == LSB_OK; ++k) {
-    const Job& j = jobs[ids[k]];
-    rc = hip_rc(hipMemcpyAsync(j.A, sl.slab_a + offsets[k], j.n * sizeof(lsb_elem),
-                               hipMemcpyDeviceToDevice, sl.s));
-  }
+  lsb_config c;
+  lsb_config_init(&c);
+  c.key_bits = key_class & 0xFFu;
+  c.flags = key_class >> 8;
+  c.stream = sl.s;
+  if (cfg.use_graphs) c.flags |= LSB_GRAPH;
+  int rc = lsb_sort_batch(as.data(), bs.data(), ns.data(), (uint32_t)ids.size(), &c);
   Launch L;
   L.stream = si;
   L.batch = true;
@@ -281,7 +278,7 @@ bool lsb_server::admit() {
 
 // An open batch is launched when a stream is idle (nothing to wait for),
 // when the batch window has passed, or when nothing else is pending.  A
-// batch of one job runs as a plain in-place sort (no staging copies).
+// batch of one job runs as a plain sort.
 bool lsb_server::flush_batches() {
   bool any = false;
   bool idle_stream = false;
@@ -339,9 +336,9 @@ void lsb_server_config_init(lsb_server_config* c) {
   memset(c, 0, sizeof *c);
   c->memory_budget = 200000000000ull;
   c->max_pending = 65536;
-  c->batch_max_elems = 1u << 20;
-  c->batch_max_total = 1ull << 26;
-  c->streams = 4;
+  c->batch_max_elems = 1u << 24;
+  c->batch_max_total = 1ull << 28;
+  c->streams = 8;
   c->use_graphs = 1;
   c->batch_window_us = 200;
 }
@@ -380,14 +377,12 @@ int lsb_server_create(const lsb_server_config* cfg, lsb_server** out) {
   for (auto& sl : s->slots) {
     if (rc) break;
     rc = hip_rc(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
-    if (!rc) rc = hip_rc(hipMalloc(&sl.slab_a, c.batch_max_total * sizeof(lsb_elem)));
     if (!rc) rc = hip_rc(hipMalloc(&sl.slab_b, c.batch_max_total * sizeof(lsb_elem)));
     if (!rc) rc = hip_rc(hipEventCreateWithFlags(&sl.slab_free, hipEventDisableTiming));
     if (!rc) rc = hip_rc(hipEventRecord(sl.slab_free, sl.s));
   }
   if (rc) {
     for (auto& sl : s->slots) {
-      if (sl.slab_a) (void)hipFree(sl.slab_a);
       if (sl.slab_b) (void)hipFree(sl.slab_b);
       if (sl.slab_free) (void)hipEventDestroy(sl.slab_free);
       if (sl.s) (void)hipStreamDestroy(sl.s);
@@ -524,7 +519,6 @@ int lsb_server_destroy(lsb_server* s) {
   s->worker.join();
   for (auto& sl : s->slots) {
     (void)hipStreamSynchronize(sl.s);
-    (void)hipFree(sl.slab_a);
     (void)hipFree(sl.slab_b);
     (void)hipEventDestroy(sl.slab_free);
     (void)hipStreamDestroy(sl.s);

@@ -106,6 +106,7 @@ SIGNATURES = {
     "lsb_last_hip_error": (_I, []),
     "lsb_sort_segmented": (_I, [_P, _P, ctypes.POINTER(_U64), _U32, ctypes.POINTER(Config)]),
     "lsb_segmented_workspace_size": (_SZ, [_SZ, _U32, ctypes.POINTER(Config)]),
+    "lsb_sort_batch": (_I, [_P, _P, _P, _U32, ctypes.POINTER(Config)]),
     "lsb_server_config_init": (None, [ctypes.POINTER(ServerConfig)]),
     "lsb_server_create": (_I, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
     "lsb_server_destroy": (_I, [_P]),
@@ -273,6 +274,23 @@ def sort_segmented(A, B, offsets, key_bits=64, digit_bits=0, tile=0, stream=None
     _check(lib().lsb_sort_segmented(_ptr(A), _ptr(B),
                                     off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                                     max(nseg, 0), ctypes.byref(c)), "lsb_sort_segmented")
+
+
+def sort_batch(As, Bs, ns=None, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,
+               no_skip=False, graph=False, key_order="unsigned"):
+    """Sorts each buffer As[i] (its first ns[i] elements) on its own, stably
+    by key, with Bs[i] as its scratch (None allowed for at most 8192
+    elements), in one batched launch sequence without copying anything."""
+    nseg = len(As)
+    if ns is None:
+        ns = [_nelem(a) for a in As]
+    Ap = (ctypes.c_void_p * max(nseg, 1))(*[_ptr(a) for a in As])
+    Bp = (ctypes.c_void_p * max(nseg, 1))(*[(_ptr(b) if b is not None else None) for b in Bs])
+    Np = (ctypes.c_uint64 * max(nseg, 1))(*ns)
+    flags = ((LSB_SYNC if sync else 0) | (LSB_NO_SKIP if no_skip else 0)
+             | (LSB_GRAPH if graph else 0) | _key_flags(key_order))
+    c = make_config(key_bits, digit_bits, tile, stream, flags)
+    _check(lib().lsb_sort_batch(Ap, Bp, Np, nseg, ctypes.byref(c)), "lsb_sort_batch")
 
 
 def segmented_workspace_size(total_n, nseg, key_bits=64, digit_bits=0, tile=0):

@@ -158,6 +158,19 @@ int lsb_sort_segmented(lsb_elem* A, lsb_elem* B, const uint64_t* seg_offsets, ui
 /* Device workspace bytes for a caller-provided segmented workspace. */
 size_t lsb_segmented_workspace_size(size_t total_n, uint32_t nseg, const lsb_config* cfg);
 
+/* The same batched sort over segments that live in separate buffers:
+ * segment i is As[i][0 .. ns[i]), sorted in place, with Bs[i] (at least
+ * ns[i] elements, not overlapping As[i]) as its scratch.  Bs[i] may be NULL
+ * for segments of at most 8192 elements (sorted inside one workgroup), and
+ * Bs itself when every segment is that small.  The pointers are device
+ * addresses the caller guarantees (only null, alignment and A/B overlap are
+ * checked).  Nothing is copied: callers that hold many separate arrays (the
+ * serving loop) batch them without staging copies.  The pointer lists are
+ * read before the call returns.  lsb_segmented_workspace_size(sum of ns,
+ * nseg, cfg) sizes a caller workspace. */
+int lsb_sort_batch(lsb_elem* const* As, lsb_elem* const* Bs, const uint64_t* ns, uint32_t nseg,
+                   const lsb_config* cfg);
+
 /* ---------------------------------------------------------------- serving */
 
 /* A sort service over jobs resident in HBM.  Jobs are admitted in arrival
@@ -172,9 +185,10 @@ typedef struct lsb_server lsb_server;
 typedef struct lsb_server_config {
   size_t memory_budget;      /* bytes; 0 = 200e9 */
   uint32_t max_pending;      /* submitted, not yet admitted; 0 = 65536 */
-  uint32_t batch_max_elems;  /* jobs of at most this many elements are coalesced; 0 = 2^20 */
-  uint64_t batch_max_total;  /* elements per coalesced launch (staging slab); 0 = 2^26 */
-  uint32_t streams;          /* device streams; 0 = 4 */
+  uint32_t batch_max_elems;  /* jobs of at most this many elements are coalesced; 0 = 2^24 */
+  uint64_t batch_max_total;  /* elements per coalesced launch; each stream owns a scratch slab
+                                of this many elements (16 B each); 0 = 2^28 */
+  uint32_t streams;          /* device streams; 0 = 8 */
   uint32_t use_graphs;       /* 1 = coalesced launches replay captured hipGraphs */
   uint32_t batch_window_us;  /* longest wait before an open batch is launched; 0 = 200 */
   uint32_t reserved;

@@ -130,6 +130,30 @@ def test_segmented_argument_errors_need_no_gpu():
     assert _seg_rc(a, b, [0, 10], flags=lsb_amd.LSB_PROFILE) == lsb_amd.LSB_EINVAL
 
 
+def _batch_rc(As, Bs, ns, **kw):
+    k = len(ns)
+    Ap = (ctypes.c_void_p * max(k, 1))(*As)
+    Bp = (ctypes.c_void_p * max(k, 1))(*Bs) if Bs is not None else None
+    Np = (ctypes.c_uint64 * max(k, 1))(*ns)
+    c = lsb_amd.make_config(stream=0, **kw)
+    return lsb_amd.lib().lsb_sort_batch(Ap, Bp, Np, k, ctypes.byref(c))
+
+
+def test_batch_argument_errors_need_no_gpu():
+    a, b, big = 0x10000, 0x80000000, 20000
+    assert _batch_rc([], [], []) == 0                                  # no segments
+    assert _batch_rc([None, None], [None, None], [0, 1]) == 0          # nothing to sort
+    assert _batch_rc([None], [b], [big]) == lsb_amd.LSB_EINVAL         # null A
+    assert _batch_rc([a + 8], [b], [10]) == lsb_amd.LSB_EINVAL         # misaligned A
+    assert _batch_rc([a], [None], [big]) == lsb_amd.LSB_EINVAL         # large needs scratch
+    assert _batch_rc([a], None, [big]) == lsb_amd.LSB_EINVAL
+    assert _batch_rc([a], [b + 8], [big]) == lsb_amd.LSB_EINVAL        # misaligned B
+    assert _batch_rc([a], [a + 16 * 100], [big]) == lsb_amd.LSB_EINVAL  # A and B overlap
+    assert _batch_rc([a], [b], [lsb_amd.LSB_MAX_N + 1]) == lsb_amd.LSB_ERANGE
+    assert _batch_rc([a], [b], [big], key_bits=70) == lsb_amd.LSB_EINVAL
+    assert _batch_rc([a], [b], [big], flags=lsb_amd.LSB_PROFILE) == lsb_amd.LSB_EINVAL
+
+
 def test_segmented_workspace_grows_with_capacity():
     w1 = lsb_amd.segmented_workspace_size(1 << 20, 16)
     w2 = lsb_amd.segmented_workspace_size(1 << 24, 16)
@@ -141,7 +165,7 @@ def test_server_rejects_bad_config_without_gpu():
     L = lsb_amd.lib()
     c = lsb_amd.ServerConfig()
     L.lsb_server_config_init(ctypes.byref(c))
-    assert c.memory_budget == 200_000_000_000 and c.streams == 4
+    assert c.memory_budget == 200_000_000_000 and c.streams == 8
     h = ctypes.c_void_p()
     bad = lsb_amd.ServerConfig()
     L.lsb_server_config_init(ctypes.byref(bad))

@@ -115,6 +115,27 @@ def test_segmented_key_orders(order):
     assert np.array_equal(from_dev(A), want)
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_of_separate_buffers(graph):
+    """lsb_sort_batch: arrays in separate buffers (tiny, small and tile-
+    pipeline sizes, scratch only where needed) each come back exactly as a
+    lone sort of that array, over several batches of different shapes (with
+    graph=True the later ones replay one captured sequence)."""
+    rng = np.random.default_rng(31)
+    for rep in range(3):
+        sizes = [int(x) for x in rng.choice([0, 1, 2, 700, 2048, 2049, 8192, 9000, 70001,
+                                             300000], size=12)]
+        arrs = [O.gen_uniform(n, ranks=2, seed_base=100 * rep + i) if n else
+                np.zeros(0, dtype=O.ELEM) for i, n in enumerate(sizes)]
+        As = [to_dev(a) if len(a) else torch.empty((0, 2), dtype=torch.int64, device=DEV)
+              for a in arrs]
+        Bs = [torch.empty_like(A) if len(a) > 8192 else None for a, A in zip(arrs, As)]
+        lsb_amd.sort_batch(As, Bs, ns=sizes, graph=graph, sync=True)
+        for a, A in zip(arrs, As):
+            if len(a):
+                assert np.array_equal(from_dev(A), a[np.argsort(a["key"], kind="stable")])
+
+
 def test_segment_alone_equals_segment_in_batch():
     """A segment's result does not depend on its neighbours."""
     a, off = batch([20000, 5000, 90000, 3], seed=5)
