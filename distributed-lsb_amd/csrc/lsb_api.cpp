@@ -415,7 +415,7 @@ struct SegLayout {
   lsb::KeyOrder ko;
   size_t cap_n;
   uint32_t cap_segs, max_tiles;
-  size_t hdr, meta, counts, plan, tickets, status, total;
+  size_t hdr, meta, counts, plan, tickets, status, tileseg, total;
 };
 
 int make_seg_layout(size_t cap_n, uint32_t cap_segs, const lsb_config& c, SegLayout* L,
@@ -468,6 +468,7 @@ int make_seg_layout(size_t cap_n, uint32_t cap_segs, const lsb_config& c, SegLay
   L->plan = off;    off += align_up((size_t)L->cap_segs * 8);
   L->tickets = off; off += align_up(16 * 4);
   L->status = off;  off += align_up(2 * (size_t)L->max_tiles * L->nb * 4);
+  L->tileseg = off; off += align_up((size_t)L->max_tiles * 4);
   L->total = off;
   return LSB_OK;
 }
@@ -508,9 +509,10 @@ int enqueue_segmented(const SegLayout& L, char* ws, uint32_t flags, hipStream_t 
   uint32_t* plan = reinterpret_cast<uint32_t*>(ws + L.plan);
   uint32_t* tickets = reinterpret_cast<uint32_t*>(ws + L.tickets);
   uint32_t* status = reinterpret_cast<uint32_t*>(ws + L.status);
+  uint32_t* tileseg = reinterpret_cast<uint32_t*>(ws + L.tileseg);
   const uint32_t cus = device_cus();
   LSB_HIP(lsb::launch_seg_prepare(L.d, hdr, large, L.tile, L.key_bits, L.ko, L.passes, counts,
-                                  plan, tickets, status, flags, 4 * cus, s));
+                                  plan, tickets, status, tileseg, flags, 4 * cus, s));
   LSB_HIP(lsb::launch_seg_small(hdr, small, L.key_bits, L.ko, 4 * cus, s));
   const uint32_t grid = L.v->blocks_per_cu() * cus;
   for (uint32_t q = 0; q < L.passes; ++q) {
@@ -518,6 +520,7 @@ int enqueue_segmented(const SegLayout& L, char* ws, uint32_t flags, hipStream_t 
     a.hdr = hdr;
     a.large = large;
     a.seg_plan = plan;
+    a.tile_seg = tileseg;
     a.counts = counts;
     a.tickets = tickets;
     a.status = status;
@@ -529,7 +532,7 @@ int enqueue_segmented(const SegLayout& L, char* ws, uint32_t flags, hipStream_t 
     a.ko = L.ko;
     LSB_HIP(L.v->scatter(a, grid, s));
   }
-  LSB_HIP(lsb::launch_seg_copy(hdr, large, plan, L.tile, 4 * cus, s));
+  LSB_HIP(lsb::launch_seg_copy(hdr, large, plan, tileseg, L.tile, 4 * cus, s));
   return LSB_OK;
 }
 

@@ -110,6 +110,7 @@ struct SegPassArgs {
   const SegHeader* hdr;
   const SegLarge* large;
   const uint32_t* seg_plan;  // [2*s] executed-pass mask, [2*s+1] result-in-B flag
+  const uint32_t* tile_seg;  // large segment of every global tile
   const uint32_t* counts;    // per segment [MAXP][2^d] starts (after the scan)
   uint32_t* tickets;
   uint32_t* status;          // two slices of max_tiles * 2^d words
@@ -135,10 +136,11 @@ hipError_t launch_seg_small(const SegHeader* hdr, const SegSmall* list,
 hipError_t launch_seg_prepare(uint32_t d, const SegHeader* hdr,
                               const SegLarge* large, uint32_t tile, uint32_t key_bits,
                               const KeyOrder& ko, uint32_t passes, uint32_t* counts, uint32_t* seg_plan,
-                              uint32_t* tickets, uint32_t* status0, uint32_t flags,
-                              uint32_t grid, hipStream_t s);
+                              uint32_t* tickets, uint32_t* status0, uint32_t* tile_seg,
+                              uint32_t flags, uint32_t grid, hipStream_t s);
 hipError_t launch_seg_copy(const SegHeader* hdr, const SegLarge* large,
-                           const uint32_t* seg_plan, uint32_t tile, uint32_t grid,
+                           const uint32_t* seg_plan, const uint32_t* tile_seg, uint32_t tile,
+                           uint32_t grid,
                            hipStream_t s);
 
 // Workload generation and verification (lsb_workload.hip).

@@ -168,113 +168,6 @@ __device__ __forceinline__ void tile_load(elem_t (&e)[ITEMS], const elem_t* __re
   }
 }
 
-// --------------------------------------------------------- onesweep_tile
-// One tile of one pass over an array (or one segment) of n elements whose
-// elements are already in registers (tile_load): dst points at element 0 of
-// that array; `tile` counts from its first tile; st_cur/st_next are the
-// array's rows of the two look-back slices; starts[b] is the global
-// exclusive start of bucket b for this digit.  Must be entered by the whole
-// block after a barrier that ends every earlier use of sm.
-template <int D, int BLOCK, int ITEMS>
-__device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm,
-                                                   const elem_t (&e)[ITEMS],
-                                                   elem_t* __restrict__ dst, uint32_t n,
-                                                   uint32_t tile, uint32_t* st_cur,
-                                                   uint32_t* st_next,
-                                                   const uint32_t* __restrict__ starts,
-                                                   uint32_t shift, uint32_t mask,
-                                                   const KeyOrder& ko) {
-  using S = TileSmem<D, BLOCK, ITEMS>;
-  constexpr int NB = S::NB, W = S::W, T = S::T;
-  constexpr int DPT = (NB + BLOCK - 1) / BLOCK;  // digits per thread
-  const uint32_t t = threadIdx.x;
-  const uint32_t lane = t & (kWave - 1);
-  const uint32_t wave = t / kWave;
-  uint32_t* whist = reinterpret_cast<uint32_t*>(sm.elems);
-
-  for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
-  for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) st_next[(size_t)tile * NB + b] = 0u;
-
-  const uint32_t tile_base = tile * (uint32_t)T;
-  const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
-  const uint32_t wrun = wave * (kWave * ITEMS);
-  const uint32_t lim = tile_n > wrun ? tile_n - wrun : 0u;
-  lds_barrier();
-
-  uint32_t dg[ITEMS], rk[ITEMS];
-  uint32_t* wh = whist + wave * NB;
-  wave_rank<D, ITEMS>(e, lim, shift, mask, wh, dg, rk);
-  lds_barrier();
-
-  // Per digit: exclusive prefix over waves (in place) and the tile total,
-  // published at once as this tile's aggregate.  The first look-back round
-  // is issued right away and consumed only after the tile-local scan and
-  // the reorder through LDS, so its memory latency hides under that work.
-  uint32_t tot[DPT], gstart[DPT];
-  uint32_t lbw[DPT][kLookback];
-#pragma unroll
-  for (int k = 0; k < DPT; ++k) {
-    const uint32_t b = t + k * BLOCK;
-    tot[k] = 0;
-    gstart[k] = 0;
-    if (b < (uint32_t)NB) {
-      gstart[k] = starts[b];
-      uint32_t s = 0;
-#pragma unroll
-      for (int w = 0; w < W; ++w) {
-        const uint32_t c = whist[w * NB + b];
-        whist[w * NB + b] = s;
-        s += c;
-      }
-      tot[k] = s;
-      sm.start[b] = s;
-      uint32_t* mine = st_cur + (size_t)tile * NB + b;
-      if (tile == 0) {
-        st_status(mine, (s + 1u) | kFlagInclusive);
-      } else {
-        st_status(mine, s + 1u);
-        lookback_issue(st_cur, tile, NB, b, lbw[k]);
-      }
-    }
-  }
-  block_exclusive_scan<NB, BLOCK>(sm.start, sm.wsum);
-
-  uint32_t pos[ITEMS];
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) pos[i] = sm.start[dg[i]] + wh[dg[i]] + rk[i];
-  lds_barrier();  // counters are dead from here: the buffer takes elements
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i)
-    if ((uint32_t)i * kWave + lane < lim) sm.elems[pos[i]] = e[i];
-
-#pragma unroll
-  for (int k = 0; k < DPT; ++k) {
-    const uint32_t b = t + k * BLOCK;
-    if (b < (uint32_t)NB) {
-      uint32_t prefix = 0;
-      if (tile > 0) {
-        prefix = lookback_finish(st_cur, tile, NB, b, lbw[k]);
-        st_status(st_cur + (size_t)tile * NB + b, (prefix + tot[k] + 1u) | kFlagInclusive);
-      }
-      sm.grs[b].x = gstart[k] + prefix - sm.start[b];
-    }
-  }
-  lds_barrier();
-
-  // Write out in digit order: consecutive lanes write consecutive slots of
-  // one bucket run.
-#pragma unroll
-  for (int k = 0; k < ITEMS; ++k) {
-    const uint32_t j = k * BLOCK + t;
-    if (j < tile_n) {
-      elem_t x = sm.elems[j];
-      const uint32_t d = digit_of(x.key, shift, mask);
-      x.key = key_inv(x.key, ko);
-      dst[(size_t)(uint32_t)(sm.grs[d].x + j)] = x;
-    }
-  }
-}
-
 // ------------------------------------------------------------ store_run
 // Writes a tile held in LDS in digit order (slot j of bucket d goes to
 // dst[grs[d].x + j]).  With COUNT, every element is also counted in hnext
@@ -329,6 +222,115 @@ __device__ __forceinline__ void store_run(const elem_t* elems, const uint2* grs,
       dst[(size_t)(g.x + j)] = x;
     }
   }
+}
+
+// --------------------------------------------------------- onesweep_tile
+// One tile of one pass over an array (or one segment) of n elements whose
+// elements are already in registers (tile_load): dst points at element 0 of
+// that array; `tile` counts from its first tile; st_cur/st_next are the
+// array's rows of the two look-back slices; starts[b] is the global
+// exclusive start of bucket b for this digit.  Must be entered by the whole
+// block after a barrier that ends every earlier use of sm.
+template <int D, int BLOCK, int ITEMS>
+__device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm,
+                                                   const elem_t (&e)[ITEMS],
+                                                   elem_t* __restrict__ dst, uint32_t n,
+                                                   uint32_t tile, uint32_t* st_cur,
+                                                   uint32_t* st_next,
+                                                   const uint32_t* __restrict__ starts,
+                                                   uint32_t shift, uint32_t mask,
+                                                   const KeyOrder& ko,
+                                                   uint32_t* next_ticket = nullptr) {
+  using S = TileSmem<D, BLOCK, ITEMS>;
+  constexpr int NB = S::NB, W = S::W, T = S::T;
+  constexpr int DPT = (NB + BLOCK - 1) / BLOCK;  // digits per thread
+  const uint32_t t = threadIdx.x;
+  const uint32_t lane = t & (kWave - 1);
+  const uint32_t wave = t / kWave;
+  uint32_t* whist = reinterpret_cast<uint32_t*>(sm.elems);
+
+  for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
+  for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) st_next[(size_t)tile * NB + b] = 0u;
+
+  const uint32_t tile_base = tile * (uint32_t)T;
+  const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
+  const uint32_t wrun = wave * (kWave * ITEMS);
+  const uint32_t lim = tile_n > wrun ? tile_n - wrun : 0u;
+  lds_barrier();
+
+  uint32_t dg[ITEMS], rk[ITEMS];
+  uint32_t* wh = whist + wave * NB;
+  wave_rank<D, ITEMS>(e, lim, shift, mask, wh, dg, rk);
+  lds_barrier();
+
+  // Per digit: exclusive prefix over waves (in place) and the tile total,
+  // published at once as this tile's aggregate.  The first look-back round
+  // is issued right away and consumed only after the tile-local scan and
+  // the reorder through LDS, so its memory latency hides under that work.
+  uint32_t tot[DPT], gstart[DPT];
+  uint32_t lbw[DPT][kLookback];
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const uint32_t b = t + k * BLOCK;
+    tot[k] = 0;
+    gstart[k] = 0;
+    if (b < (uint32_t)NB) {
+      gstart[k] = starts[b];
+      uint32_t s = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const uint32_t c = whist[w * NB + b];
+        whist[w * NB + b] = s;
+        s += c;
+      }
+      tot[k] = s;
+      sm.start[b] = s;
+      uint32_t* mine = st_cur + (size_t)tile * NB + b;
+      if (tile == 0) {
+        st_status(mine, (s + 1u) | kFlagInclusive);
+      } else {
+        st_status(mine, s + 1u);
+        lookback_issue(st_cur, tile, NB, b, lbw[k]);
+      }
+    }
+  }
+  // The scan also adds each bucket's tile start into every wave's row, so
+  // an element's slot is one LDS read: wh[digit] + rank.
+  block_exclusive_scan<NB, BLOCK, W>(sm.start, sm.wsum, whist);
+
+  uint32_t pos[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) pos[i] = wh[dg[i]] + rk[i];
+  lds_barrier();  // counters are dead from here: the buffer takes elements
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if ((uint32_t)i * kWave + lane < lim) sm.elems[pos[i]] = e[i];
+
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const uint32_t b = t + k * BLOCK;
+    if (b < (uint32_t)NB) {
+      uint32_t prefix = 0;
+      if (tile > 0) {
+        prefix = lookback_finish(st_cur, tile, NB, b, lbw[k]);
+        st_status(st_cur + (size_t)tile * NB + b, (prefix + tot[k] + 1u) | kFlagInclusive);
+      }
+      sm.grs[b].x = gstart[k] + prefix - sm.start[b];
+    }
+  }
+  lds_barrier();
+
+  // A persistent caller's next ticket is requested before the store phase
+  // so its latency hides under the stores; thread 0 leaves it in
+  // sm.scalar[3].
+  uint32_t nt;
+  if (next_ticket && t == 0) nt = atomicAdd(next_ticket, 1u);
+
+  // Write out in digit order: consecutive lanes write consecutive slots of
+  // one bucket run.
+  store_run<BLOCK, ITEMS, NB, false>(sm.elems, sm.grs, nullptr, tile_n, dst, shift, mask, 0u, 0u,
+                                     ko);
+  if (next_ticket && t == 0) sm.scalar[3] = nt;
 }
 
 // ------------------------------------------------------- region form
@@ -456,12 +458,13 @@ __device__ __forceinline__ void onesweep_tile(TileSmem<D, BLOCK, ITEMS>& sm,
                                               uint32_t* st_next,
                                               const uint32_t* __restrict__ starts,
                                               uint32_t shift, uint32_t mask,
-                                              const KeyOrder& ko) {
+                                              const KeyOrder& ko,
+                                              uint32_t* next_ticket = nullptr) {
   elem_t e[ITEMS];
   tile_load<BLOCK, ITEMS>(e, src, n, tile);
   keys_fwd<ITEMS>(e, ko);
   onesweep_tile_regs<D, BLOCK, ITEMS>(sm, e, dst, n, tile, st_cur, st_next, starts, shift, mask,
-                                      ko);
+                                      ko, next_ticket);
 }
 
 // ----------------------------------------------------------- small_sort

@@ -54,6 +54,19 @@ __global__ __launch_bounds__(BLOCK) void k_seg_small(const SegHeader* __restrict
   }
 }
 
+// ------------------------------------------------------------ tile map
+// tile_seg[t] = the large segment that global tile t belongs to.
+__global__ __launch_bounds__(256) void k_seg_tilemap(const SegHeader* __restrict__ hdr,
+                                                     const SegLarge* __restrict__ large,
+                                                     uint32_t tile, uint32_t* __restrict__ tile_seg) {
+  const uint32_t nl = hdr->n_large;
+  for (uint32_t s = blockIdx.x; s < nl; s += gridDim.x) {
+    const SegLarge L = large[s];
+    const uint32_t nt = (L.n + tile - 1) / tile;
+    for (uint32_t i = threadIdx.x; i < nt; i += 256) tile_seg[L.tile_begin + i] = s;
+  }
+}
+
 // ------------------------------------------------------------ counters
 __global__ __launch_bounds__(256) void k_seg_zero(const SegHeader* __restrict__ hdr,
                                                   uint32_t* __restrict__ counts,
@@ -180,29 +193,20 @@ template <int D, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(SegPassArgs args) {
   constexpr int NB = 1 << D;
   constexpr int MAXP = (64 + D - 1) / D;
-  constexpr uint32_t T = BLOCK * ITEMS;
   __shared__ TileSmem<D, BLOCK, ITEMS> sm;
   const uint32_t total = args.hdr->total_tiles;
-  const uint32_t nlarge = args.hdr->n_large;
   const uint32_t pass = args.pass;
   const size_t slice = (size_t)args.max_tiles * NB;
+  uint32_t* ticket = &args.tickets[pass];
+  if (threadIdx.x == 0) sm.scalar[3] = atomicAdd(ticket, 1u);
+  __syncthreads();
   while (true) {
-    if (threadIdx.x == 0) {
-      const uint32_t tile = atomicAdd(&args.tickets[pass], 1u);
-      uint32_t seg = 0, mask = 0;
-      if (tile < total) {
-        seg = find_segment(args.large, nlarge, tile);
-        mask = args.seg_plan[2 * seg];
-      }
-      sm.scalar[0] = tile;
-      sm.scalar[1] = seg;
-      sm.scalar[2] = mask;
-    }
-    __syncthreads();
-    const uint32_t tile = sm.scalar[0];
+    // The next tile comes from the previous tile's early ticket; its segment
+    // from the tile map (uniform loads).
+    const uint32_t tile = sm.scalar[3];
     if (tile >= total) return;
-    const uint32_t seg = sm.scalar[1];
-    const uint32_t mask = sm.scalar[2];
+    const uint32_t seg = args.tile_seg[tile];
+    const uint32_t mask = args.seg_plan[2 * seg];
     if (mask & (1u << pass)) {
       const SegLarge L = args.large[seg];
       const uint32_t ord = (uint32_t)__popc(mask & ((1u << pass) - 1u));
@@ -215,8 +219,9 @@ __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(SegPassArgs args) {
           args.status + (size_t)((ord + 1u) & 1u) * slice + (size_t)L.tile_begin * NB;
       const uint32_t* starts = args.counts + (size_t)seg * MAXP * NB + (size_t)pass * NB;
       onesweep_tile<D, BLOCK, ITEMS>(sm, src, dst, L.n, tile - L.tile_begin, st_cur, st_next,
-                                     starts, args.shift, args.mask, args.ko);
-      (void)T;
+                                     starts, args.shift, args.mask, args.ko, ticket);
+    } else if (threadIdx.x == 0) {
+      sm.scalar[3] = atomicAdd(ticket, 1u);  // this segment skips the pass
     }
     __syncthreads();
   }
@@ -226,11 +231,11 @@ __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(SegPassArgs args) {
 __global__ __launch_bounds__(256) void k_seg_copy(const SegHeader* __restrict__ hdr,
                                                   const SegLarge* __restrict__ large,
                                                   const uint32_t* __restrict__ seg_plan,
+                                                  const uint32_t* __restrict__ tile_seg,
                                                   uint32_t tile) {
   const uint32_t total = hdr->total_tiles;
-  const uint32_t nlarge = hdr->n_large;
   for (uint32_t tl = blockIdx.x; tl < total; tl += gridDim.x) {
-    const uint32_t seg = find_segment(large, nlarge, tl);
+    const uint32_t seg = tile_seg[tl];
     if (!seg_plan[2 * seg + 1]) continue;
     const SegLarge L = large[seg];
     const uint32_t first = (tl - L.tile_begin) * tile;
@@ -285,8 +290,9 @@ hipError_t launch_seg_small(const SegHeader* hdr, const SegSmall* list,
 hipError_t launch_seg_prepare(uint32_t d, const SegHeader* hdr,
                               const SegLarge* large, uint32_t tile, uint32_t key_bits,
                               const KeyOrder& ko, uint32_t passes, uint32_t* counts, uint32_t* seg_plan,
-                              uint32_t* tickets, uint32_t* status0, uint32_t flags,
-                              uint32_t grid, hipStream_t s) {
+                              uint32_t* tickets, uint32_t* status0, uint32_t* tile_seg,
+                              uint32_t flags, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_tilemap, dim3(grid), dim3(256), 0, s, hdr, large, tile, tile_seg);
   if (d == 8) {
     constexpr int MAXP = 8;
     hipLaunchKernelGGL(k_seg_zero, dim3(grid), dim3(256), 0, s, hdr, counts, MAXP * 256u);
@@ -308,9 +314,10 @@ hipError_t launch_seg_prepare(uint32_t d, const SegHeader* hdr,
 }
 
 hipError_t launch_seg_copy(const SegHeader* hdr, const SegLarge* large,
-                           const uint32_t* seg_plan, uint32_t tile, uint32_t grid,
-                           hipStream_t s) {
-  hipLaunchKernelGGL(k_seg_copy, dim3(grid), dim3(256), 0, s, hdr, large, seg_plan, tile);
+                           const uint32_t* seg_plan, const uint32_t* tile_seg, uint32_t tile,
+                           uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_copy, dim3(grid), dim3(256), 0, s, hdr, large, seg_plan, tile_seg,
+                     tile);
   return hipGetLastError();
 }
 
