@@ -257,6 +257,11 @@ __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(Pass
   __shared__ TileSmem<D, BLOCK, ITEMS> sm;
   __shared__ uint32_t s_rbase[NB];
   __shared__ uint32_t s_hnext[kRegions * NB];
+  // Per-wave counters get their own LDS when a block fills the CU alone
+  // (512+ threads); smaller blocks keep them in the staging buffer so two
+  // blocks still fit.
+  constexpr bool kOwnCounters = BLOCK >= 512;
+  __shared__ uint32_t s_whist[kOwnCounters ? (BLOCK / kWave) * NB : 1];
   const uint32_t* plan = args.plan;
   const uint32_t pass = args.pass;
   if (!plan[4 * pass]) return;
@@ -294,6 +299,8 @@ __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(Pass
     s_rbase[b] = s;
   }
   for (uint32_t i = threadIdx.x; i < (uint32_t)(kRegions * NB); i += BLOCK) s_hnext[i] = 0u;
+  if (kOwnCounters)
+    for (uint32_t i = threadIdx.x; i < (uint32_t)((BLOCK / kWave) * NB); i += BLOCK) s_whist[i] = 0u;
   uint32_t* ticket = &args.tickets[pass * kRegions + region];
   if (threadIdx.x == 0) sm.scalar[0] = atomicAdd(ticket, 1u);
   lds_barrier();
@@ -307,7 +314,8 @@ __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(Pass
     onesweep_tile_region<D, BLOCK, ITEMS>(sm, e, dst, args.n, tile, local == 0, st_cur, st_next,
                                           s_rbase, args.shift, args.mask, ticket,
                                           q >= 0 ? s_hnext : nullptr, nshift, nmask,
-                                          region_elems, args.ko);
+                                          region_elems, args.ko,
+                                          kOwnCounters ? s_whist : nullptr);
     lds_barrier();
   }
   if (q >= 0) {

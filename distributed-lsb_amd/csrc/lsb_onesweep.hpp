@@ -351,22 +351,30 @@ __device__ __forceinline__ void onesweep_tile_region(
     TileSmem<D, BLOCK, ITEMS>& sm, const elem_t (&e)[ITEMS], elem_t* __restrict__ dst,
     uint32_t n, uint32_t tile, bool chain_first, uint32_t* st_cur, uint32_t* st_next,
     const uint32_t* rbase, uint32_t shift, uint32_t mask, uint32_t* ticket, uint32_t* hnext,
-    uint32_t nshift, uint32_t nmask, uint32_t region_elems, const KeyOrder& ko) {
+    uint32_t nshift, uint32_t nmask, uint32_t region_elems, const KeyOrder& ko,
+    uint32_t* whist) {
   using S = TileSmem<D, BLOCK, ITEMS>;
   constexpr int NB = S::NB, W = S::W, T = S::T;
   constexpr int DPT = (NB + BLOCK - 1) / BLOCK;
   const uint32_t t = threadIdx.x;
   const uint32_t lane = t & (kWave - 1);
   const uint32_t wave = t / kWave;
-  uint32_t* whist = reinterpret_cast<uint32_t*>(sm.elems);
-
-  for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
+  // whist: the per-wave counters.  Given their own LDS (W * NB words) they
+  // are zero on entry and cleared again once this tile's slots are known,
+  // so a tile starts ranking without a barrier of its own; passed as null
+  // they share the staging buffer (blocks whose LDS has no room to spare)
+  // and are cleared here, behind a barrier.
+  const bool own_counters = whist != nullptr;
+  if (!own_counters) {
+    whist = reinterpret_cast<uint32_t*>(sm.elems);
+    for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
+  }
   for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) st_next[(size_t)tile * NB + b] = 0u;
   const uint32_t tile_base = tile * (uint32_t)T;
   const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
   const uint32_t wrun = wave * (kWave * ITEMS);
   const uint32_t lim = tile_n > wrun ? tile_n - wrun : 0u;
-  lds_barrier();
+  if (!own_counters) lds_barrier();
 
   uint32_t dg[ITEMS], rk[ITEMS];
   uint32_t* wh = whist + wave * NB;
@@ -408,6 +416,8 @@ __device__ __forceinline__ void onesweep_tile_region(
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) pos[i] = wh[dg[i]] + rk[i];
   lds_barrier();
+  if (own_counters)
+    for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;  // for the next tile
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
     if ((uint32_t)i * kWave + lane < lim) sm.elems[pos[i]] = e[i];
