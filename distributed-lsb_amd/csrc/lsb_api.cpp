@@ -79,44 +79,44 @@ std::vector<lsb_tune_entry> g_tune;
 // made by lsb_autotune or lsb_tune_load takes precedence; lsb_tune_clear
 // returns to this one.
 const lsb_tune_entry kBuiltinTune[] = {
-    {12, 64, 8, 2048, 0, 0, 0.0818f, 0.0865f},
-    {13, 64, 8, 2048, 0, 0, 0.0798f, 0.0869f},
-    {14, 64, 8, 2048, 0, 0, 0.0864f, 0.0971f},
-    {15, 64, 8, 2048, 0, 0, 0.0983f, 0.1107f},
-    {16, 64, 8, 2048, 0, 0, 0.1199f, 0.1256f},
-    {17, 64, 8, 2048, 0, 0, 0.1295f, 0.1345f},
-    {18, 64, 8, 2048, 0, 0, 0.1447f, 0.1458f},
-    {19, 64, 8, 2048, 3, 0, 0.1718f, 0.1755f},
-    {20, 64, 8, 4096, 3, 0, 0.2094f, 0.2098f},
-    {21, 64, 8, 8192, 0, 0, 0.2415f, 0.2483f},
-    {22, 64, 8, 4096, 3, 0, 0.3613f, 0.3703f},
-    {23, 64, 8, 8192, 3, 0, 0.6256f, 0.6406f},
-    {24, 64, 8, 8192, 3, 0, 1.1924f, 1.2601f},
-    {25, 64, 8, 8192, 3, 0, 2.1836f, 2.3507f},
-    {26, 64, 8, 8192, 3, 0, 4.1360f, 4.5398f},
-    {27, 64, 8, 8192, 3, 0, 8.0769f, 8.8606f},
-    {28, 64, 8, 8192, 3, 0, 16.6266f, 17.8641f},
-    {29, 64, 8, 8192, 3, 0, 33.4812f, 35.9129f},
-    {30, 64, 8, 8192, 3, 0, 67.7886f, 71.9106f},
-    {12, 32, 8, 2048, 0, 0, 0.0462f, 0.0490f},
-    {13, 32, 8, 2048, 0, 0, 0.0481f, 0.0499f},
-    {14, 32, 8, 2048, 0, 0, 0.0495f, 0.0538f},
-    {15, 32, 8, 2048, 0, 0, 0.0562f, 0.0618f},
-    {16, 32, 8, 2048, 0, 0, 0.0679f, 0.0712f},
-    {17, 32, 8, 2048, 0, 0, 0.0726f, 0.0744f},
-    {18, 32, 8, 2048, 3, 0, 0.0806f, 0.0813f},
-    {19, 32, 8, 2048, 3, 0, 0.0943f, 0.0959f},
-    {20, 32, 8, 8192, 0, 0, 0.1122f, 0.1129f},
-    {21, 32, 8, 8192, 3, 0, 0.1284f, 0.1305f},
-    {22, 32, 8, 4096, 3, 0, 0.1863f, 0.1907f},
-    {23, 32, 8, 8192, 3, 0, 0.3064f, 0.3104f},
-    {24, 32, 8, 8192, 3, 0, 0.5948f, 0.6286f},
-    {25, 32, 8, 8192, 3, 0, 1.1196f, 1.1948f},
-    {26, 32, 8, 8192, 3, 0, 2.1239f, 2.3035f},
-    {27, 32, 8, 8192, 3, 0, 4.1112f, 4.4716f},
-    {28, 32, 8, 8192, 3, 0, 8.4561f, 8.9874f},
-    {29, 32, 8, 8192, 3, 0, 16.9845f, 18.0351f},
-    {30, 32, 8, 8192, 3, 0, 34.0582f, 35.8162f},
+    {12, 64, 8, 2048, 0, 0, 0.0680f, 0.0732f},
+    {13, 64, 8, 2048, 0, 0, 0.0746f, 0.0802f},
+    {14, 64, 8, 2048, 0, 0, 0.0764f, 0.0873f},
+    {15, 64, 8, 2048, 0, 0, 0.0844f, 0.0961f},
+    {16, 64, 8, 2048, 0, 0, 0.0956f, 0.1034f},
+    {17, 64, 8, 2048, 0, 0, 0.1168f, 0.1177f},
+    {18, 64, 8, 2048, 0, 0, 0.1323f, 0.1325f},
+    {19, 64, 8, 4096, 0, 0, 0.1600f, 0.1647f},
+    {20, 64, 8, 8192, 3, 0, 0.1894f, 0.1918f},
+    {21, 64, 8, 8192, 3, 0, 0.2087f, 0.2261f},
+    {22, 64, 8, 8192, 3, 0, 0.3191f, 0.3413f},
+    {23, 64, 8, 8192, 3, 0, 0.5221f, 0.5950f},
+    {24, 64, 8, 8192, 3, 0, 1.0284f, 1.2199f},
+    {25, 64, 8, 8192, 3, 0, 1.8990f, 2.2532f},
+    {26, 64, 8, 8192, 3, 0, 3.6444f, 4.3267f},
+    {27, 64, 8, 8192, 3, 0, 7.1660f, 8.4423f},
+    {28, 64, 8, 8192, 3, 0, 14.3547f, 17.4303f},
+    {29, 64, 8, 8192, 3, 0, 29.5093f, 34.8872f},
+    {30, 64, 8, 8192, 3, 0, 60.7763f, 69.5869f},
+    {12, 32, 8, 2048, 0, 0, 0.0424f, 0.0427f},
+    {13, 32, 8, 2048, 0, 0, 0.0421f, 0.0458f},
+    {14, 32, 8, 2048, 0, 0, 0.0448f, 0.0496f},
+    {15, 32, 8, 2048, 0, 0, 0.0492f, 0.0562f},
+    {16, 32, 8, 2048, 0, 0, 0.0548f, 0.0574f},
+    {17, 32, 8, 2048, 0, 0, 0.0660f, 0.0667f},
+    {18, 32, 8, 2048, 3, 0, 0.0734f, 0.0744f},
+    {19, 32, 8, 2048, 3, 0, 0.0888f, 0.0893f},
+    {20, 32, 8, 8192, 3, 0, 0.1017f, 0.1043f},
+    {21, 32, 8, 8192, 3, 0, 0.1117f, 0.1200f},
+    {22, 32, 8, 8192, 3, 0, 0.1655f, 0.1769f},
+    {23, 32, 8, 8192, 3, 0, 0.2732f, 0.3023f},
+    {24, 32, 8, 8192, 3, 0, 0.5281f, 0.6223f},
+    {25, 32, 8, 8192, 3, 0, 0.9889f, 1.1742f},
+    {26, 32, 8, 8192, 3, 0, 1.9002f, 2.2507f},
+    {27, 32, 8, 8192, 3, 0, 3.6999f, 4.3865f},
+    {28, 32, 8, 8192, 3, 0, 7.4515f, 8.9221f},
+    {29, 32, 8, 8192, 3, 0, 15.1107f, 17.7994f},
+    {30, 32, 8, 8192, 3, 0, 30.6882f, 35.5001f},
 };
 
 uint32_t variant_id(uint32_t d, uint32_t tile, uint32_t persistent) {

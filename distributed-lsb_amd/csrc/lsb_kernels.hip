@@ -356,7 +356,7 @@ static void launch_onesweep(const PassArgs& a, hipStream_t s) {
 // region form, 0 the one-tile-per-block form.  The first one per digit
 // width is the fallback for that width when no tuned choice applies.
 static const SortVariant kVariants[] = {
-    {8, 512 * 16, 512, 3, &launch_onesweep_rg<8, 512, 16>},
+    {8, 1024 * 8, 1024, 3, &launch_onesweep_rg<8, 1024, 8>},
     {8, 256 * 16, 256, 3, &launch_onesweep_rg<8, 256, 16>},
     {8, 256 * 8, 256, 3, &launch_onesweep_rg<8, 256, 8>},
     {8, 256 * 16, 256, 0, &launch_onesweep<8, 256, 16>},

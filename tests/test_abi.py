@@ -188,7 +188,8 @@ def test_tune_table_drives_auto_choice(tmp_path):
     assert lsb_amd.tune_table() == []
     default = lsb_amd.plan(1 << 20)
     assert lsb_amd.plan(1 << 16)[:2] == (8, 2048)
-    assert lsb_amd.plan(1 << 22)[:2] == (8, 4096)
+    assert lsb_amd.plan(1 << 19)[:2] == (8, 4096)
+    assert lsb_amd.plan(1 << 22)[:2] == (8, 8192)
     assert lsb_amd.plan(1 << 30)[:2] == (8, 8192)
     assert lsb_amd.plan(1 << 30, key_bits=32)[:2] == (8, 8192)
     p = tmp_path / "tune.txt"
