@@ -304,18 +304,34 @@ __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(Pass
   uint32_t* ticket = &args.tickets[pass * kRegions + region];
   if (threadIdx.x == 0) sm.scalar[0] = atomicAdd(ticket, 1u);
   lds_barrier();
+  // With its own counters a block leaves the previous tile in LDS until the
+  // next tile's rank is done, so it counts that tile for the next pass while
+  // the next tile's loads are in flight instead of in the store phase.
+  constexpr bool kCountLater = kOwnCounters;
+  const bool count_later = kCountLater && q >= 0;
+  uint32_t prev_n = 0;  // elements of the tile this block finished last
   while (true) {
     const uint32_t local = sm.scalar[0];
     if (local >= t_count) break;
     const uint32_t tile = t_begin + local;
     elem_t e[ITEMS];
     tile_load<BLOCK, ITEMS>(e, src, args.n, tile);
+    if (count_later && prev_n)
+      count_run<BLOCK, ITEMS, NB>(sm.elems, sm.grs, s_hnext, prev_n, args.shift, args.mask,
+                                  nshift, nmask);
     keys_fwd<ITEMS>(e, args.ko);
     onesweep_tile_region<D, BLOCK, ITEMS>(sm, e, dst, args.n, tile, local == 0, st_cur, st_next,
                                           s_rbase, args.shift, args.mask, ticket,
                                           q >= 0 ? s_hnext : nullptr, nshift, nmask,
                                           region_elems, args.ko,
-                                          kOwnCounters ? s_whist : nullptr);
+                                          kOwnCounters ? s_whist : nullptr, count_later);
+    const uint32_t tb = tile * (uint32_t)(BLOCK * ITEMS);
+    prev_n = args.n - tb < (uint32_t)(BLOCK * ITEMS) ? args.n - tb : (uint32_t)(BLOCK * ITEMS);
+    lds_barrier();
+  }
+  if (count_later && prev_n) {
+    count_run<BLOCK, ITEMS, NB>(sm.elems, sm.grs, s_hnext, prev_n, args.shift, args.mask, nshift,
+                                nmask);
     lds_barrier();
   }
   if (q >= 0) {

@@ -224,6 +224,43 @@ __device__ __forceinline__ void store_run(const elem_t* elems, const uint2* grs,
   }
 }
 
+// ------------------------------------------------------------ count_run
+// The counting half of store_run<..., true> on its own: counts the tile
+// still held in LDS (elements in digit order, grs as store_run reads it) into
+// hnext by destination region and next digit.  Run while the block waits
+// for its next tile's loads, it takes that work off the store phase.
+template <int BLOCK, int ITEMS, int NB>
+__device__ __forceinline__ void count_run(const elem_t* elems, const uint2* grs, uint32_t* hnext,
+                                          uint32_t tile_n, uint32_t shift, uint32_t mask,
+                                          uint32_t nshift, uint32_t nmask) {
+  const uint32_t t = threadIdx.x;
+  if (tile_n == (uint32_t)(BLOCK * ITEMS)) {
+    uint64_t key[ITEMS];
+    uint2 g[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) key[k] = elems[k * BLOCK + t].key;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) g[k] = grs[digit_of(key[k], shift, mask)];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const uint32_t j = k * BLOCK + t;
+      const uint32_t r = (g[k].y & 7u) + (j >= (g[k].y >> 3) ? 1u : 0u);
+      atomicAdd(&hnext[r * NB + digit_of(key[k], nshift, nmask)], 1u);
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t j = k * BLOCK + t;
+    if (j < tile_n) {
+      const uint64_t key = elems[j].key;
+      const uint2 g = grs[digit_of(key, shift, mask)];
+      const uint32_t r = (g.y & 7u) + (j >= (g.y >> 3) ? 1u : 0u);
+      atomicAdd(&hnext[r * NB + digit_of(key, nshift, nmask)], 1u);
+    }
+  }
+}
+
 // --------------------------------------------------------- onesweep_tile
 // One tile of one pass over an array (or one segment) of n elements whose
 // elements are already in registers (tile_load): dst points at element 0 of
@@ -343,16 +380,19 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
 //    latency hides under the stores (thread 0 leaves it in sm.scalar[0]).
 //    It is not requested earlier: a ticket held by a block that cannot yet
 //    start its tile delays every later tile's look-back in that region;
-//  * when hnext is given, every stored element is counted by the region its
+//  * when hnext is given, every element is counted by the region its
 //    destination falls in and by its digit for the next executed pass
-//    (nshift, nmask) -- the per-region counts that pass needs.
+//    (nshift, nmask) -- the per-region counts that pass needs.  With
+//    count_later the store phase leaves that to the caller, who runs
+//    count_run over the tile left in LDS before the block's next tile
+//    overwrites it.
 template <int D, int BLOCK, int ITEMS>
 __device__ __forceinline__ void onesweep_tile_region(
     TileSmem<D, BLOCK, ITEMS>& sm, const elem_t (&e)[ITEMS], elem_t* __restrict__ dst,
     uint32_t n, uint32_t tile, bool chain_first, uint32_t* st_cur, uint32_t* st_next,
     const uint32_t* rbase, uint32_t shift, uint32_t mask, uint32_t* ticket, uint32_t* hnext,
     uint32_t nshift, uint32_t nmask, uint32_t region_elems, const KeyOrder& ko,
-    uint32_t* whist) {
+    uint32_t* whist, bool count_later) {
   using S = TileSmem<D, BLOCK, ITEMS>;
   constexpr int NB = S::NB, W = S::W, T = S::T;
   constexpr int DPT = (NB + BLOCK - 1) / BLOCK;
@@ -382,7 +422,7 @@ __device__ __forceinline__ void onesweep_tile_region(
   lds_barrier();
 
   uint32_t tot[DPT], gstart[DPT];
-  uint32_t lbw[DPT][kLookback];
+  uint32_t lbw[DPT][kLookback] = {};
 #pragma unroll
   for (int k = 0; k < DPT; ++k) {
     const uint32_t b = t + k * BLOCK;
@@ -441,16 +481,28 @@ __device__ __forceinline__ void onesweep_tile_region(
         uint32_t r0 = 0;
 #pragma unroll
         for (int q = 1; q < kRegionsDev; ++q) r0 += first >= (uint32_t)q * region_elems ? 1u : 0u;
-        info = ((st + (r0 + 1u) * region_elems - first) << 3) | r0;
+        // slots at or past `split` fall in region r0 + 1 (none when the
+        // boundary lies beyond this tile; clamped so the shift cannot
+        // overflow)
+        uint32_t split = st + (r0 + 1u) * region_elems - first;
+        if (split > (uint32_t)T) split = (uint32_t)T;
+        info = (split << 3) | r0;
       }
       sm.grs[b] = make_uint2(first - st, info);
     }
   }
+  // Take every look-back load, used or not, on every path: one the compiler
+  // cannot prove retired makes it wait for all outstanding memory operations
+  // (the next tile's loads included) before it next reuses that register.
+#pragma unroll
+  for (int k = 0; k < DPT; ++k)
+#pragma unroll
+    for (int j = 0; j < kLookback; ++j) asm volatile("" ::"v"(lbw[k][j]));
   lds_barrier();
 
   uint32_t next;
   if (t == 0) next = atomicAdd(ticket, 1u);
-  if (hnext)
+  if (hnext && !count_later)
     store_run<BLOCK, ITEMS, NB, true>(sm.elems, sm.grs, hnext, tile_n, dst, shift, mask, nshift,
                                       nmask, ko);
   else
