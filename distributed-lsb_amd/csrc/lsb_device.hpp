@@ -156,18 +156,23 @@ __device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d, bool valid) {
   }
 }
 
-// hist_add for a wave whose 64 lanes are all active and valid, without
-// branches or exec-mask changes: when every lane holds one digit, lane 0
-// adds 64 to its counter and the other lanes add 0 to their own word of
-// `spare` (64 words of LDS, one per lane, so no two lanes share an address);
-// otherwise every lane adds 1 to its digit's counter.  lane0_64 is 64 on
-// lane 0 and 0 elsewhere.
-__device__ __forceinline__ void hist_add_full(uint32_t* h, uint32_t d, uint32_t* spare_lane,
-                                              uint32_t lane0_64) {
-  const uint32_t first = __builtin_amdgcn_readfirstlane(d);
-  const bool uni = __ballot(d != first) == 0;
-  uint32_t* addr = (uni && lane0_64 == 0u) ? spare_lane : h + d;
-  atomicAdd(addr, uni ? lane0_64 : 1u);
+// hist_add for one round of U keys per lane of a wave whose 64 lanes are all
+// active and valid: d[u] is key u's digit.  When all 64*U digits are equal
+// (sorted keys, zero high digits, heavy duplicates) one lane adds the whole
+// count -- a wave-uniform branch, tested once per round -- otherwise every
+// digit gets its own atomic.
+template <int U>
+__device__ __forceinline__ void hist_add_round(uint32_t* h, const uint32_t (&d)[U]) {
+  const uint32_t first = __builtin_amdgcn_readfirstlane(d[0]);
+  bool diff = false;
+#pragma unroll
+  for (int u = 0; u < U; ++u) diff |= d[u] != first;
+  if (__ballot(diff) == 0) {
+    if (lane_id() == 0) atomicAdd(&h[first], (uint32_t)(kWave * U));
+  } else {
+#pragma unroll
+    for (int u = 0; u < U; ++u) atomicAdd(&h[d[u]], 1u);
+  }
 }
 
 // ------------------------------------------------------------- pcg64 (device)

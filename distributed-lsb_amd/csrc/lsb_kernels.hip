@@ -42,7 +42,7 @@ void launch_zero(uint32_t* p, size_t words, hipStream_t s) {
 // chain.  Each block owns one contiguous chunk inside one region and adds
 // its counts for every digit into regcount[p][region][b].  P (the digit
 // count) is a template parameter so the per-key digit loop unrolls; full
-// rounds of 256 x U keys take hist_add_full, the chunk's tail the checked
+// rounds of 256 x U keys take hist_add_round, the chunk's tail the checked
 // hist_add.
 template <int D, int P>
 __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A, uint32_t n,
@@ -55,11 +55,8 @@ __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A,
   constexpr int NB = 1 << D;
   constexpr int U = 8;  // keys in flight per thread
   __shared__ uint32_t h[P * NB];
-  __shared__ uint32_t spare[kWave];
   const uint32_t t = threadIdx.x;
   for (uint32_t i = t; i < (uint32_t)(P * NB); i += 256) h[i] = 0u;
-  uint32_t* spare_lane = spare + lane_id();
-  const uint32_t lane0_64 = lane_id() == 0 ? (uint32_t)kWave : 0u;
   uint32_t mask[P];
 #pragma unroll
   for (int p = 0; p < P; ++p) {
@@ -92,11 +89,43 @@ __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A,
 #pragma unroll
       for (int u = 0; u < U; ++u) k[u] = key_fwd(k[u], ko);
     }
+    if constexpr (D == 8) {
+      // 8-bit digits never straddle the key's 32-bit halves: one field
+      // extract each.  Which digits the whole round shares is read from the
+      // OR of every key's XOR with lane 0's first key.
+      const uint32_t f_lo = __builtin_amdgcn_readfirstlane((uint32_t)k[0]);
+      const uint32_t f_hi = __builtin_amdgcn_readfirstlane((uint32_t)(k[0] >> 32));
+      uint32_t x_lo = 0, x_hi = 0;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+      for (int u = 0; u < U; ++u) {
+        x_lo |= (uint32_t)k[u] ^ f_lo;
+        x_hi |= (uint32_t)(k[u] >> 32) ^ f_hi;
+      }
 #pragma unroll
-      for (int p = 0; p < P; ++p)
-        hist_add_full(h + p * NB, digit_of(k[u], p * D, mask[p]), spare_lane, lane0_64);
+      for (int p = 0; p < P; ++p) {
+        const uint32_t sh = (uint32_t)(p * 8) & 31u;
+        const uint32_t w = (uint32_t)__popc(mask[p]);
+        const uint32_t x = p < 4 ? x_lo : x_hi;
+        if (__ballot(__builtin_amdgcn_ubfe(x, sh, w) != 0u) == 0) {
+          if (lane_id() == 0)
+            atomicAdd(&h[p * NB + __builtin_amdgcn_ubfe(p < 4 ? f_lo : f_hi, sh, w)],
+                      (uint32_t)(kWave * U));
+        } else {
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const uint32_t half = p < 4 ? (uint32_t)k[u] : (uint32_t)(k[u] >> 32);
+            atomicAdd(&h[p * NB + __builtin_amdgcn_ubfe(half, sh, w)], 1u);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        uint32_t d[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) d[u] = digit_of(k[u], p * D, mask[p]);
+        hist_add_round<U>(h + p * NB, d);
+      }
     }
   }
   for (; base < c1; base += 256) {
