@@ -31,33 +31,47 @@ __device__ __forceinline__ void wave_rank(const elem_t (&e)[ITEMS], uint32_t lim
                                           uint32_t mask, uint32_t* wh, uint32_t (&dg)[ITEMS],
                                           uint32_t (&rk)[ITEMS]) {
   const uint32_t lane = lane_id();
-  uint32_t old[ITEMS], lead[ITEMS];
+  uint32_t old[ITEMS], lead[ITEMS], plo[ITEMS], phi[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    const bool valid = (uint32_t)i * kWave + lane < lim;
-    const uint32_t d = digit_of(e[i].key, shift, mask);
-    const uint64_t vmask = __ballot(valid);
-    // Per bit: c = the lane's bit broadcast to 0 / ~0 (v_bfe_i32), the
-    // wave's ballot of it, and peers &= ~(ballot ^ c) on both halves with
-    // one v_bitop3 each (truth table 0x90 = a & (b == c)).
-    uint32_t plo = (uint32_t)vmask, phi = (uint32_t)(vmask >> 32);
+    dg[i] = digit_of(e[i].key, shift, mask);
+    const uint64_t vmask = __ballot((uint32_t)i * kWave + lane < lim);
+    plo[i] = (uint32_t)vmask;
+    phi[i] = (uint32_t)(vmask >> 32);
+  }
+  // Per bit: c = the lane's bit broadcast to 0 / ~0 (v_bfe_i32), the wave's
+  // ballot of it, and peers &= ~(ballot ^ c) on both halves with one v_bitop3
+  // each (truth table 0x90 = a & (b == c)).  Bits outside, a group of rows
+  // inside: the rows are independent, so one row's instructions fill the
+  // wait states between the other's ballot and its use.
+  // Rows in pairs: with four or more at once the wider live set spills.
+  constexpr int G = ITEMS < 2 ? ITEMS : 2;
+  static_assert(ITEMS % G == 0, "rows come in whole groups");
+#pragma unroll
+  for (int i0 = 0; i0 < ITEMS; i0 += G) {
 #pragma unroll
     for (int b = 0; b < D; ++b) {
-      // A pure VALU op in asm: the compiler would otherwise rewrite the
-      // field extract as two shifts and test the sign of the first.
-      uint32_t c;
-      asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(c) : "v"(d), "i"(b));
-      const uint64_t bal = __ballot(c != 0u);
-      plo = __builtin_amdgcn_bitop3_b32(plo, (uint32_t)bal, c, 0x90);
-      phi = __builtin_amdgcn_bitop3_b32(phi, (uint32_t)(bal >> 32), c, 0x90);
+#pragma unroll
+      for (int i = i0; i < i0 + G; ++i) {
+        // A pure VALU op in asm: the compiler would otherwise rewrite the
+        // field extract as two shifts and test the sign of the first.
+        uint32_t c;
+        asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(c) : "v"(dg[i]), "i"(b));
+        const uint64_t bal = __ballot(c != 0u);
+        plo[i] = __builtin_amdgcn_bitop3_b32(plo[i], (uint32_t)bal, c, 0x90);
+        phi[i] = __builtin_amdgcn_bitop3_b32(phi[i], (uint32_t)(bal >> 32), c, 0x90);
+      }
     }
-    const uint64_t peers = ((uint64_t)phi << 32) | plo;
-    const uint32_t below = count_below(peers);
-    old[i] = 0;
-    if (valid && below == 0) old[i] = atomicAdd(&wh[d], (uint32_t)__popcll(peers));
-    lead[i] = peers ? (uint32_t)__builtin_ctzll(peers) : lane;
-    dg[i] = d;
-    rk[i] = below;
+#pragma unroll
+    for (int i = i0; i < i0 + G; ++i) {
+      const bool valid = (uint32_t)i * kWave + lane < lim;
+      const uint64_t peers = ((uint64_t)phi[i] << 32) | plo[i];
+      const uint32_t below = count_below(peers);
+      old[i] = 0;
+      if (valid && below == 0) old[i] = atomicAdd(&wh[dg[i]], (uint32_t)__popcll(peers));
+      lead[i] = peers ? (uint32_t)__builtin_ctzll(peers) : lane;
+      rk[i] = below;
+    }
   }
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
