@@ -195,7 +195,8 @@ int make_plan(size_t n, const lsb_config& c, Plan* p) {
   p->ntiles = (uint32_t)((n + p->tile - 1) / p->tile);
   p->region_tiles = (p->ntiles + lsb::kRegions - 1) / lsb::kRegions;
   size_t off = 0;
-  p->counts = off;  off += align_up((size_t)p->passes * lsb::kRegions * p->nb * 4);
+  // counts: [P][R][2^d] words, then the lean histogram's two OR words
+  p->counts = off;  off += align_up((size_t)p->passes * lsb::kRegions * p->nb * 4 + 8);
   p->regnext = off; off += align_up((size_t)p->passes * lsb::kRegions * p->nb * 4);
   p->starts = off;  off += align_up((size_t)p->passes * p->nb * 4);
   p->plan = off;    off += align_up((size_t)(4 * p->passes + 4) * 4);
@@ -315,13 +316,23 @@ int enqueue_sort(lsb_elem* A, lsb_elem* B, uint32_t n, const Plan& p, char* ws,
     g_prof.valid = false;
     if (p.passes > 16) return LSB_EINVAL;
   }
+  // Region passes count each later digit for their successor, so their
+  // histogram is the lean one: the first kLeanDigits digits plus the OR mask
+  // that finds single-bucket digits (d = 8 only).
+  const size_t count_words = (size_t)p.passes * lsb::kRegions * p.nb;
+  const bool lean = p.v->persistent == 3 && p.d == 8 && p.passes > (uint32_t)lsb::kLeanDigits;
+  const uint32_t counted = lean ? (uint32_t)lsb::kLeanDigits : p.passes;
+  uint32_t* orr = lean ? counts + count_words : nullptr;
   if ((rc = prof_mark(prof, 0, s))) return rc;
-  lsb::launch_zero(counts, (size_t)p.passes * lsb::kRegions * p.nb, s);
-  LSB_HIP(lsb::launch_histogram(p.d, A, n, p.key_bits, p.ko, p.passes, counts,
-                                p.region_tiles * p.tile, status, slice, s));
+  lsb::launch_zero(counts, count_words + 2, s);
+  LSB_HIP(lsb::launch_histogram(p.d, A, n, p.key_bits, p.ko, p.passes, counted, counts,
+                                p.region_tiles * p.tile, status, slice, orr, s));
   if ((rc = prof_mark(prof, 1, s))) return rc;
   LSB_HIP(lsb::launch_scan_plan(p.d, counts, starts, plan, tickets, regnext, n, p.passes, flags,
-                                s));
+                                orr, p.key_bits, s));
+  if (lean)
+    LSB_HIP(lsb::launch_hist_first(p.d, A, n, p.key_bits, p.ko, counts, p.region_tiles * p.tile,
+                                   plan, p.passes, counted, s));
   if ((rc = prof_mark(prof, 2, s))) return rc;
   for (uint32_t q = 0; q < p.passes; ++q) {
     lsb::PassArgs a;
@@ -1028,8 +1039,8 @@ int lsb_debug_histogram(const lsb_elem* A, size_t n, const lsb_config* cfg, uint
   const size_t words = (size_t)p.passes * lsb::kRegions * p.nb;
   LSB_HIP(hipMallocAsync(&tmp, words * 4, s));
   lsb::launch_zero(tmp, words, s);
-  if (n) LSB_HIP(lsb::launch_histogram(p.d, A, (uint32_t)n, p.key_bits, p.ko, p.passes, tmp,
-                                       (uint32_t)n, nullptr, 0, s));
+  if (n) LSB_HIP(lsb::launch_histogram(p.d, A, (uint32_t)n, p.key_bits, p.ko, p.passes, p.passes,
+                                       tmp, (uint32_t)n, nullptr, 0, nullptr, s));
   for (uint32_t q = 0; q < p.passes; ++q)
     LSB_HIP(hipMemcpyAsync(counts + (size_t)q * p.nb, tmp + (size_t)q * lsb::kRegions * p.nb,
                            p.nb * 4, hipMemcpyDeviceToDevice, s));

@@ -35,6 +35,7 @@ struct PassArgs {
   uint32_t mask;
   const uint32_t* plan;
   const uint32_t* starts;  // starts of this pass: global exclusive bucket offsets
+                           // (the region form makes its own from the counts)
   uint32_t* tickets;       // tile ticket counters [pass][region]
   uint32_t* status;        // two look-back slices of tiles * 2^d words
   uint32_t ntiles;
@@ -62,17 +63,32 @@ const SortVariant* sort_variants(int* count);
 
 // Zeroes p[0..words), launched before the histogram.
 void launch_zero(uint32_t* p, size_t words, hipStream_t s);
-// All-digit histogram per region: regcount[p][region][b] += ..., with
-// regions of region_elems elements; zeroes zero[0..zero_words).
+// Digits the lean histogram counts up front (the region passes count each
+// later digit for their successor while they scatter).
+constexpr int kLeanDigits = 2;
+// Histogram per region of digits 0 .. counted-1: regcount[p][region][b] +=
+// ..., with regions of region_elems elements; zeroes zero[0..zero_words).
+// counted == passes counts every digit; counted == kLeanDigits (d = 8 only)
+// is the lean form, which also ORs every key's XOR with key 0 into orr[0..1]
+// (zeroed beforehand; n >= 1).
 hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
-                            const KeyOrder& ko, uint32_t passes, uint32_t* regcount,
-                            uint32_t region_elems, uint32_t* zero, size_t zero_words,
-                            hipStream_t s);
+                            const KeyOrder& ko, uint32_t passes, uint32_t counted,
+                            uint32_t* regcount, uint32_t region_elems, uint32_t* zero,
+                            size_t zero_words, uint32_t* orr, hipStream_t s);
+// Lean form, after the plan: the per-region counts of the first executed
+// pass when it is not one of the `counted` digits (else every block returns
+// at once).
+hipError_t launch_hist_first(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
+                             const KeyOrder& ko, uint32_t* regcount, uint32_t region_elems,
+                             const uint32_t* plan, uint32_t passes, uint32_t counted,
+                             hipStream_t s);
 // Global counts (sum of regions), exclusive scans, skip detection, plan;
-// clears tickets[P][R] and regnext[P][R][2^d].
+// clears tickets[P][R] and regnext[P][R][2^d].  With orr (lean form) skip
+// detection reads orr and no starts are written.
 hipError_t launch_scan_plan(uint32_t d, const uint32_t* regcount, uint32_t* starts,
                             uint32_t* plan, uint32_t* tickets, uint32_t* regnext, uint32_t n,
-                            uint32_t passes, uint32_t flags, hipStream_t s);
+                            uint32_t passes, uint32_t flags, const uint32_t* orr,
+                            uint32_t key_bits, hipStream_t s);
 // Whole sort of one array of n <= small_sort_capacity() elements inside one
 // workgroup (one HBM read and write per element).
 uint32_t small_sort_capacity();

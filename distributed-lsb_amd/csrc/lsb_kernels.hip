@@ -40,26 +40,75 @@ void launch_zero(uint32_t* p, size_t words, hipStream_t s) {
 // The array is cut into kRegions equal regions of `region_elems` elements
 // (whole tiles); the scatter passes give each region its own look-back
 // chain.  Each block owns one contiguous chunk inside one region and adds
-// its counts for every digit into regcount[p][region][b].  P (the digit
-// count) is a template parameter so the per-key digit loop unrolls; full
-// rounds of 256 x U keys take hist_add_round, the chunk's tail the checked
-// hist_add.
-template <int D, int P>
+// its counts for digits 0 .. L-1 into regcount[p][region][b].  P (the digit
+// count of the key) and L are template parameters so the per-key digit loop
+// unrolls.  Full rounds of 256 x U keys take count_row, the chunk's tail the
+// checked hist_add.
+//
+// With orr given (L < P: the region passes count each later digit
+// themselves) the kernel also ORs every key's XOR with key 0 into orr[0..1]:
+// digit p is single-bucket exactly when its bits of that OR are zero.
+
+// Counts digits 0 .. L-1 of one key per lane, all 64 lanes active and valid.
+// A digit every lane of the wave shares (zero high digits, sorted or
+// duplicate keys) is added once by lane 0 instead of by a 64-way
+// same-address atomic.
+template <int D, int L>
+__device__ __forceinline__ void count_row(uint32_t* h, uint64_t key, const uint32_t (&mask)[L]) {
+  constexpr int NB = 1 << D;
+  if constexpr (D == 8) {
+    // 8-bit digits never straddle the key's 32-bit halves: one field
+    // extract each; the digits lane 0 shares with everyone are read from
+    // the XOR with lane 0's key.
+    const uint32_t lo = (uint32_t)key, hi = (uint32_t)(key >> 32);
+    const uint32_t f_lo = __builtin_amdgcn_readfirstlane(lo);
+    const uint32_t f_hi = __builtin_amdgcn_readfirstlane(hi);
+    const uint32_t y_lo = lo ^ f_lo, y_hi = hi ^ f_hi;
+#pragma unroll
+    for (int p = 0; p < L; ++p) {
+      const uint32_t sh = (uint32_t)(p * 8) & 31u;
+      const uint32_t w = (uint32_t)__popc(mask[p]);
+      if (__ballot(__builtin_amdgcn_ubfe(p < 4 ? y_lo : y_hi, sh, w) != 0u) == 0) {
+        if (lane_id() == 0)
+          atomicAdd(&h[p * NB + __builtin_amdgcn_ubfe(p < 4 ? f_lo : f_hi, sh, w)],
+                    (uint32_t)kWave);
+      } else {
+        atomicAdd(&h[p * NB + __builtin_amdgcn_ubfe(p < 4 ? lo : hi, sh, w)], 1u);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int p = 0; p < L; ++p) {
+      const uint32_t d = digit_of(key, p * D, mask[p]);
+      const uint32_t first = __builtin_amdgcn_readfirstlane(d);
+      if (__ballot(d != first) == 0) {
+        if (lane_id() == 0) atomicAdd(&h[p * NB + first], (uint32_t)kWave);
+      } else {
+        atomicAdd(&h[p * NB + d], 1u);
+      }
+    }
+  }
+}
+
+template <int D, int P, int L>
 __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A, uint32_t n,
                                                    uint32_t key_bits, KeyOrder ko,
                                                    uint32_t* __restrict__ regcount,
                                                    uint32_t region_elems,
                                                    uint32_t blocks_per_region,
                                                    uint32_t* __restrict__ zero,
-                                                   size_t zero_words) {
+                                                   size_t zero_words, uint32_t* __restrict__ orr) {
+  static_assert(L >= 1 && L <= P, "digits counted");
   constexpr int NB = 1 << D;
   constexpr int U = 8;  // keys in flight per thread
-  __shared__ uint32_t h[P * NB];
+  __shared__ uint32_t h[L * NB];
+  __shared__ uint32_t s_or[2];
   const uint32_t t = threadIdx.x;
-  for (uint32_t i = t; i < (uint32_t)(P * NB); i += 256) h[i] = 0u;
-  uint32_t mask[P];
+  for (uint32_t i = t; i < (uint32_t)(L * NB); i += 256) h[i] = 0u;
+  if (t < 2) s_or[t] = 0u;
+  uint32_t mask[L];
 #pragma unroll
-  for (int p = 0; p < P; ++p) {
+  for (int p = 0; p < L; ++p) {
     const uint32_t w = key_bits - p * D < (uint32_t)D ? key_bits - p * D : (uint32_t)D;
     mask[p] = (1u << w) - 1u;
   }
@@ -80,92 +129,91 @@ __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A,
   const size_t per = (region_elems + blocks_per_region - 1) / blocks_per_region;
   const size_t c0 = r0 + (size_t)chunk * per;
   const size_t c1 = c0 + per < r1 ? c0 + per : r1;
+  const uint64_t ref = orr ? key_fwd(A[0].key, ko) : 0ull;
+  uint64_t acc = 0;
   size_t base = c0;
-  for (; base + 256 * U <= c1; base += 256 * U) {
-    uint64_t k[U];
+  // Double-buffered: the next round's loads are issued before this round is
+  // counted, so the counting hides under them.
+  uint64_t k[U], kn[U];
+  if (base + 256 * U <= c1) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) k[u] = A[base + (size_t)u * 256 + t].key;
+    for (int u = 0; u < U; ++u) kn[u] = A[base + (size_t)u * 256 + t].key;
+  }
+  for (; base + 256 * U <= c1; base += 256 * U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) k[u] = kn[u];
+    if (base + 2 * 256 * U <= c1) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) kn[u] = A[base + 256 * U + (size_t)u * 256 + t].key;
+    }
     if (ko.sign) {
 #pragma unroll
       for (int u = 0; u < U; ++u) k[u] = key_fwd(k[u], ko);
     }
-    if constexpr (D == 8) {
-      // 8-bit digits never straddle the key's 32-bit halves: one field
-      // extract each.  Which digits the whole round shares is read from the
-      // OR of every key's XOR with lane 0's first key.
-      const uint32_t f_lo = __builtin_amdgcn_readfirstlane((uint32_t)k[0]);
-      const uint32_t f_hi = __builtin_amdgcn_readfirstlane((uint32_t)(k[0] >> 32));
-      uint32_t x_lo = 0, x_hi = 0;
+    if (orr) {
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        x_lo |= (uint32_t)k[u] ^ f_lo;
-        x_hi |= (uint32_t)(k[u] >> 32) ^ f_hi;
-      }
-#pragma unroll
-      for (int p = 0; p < P; ++p) {
-        const uint32_t sh = (uint32_t)(p * 8) & 31u;
-        const uint32_t w = (uint32_t)__popc(mask[p]);
-        const uint32_t x = p < 4 ? x_lo : x_hi;
-        if (__ballot(__builtin_amdgcn_ubfe(x, sh, w) != 0u) == 0) {
-          if (lane_id() == 0)
-            atomicAdd(&h[p * NB + __builtin_amdgcn_ubfe(p < 4 ? f_lo : f_hi, sh, w)],
-                      (uint32_t)(kWave * U));
-        } else {
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const uint32_t half = p < 4 ? (uint32_t)k[u] : (uint32_t)(k[u] >> 32);
-            atomicAdd(&h[p * NB + __builtin_amdgcn_ubfe(half, sh, w)], 1u);
-          }
-        }
-      }
-    } else {
-#pragma unroll
-      for (int p = 0; p < P; ++p) {
-        uint32_t d[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) d[u] = digit_of(k[u], p * D, mask[p]);
-        hist_add_round<U>(h + p * NB, d);
-      }
+      for (int u = 0; u < U; ++u) acc |= k[u] ^ ref;
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u) count_row<D, L>(h, k[u], mask);
   }
   for (; base < c1; base += 256) {
     const size_t i = base + t;
     const bool v = i < c1;
     const uint64_t k = v ? key_fwd(A[i].key, ko) : 0ull;
+    if (v) acc |= k ^ ref;
 #pragma unroll
-    for (int p = 0; p < P; ++p) hist_add(h + p * NB, digit_of(k, p * D, mask[p]), v);
+    for (int p = 0; p < L; ++p) hist_add(h + p * NB, digit_of(k, p * D, mask[p]), v);
+  }
+  if (orr) {
+    if ((uint32_t)acc) atomicOr(&s_or[0], (uint32_t)acc);
+    if ((uint32_t)(acc >> 32)) atomicOr(&s_or[1], (uint32_t)(acc >> 32));
   }
   __syncthreads();
+  if (orr && t < 2 && s_or[t]) atomicOr(&orr[t], s_or[t]);
   if (c0 >= c1) return;
-  for (uint32_t i = t; i < (uint32_t)(P * NB); i += 256) {
+  for (uint32_t i = t; i < (uint32_t)(L * NB); i += 256) {
     const uint32_t c = h[i];
     const uint32_t p = i / NB, b = i % NB;
     if (c) atomicAdd(&regcount[((size_t)p * kRegions + region) * NB + b], c);
   }
 }
 
-template <int D, int P>
+template <int D, int P, int L>
 static void launch_hist(const elem_t* a, uint32_t n, uint32_t key_bits, const KeyOrder& ko,
                         uint32_t* regcount, uint32_t region_elems, uint32_t per_region,
-                        uint32_t* zero, size_t zero_words, hipStream_t s) {
-  hipLaunchKernelGGL((k_histogram<D, P>), dim3(per_region * kRegions), dim3(256), 0, s, a, n,
-                     key_bits, ko, regcount, region_elems, per_region, zero, zero_words);
+                        uint32_t* zero, size_t zero_words, uint32_t* orr, hipStream_t s) {
+  hipLaunchKernelGGL((k_histogram<D, P, L>), dim3(per_region * kRegions), dim3(256), 0, s, a, n,
+                     key_bits, ko, regcount, region_elems, per_region, zero, zero_words, orr);
 }
 
-hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
-                            const KeyOrder& ko, uint32_t passes, uint32_t* regcount,
-                            uint32_t region_elems, uint32_t* zero, size_t zero_words,
-                            hipStream_t s) {
-  // Enough blocks to keep every CU streaming, few enough that the final
-  // flush of LDS counters stays small.
+// Blocks per region of the histogram kernels: enough to keep every CU
+// streaming, few enough that the final flush of LDS counters stays small.
+static uint32_t hist_blocks_per_region(uint32_t region_elems) {
   size_t per_region = ((size_t)region_elems + 256 * 8 * 8 - 1) / (256 * 8 * 8);
   if (per_region < 1) per_region = 1;
   if (per_region > 256) per_region = 256;
-  const uint32_t pr = (uint32_t)per_region;
+  return (uint32_t)per_region;
+}
+
+hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
+                            const KeyOrder& ko, uint32_t passes, uint32_t counted,
+                            uint32_t* regcount, uint32_t region_elems, uint32_t* zero,
+                            size_t zero_words, uint32_t* orr, hipStream_t s) {
+  const uint32_t pr = hist_blocks_per_region(region_elems);
   const elem_t* a = static_cast<const elem_t*>(A);
-#define LSB_HIST_CASE(DD, PP) \
-  case PP:                   \
-    launch_hist<DD, PP>(a, n, key_bits, ko, regcount, region_elems, pr, zero, zero_words, s); \
+  // counted == passes: every digit (orr unused); counted == kLeanDigits with
+  // orr: the lean form for the region passes.
+  const bool lean = counted < passes;
+  if (lean && (counted != (uint32_t)kLeanDigits || !orr)) return hipErrorInvalidValue;
+#define LSB_HIST_CASE(DD, PP)                                                                    \
+  case PP:                                                                                     \
+    if (lean)                                                                                  \
+      launch_hist<DD, PP, (PP > kLeanDigits ? kLeanDigits : PP)>(                              \
+          a, n, key_bits, ko, regcount, region_elems, pr, zero, zero_words, orr, s);           \
+    else                                                                                       \
+      launch_hist<DD, PP, PP>(a, n, key_bits, ko, regcount, region_elems, pr, zero,            \
+                              zero_words, orr, s);                                             \
     break;
   if (d == 8) {
     switch (passes) {
@@ -186,24 +234,116 @@ hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_
   return hipGetLastError();
 }
 
+// ----------------------------------------------------------- k_hist_first
+// Lean form only: when the digits counted up front are all single-bucket,
+// the first executed pass is a later one and its per-region counts come from
+// here (one more read of the keys).  Otherwise every block returns at once.
+template <int D>
+__global__ __launch_bounds__(256) void k_hist_first(const elem_t* __restrict__ A, uint32_t n,
+                                                    uint32_t key_bits, KeyOrder ko,
+                                                    uint32_t* __restrict__ regcount,
+                                                    uint32_t region_elems,
+                                                    uint32_t blocks_per_region,
+                                                    const uint32_t* __restrict__ plan,
+                                                    uint32_t passes, uint32_t counted) {
+  constexpr int NB = 1 << D;
+  constexpr int U = 8;
+  uint32_t f = passes;
+  for (uint32_t p = 0; p < passes; ++p)
+    if (plan[4 * p]) {
+      f = p;
+      break;
+    }
+  if (f >= passes || f < counted) return;
+  __shared__ uint32_t h[NB];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < (uint32_t)NB; i += 256) h[i] = 0u;
+  __syncthreads();
+  const uint32_t shift = f * D;
+  const uint32_t w = key_bits - shift < (uint32_t)D ? key_bits - shift : (uint32_t)D;
+  const uint32_t mask = (1u << w) - 1u;
+  const uint32_t region = blockIdx.x / blocks_per_region;
+  const uint32_t chunk = blockIdx.x % blocks_per_region;
+  const size_t r0 = (size_t)region * region_elems;
+  const size_t r1 = r0 + region_elems < n ? r0 + region_elems : n;
+  const size_t per = (region_elems + blocks_per_region - 1) / blocks_per_region;
+  const size_t c0 = r0 + (size_t)chunk * per;
+  const size_t c1 = c0 + per < r1 ? c0 + per : r1;
+  size_t base = c0;
+  for (; base + 256 * U <= c1; base += 256 * U) {
+    uint64_t k[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) k[u] = key_fwd(A[base + (size_t)u * 256 + t].key, ko);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t d = digit_of(k[u], shift, mask);
+      const uint32_t first = __builtin_amdgcn_readfirstlane(d);
+      if (__ballot(d != first) == 0) {
+        if (lane_id() == 0) atomicAdd(&h[first], (uint32_t)kWave);
+      } else {
+        atomicAdd(&h[d], 1u);
+      }
+    }
+  }
+  for (; base < c1; base += 256) {
+    const size_t i = base + t;
+    const bool v = i < c1;
+    const uint64_t k = v ? key_fwd(A[i].key, ko) : 0ull;
+    hist_add(h, digit_of(k, shift, mask), v);
+  }
+  __syncthreads();
+  if (c0 >= c1) return;
+  for (uint32_t i = t; i < (uint32_t)NB; i += 256) {
+    const uint32_t c = h[i];
+    if (c) atomicAdd(&regcount[((size_t)f * kRegions + region) * NB + i], c);
+  }
+}
+
+hipError_t launch_hist_first(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
+                             const KeyOrder& ko, uint32_t* regcount, uint32_t region_elems,
+                             const uint32_t* plan, uint32_t passes, uint32_t counted,
+                             hipStream_t s) {
+  if (d != 8) return hipErrorInvalidValue;
+  const uint32_t pr = hist_blocks_per_region(region_elems);
+  hipLaunchKernelGGL(k_hist_first<8>, dim3(pr * kRegions), dim3(256), 0, s,
+                     static_cast<const elem_t*>(A), n, key_bits, ko, regcount, region_elems, pr,
+                     plan, passes, counted);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------ k_scan_plan
 // Global count of bucket b = sum over regions; exclusive scan per digit;
 // single-bucket detection; plan; clears the tickets and the per-region
-// counters the scatter passes fill for their successors.
+// counters the scatter passes fill for their successors.  With orr (the
+// lean histogram) a digit is single-bucket when its bits of orr are zero,
+// and no starts are made: the region passes scan their own counts.
 template <int NB>
 __global__ __launch_bounds__(1024) void k_scan_plan(const uint32_t* __restrict__ regcount,
                                                     uint32_t* __restrict__ starts,
                                                     uint32_t* __restrict__ plan,
                                                     uint32_t* __restrict__ tickets,
                                                     uint32_t* __restrict__ regnext, uint32_t n,
-                                                    uint32_t passes, uint32_t flags) {
+                                                    uint32_t passes, uint32_t flags,
+                                                    const uint32_t* __restrict__ orr,
+                                                    uint32_t key_bits, uint32_t d) {
   __shared__ uint32_t a[NB];
   __shared__ uint32_t wsum[16];
   __shared__ uint32_t trivial[16];
   const uint32_t t = threadIdx.x;
   for (uint32_t i = t; i < passes * kRegions * NB; i += 1024) regnext[i] = 0u;
   for (uint32_t i = t; i < passes * kRegions; i += 1024) tickets[i] = 0u;
-  for (uint32_t p = 0; p < passes; ++p) {
+  if (orr) {
+    if (t == 0) {
+      const uint64_t x = (uint64_t)orr[0] | ((uint64_t)orr[1] << 32);
+      for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t shift = p * d;
+        const uint32_t w = key_bits - shift < d ? key_bits - shift : d;
+        trivial[p] = ((x >> shift) & ((1ull << w) - 1ull)) == 0 ? 1u : 0u;
+      }
+    }
+    __syncthreads();
+  }
+  for (uint32_t p = 0; p < passes && !orr; ++p) {
     int triv = 0;
     for (uint32_t i = t; i < NB; i += 1024) {
       uint32_t c = 0;
@@ -234,13 +374,14 @@ __global__ __launch_bounds__(1024) void k_scan_plan(const uint32_t* __restrict__
 
 hipError_t launch_scan_plan(uint32_t d, const uint32_t* regcount, uint32_t* starts,
                             uint32_t* plan, uint32_t* tickets, uint32_t* regnext, uint32_t n,
-                            uint32_t passes, uint32_t flags, hipStream_t s) {
+                            uint32_t passes, uint32_t flags, const uint32_t* orr,
+                            uint32_t key_bits, hipStream_t s) {
   if (d == 8)
     hipLaunchKernelGGL(k_scan_plan<256>, dim3(1), dim3(1024), 0, s, regcount, starts, plan,
-                       tickets, regnext, n, passes, flags);
+                       tickets, regnext, n, passes, flags, orr, key_bits, d);
   else if (d == 11)
     hipLaunchKernelGGL(k_scan_plan<2048>, dim3(1), dim3(1024), 0, s, regcount, starts, plan,
-                       tickets, regnext, n, passes, flags);
+                       tickets, regnext, n, passes, flags, orr, key_bits, d);
   else
     return hipErrorInvalidValue;
   return hipSuccess;
@@ -321,12 +462,22 @@ __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(Pass
     const uint32_t w = args.key_bits - nshift < (uint32_t)D ? args.key_bits - nshift : (uint32_t)D;
     nmask = (1u << w) - 1u;
   }
+  // Bucket b of region r starts at the global start of b (exclusive scan of
+  // the counts summed over regions) plus b's count in regions before r.
   const uint32_t* C = (ord == 0 ? args.regcount : args.regnext) + (size_t)pass * kRegions * NB;
   for (uint32_t b = threadIdx.x; b < (uint32_t)NB; b += BLOCK) {
-    uint32_t s = args.starts[b];
-    for (uint32_t r = 0; r < region; ++r) s += C[(size_t)r * NB + b];
-    s_rbase[b] = s;
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < (uint32_t)kRegions; ++r) {
+      const uint32_t c = C[(size_t)r * NB + b];
+      before += r < region ? c : 0u;
+      all += c;
+    }
+    s_rbase[b] = before;
+    sm.start[b] = all;
   }
+  block_exclusive_scan<NB, BLOCK>(sm.start, sm.wsum);
+  for (uint32_t b = threadIdx.x; b < (uint32_t)NB; b += BLOCK) s_rbase[b] += sm.start[b];
   for (uint32_t i = threadIdx.x; i < (uint32_t)(kRegions * NB); i += BLOCK) s_hnext[i] = 0u;
   if (kOwnCounters)
     for (uint32_t i = threadIdx.x; i < (uint32_t)((BLOCK / kWave) * NB); i += BLOCK) s_whist[i] = 0u;

@@ -158,6 +158,28 @@ def test_skew_and_stability(case):
         assert np.array_equal(gpu_sort(a, variant=v, no_skip=True), want)
 
 
+@pytest.mark.parametrize("low_bits", [8, 16, 24])
+def test_constant_low_digits(low_bits):
+    """Keys whose low bytes are one constant: those passes are skipped and
+    the first executed pass is a later digit -- for the region form one the
+    up-front histogram counts (digit 1) or one a second read counts (digits
+    2, 3)."""
+    n = 300_007
+    a = O.gen_uniform(n, ranks=2)
+    low = np.uint64(0x5AA55A & ((1 << low_bits) - 1))
+    a["key"] = (a["key"] << np.uint64(low_bits)) | low
+    want = stable(a)
+    skipped = low_bits // 8
+    for v, d, tile, _ in variants():
+        A = to_dev(a)
+        B = torch.empty_like(A)
+        lsb_amd.sort(A, B, variant=v, sync=True, profile=True)
+        executed = lsb_amd.profile_read()["executed"]
+        assert np.array_equal(from_dev(A), want), (low_bits, v, d, tile)
+        if d == 8:
+            assert list(executed) == [0] * skipped + [1] * (8 - skipped), (low_bits, v, executed)
+
+
 @pytest.mark.parametrize("key_bits", [1, 8, 16, 21, 32, 33, 40])
 def test_key_bits_and_per_pass_prefix(key_bits):
     """Sorting by the low key_bits is what the first passes leave behind:
