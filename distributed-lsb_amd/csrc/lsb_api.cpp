@@ -243,6 +243,51 @@ int check_device_range(const void* ptr, size_t bytes) {
 }
 
 // Library-managed workspaces, one per (device, stream), grown on demand.
+}  // namespace
+
+namespace lsb {
+
+// One memory pool per device for the library's own stream-ordered
+// allocations (workspaces, the server's scratch).  It keeps freed memory
+// for reuse instead of returning it at every synchronisation, and being the
+// library's own pool it leaves the device's default pool as the caller set
+// it.
+hipError_t pool_alloc(void** p, size_t bytes, hipStream_t s) {
+  static std::mutex mu;
+  static std::map<int, hipMemPool_t> pools;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  hipMemPool_t pool = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = pools.find(dev);
+    if (it == pools.end()) {
+      hipMemPoolProps props;
+      memset(&props, 0, sizeof props);
+      props.allocType = hipMemAllocationTypePinned;
+      props.handleTypes = hipMemHandleTypeNone;
+      props.location.type = hipMemLocationTypeDevice;
+      props.location.id = dev;
+      if ((e = hipMemPoolCreate(&pool, &props)) != hipSuccess) return e;
+      uint64_t keep = UINT64_MAX;
+      if ((e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep)) !=
+          hipSuccess) {
+        (void)hipMemPoolDestroy(pool);
+        return e;
+      }
+      pools[dev] = pool;
+    } else {
+      pool = it->second;
+    }
+  }
+  return hipMallocFromPoolAsync(p, bytes, pool, s);
+}
+
+}  // namespace lsb
+
+namespace {
+
 struct Workspace {
   void* ptr = nullptr;
   size_t bytes = 0;
@@ -256,21 +301,18 @@ int get_workspace(void* stream, size_t need, void** out) {
   std::lock_guard<std::mutex> lock(g_ws_mu);
   Workspace& w = g_ws[std::make_pair(dev, stream)];
   if (w.bytes < need) {
-    if (w.ptr) {
-      // Earlier sorts on this stream may still use the old buffer.
-      LSB_HIP(hipStreamSynchronize((hipStream_t)stream));
-      LSB_HIP(hipFree(w.ptr));
-      w.ptr = nullptr;
-      w.bytes = 0;
-    }
-    size_t bytes = need + need / 4;
-    const hipError_t he = hipMalloc(&w.ptr, bytes);
+    // Stream-ordered: the old buffer is released after the sorts already
+    // queued on this stream, so growing waits for nothing.
+    const size_t bytes = need + need / 4;
+    void* p = nullptr;
+    const hipError_t he = lsb::pool_alloc(&p, bytes, (hipStream_t)stream);
     if (he != hipSuccess) {
       g_last_hip = (int)he;
       (void)hipGetLastError();
-      w.ptr = nullptr;
       return LSB_ENOMEM;
     }
+    if (w.ptr) LSB_HIP(hipFreeAsync(w.ptr, (hipStream_t)stream));
+    w.ptr = p;
     w.bytes = bytes;
   }
   *out = w.ptr;
@@ -709,21 +751,16 @@ int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_c
       const uint32_t grow_s =
           cap_s > W.cap_segs ? std::max<uint32_t>(cap_s, W.cap_segs + W.cap_segs / 4) : cap_s;
       if ((rc = make_seg_layout(grow_n, grow_s, c, &L, typical_n))) return rc;
-      if (W.ptr) {
-        LSB_HIP(hipStreamSynchronize(s));
-        LSB_HIP(hipFree(W.ptr));
-        W.ptr = nullptr;
-        W.bytes = 0;
-      }
-      const hipError_t he = hipMalloc(&W.ptr, L.total);
+      // Stream-ordered, like get_workspace: no wait for earlier batches.
+      void* p = nullptr;
+      const hipError_t he = lsb::pool_alloc(&p, L.total, s);
       if (he != hipSuccess) {
         g_last_hip = (int)he;
         (void)hipGetLastError();
-        W.ptr = nullptr;
-        W.cap_n = 0;
-        W.cap_segs = 0;
         return LSB_ENOMEM;
       }
+      if (W.ptr) LSB_HIP(hipFreeAsync(W.ptr, s));
+      W.ptr = p;
       W.bytes = L.total;
       W.cap_n = grow_n;
       W.cap_segs = grow_s;

@@ -61,6 +61,11 @@ struct SortVariant {
 // Compiled variants (lsb_kernels.hip).
 const SortVariant* sort_variants(int* count);
 
+// Stream-ordered allocation from the library's own memory pool on the
+// current device (freed with hipFreeAsync); the pool keeps freed memory for
+// reuse (lsb_api.cpp).
+hipError_t pool_alloc(void** p, size_t bytes, hipStream_t s);
+
 // Zeroes p[0..words), launched before the histogram.
 void launch_zero(uint32_t* p, size_t words, hipStream_t s);
 // Digits the lean histogram counts up front (the region passes count each

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "../../include/lsb.h"
+#include "lsb_internal.hpp"
 
 namespace {
 
@@ -134,7 +135,7 @@ int lsb_server::launch_plain(Job& j) {
   L.jobs.push_back(j.id);
   int rc = LSB_OK;
   if (j.n > 1) {
-    rc = hip_rc(hipMallocAsync(&L.scratch, j.n * sizeof(lsb_elem), sl.s));
+    rc = hip_rc(lsb::pool_alloc(&L.scratch, j.n * sizeof(lsb_elem), sl.s));
     if (rc == LSB_OK) {
       lsb_config c;
       lsb_config_init(&c);
@@ -329,6 +330,37 @@ void lsb_server::run() {
   }
 }
 
+namespace {
+
+// Checks of one job that need no server state.
+int check_job(const lsb_elem* A, size_t n, uint32_t key_bits, uint32_t key_flags) {
+  if (n > LSB_MAX_N) return LSB_ERANGE;
+  if (key_bits > 64 || (n > 1 && (!A || ((uintptr_t)A & 15u)))) return LSB_EINVAL;
+  if (key_flags != 0 && key_flags != LSB_KEYS_SIGNED && key_flags != LSB_KEYS_FLOAT)
+    return LSB_EINVAL;
+  if (key_flags == LSB_KEYS_FLOAT && key_bits != 32 && key_bits != 64 && key_bits != 0)
+    return LSB_EINVAL;
+  return LSB_OK;
+}
+
+// Queues one checked job; the caller holds s->mu.
+uint64_t enqueue_job(lsb_server* s, lsb_elem* A, size_t n, uint32_t key_bits,
+                     uint32_t key_flags, double t) {
+  Job j;
+  j.id = s->next_id++;
+  j.A = A;
+  j.n = n;
+  j.key_bits = key_bits ? key_bits : 64;
+  j.key_flags = key_flags;
+  j.submitted = t;
+  if (s->first_submit == 0) s->first_submit = t;
+  s->jobs[j.id] = j;
+  s->pending.push_back(j.id);
+  return j.id;
+}
+
+}  // namespace
+
 extern "C" {
 
 void lsb_server_config_init(lsb_server_config* c) {
@@ -395,31 +427,38 @@ int lsb_server_create(const lsb_server_config* cfg, lsb_server** out) {
   return LSB_OK;
 }
 
+
 int lsb_server_submit(lsb_server* s, lsb_elem* A, size_t n, uint32_t key_bits,
                       uint32_t key_flags, uint64_t* job_id) {
   if (!s || !job_id) return LSB_EINVAL;
-  if (n > LSB_MAX_N) return LSB_ERANGE;
-  if (key_bits > 64 || (n > 1 && (!A || ((uintptr_t)A & 15u)))) return LSB_EINVAL;
-  if (key_flags != 0 && key_flags != LSB_KEYS_SIGNED && key_flags != LSB_KEYS_FLOAT)
-    return LSB_EINVAL;
-  if (key_flags == LSB_KEYS_FLOAT && key_bits != 32 && key_bits != 64 && key_bits != 0)
-    return LSB_EINVAL;
+  int rc = check_job(A, n, key_bits, key_flags);
+  if (rc) return rc;
   std::lock_guard<std::mutex> lock(s->mu);
   if (s->stop) return LSB_EINVAL;
   if (s->pending.size() >= s->cfg.max_pending) return LSB_EBUSY;
   if (32ull * n > s->cfg.memory_budget) return LSB_ENOMEM;  // could never be admitted
-  Job j;
-  j.id = s->next_id++;
-  j.A = A;
-  j.n = n;
-  j.key_bits = key_bits ? key_bits : 64;
-  j.key_flags = key_flags;
-  j.submitted = now_s();
-  if (s->first_submit == 0) s->first_submit = j.submitted;
-  s->jobs[j.id] = j;
-  s->pending.push_back(j.id);
-  *job_id = j.id;
+  *job_id = enqueue_job(s, A, n, key_bits, key_flags, now_s());
   s->cv_submit.notify_one();
+  return LSB_OK;
+}
+
+int lsb_server_submit_batch(lsb_server* s, lsb_elem* const* As, const size_t* ns,
+                            const uint32_t* key_bits, const uint32_t* key_flags, size_t count,
+                            uint64_t* job_ids) {
+  if (!s || (count && (!As || !ns || !key_bits || !key_flags || !job_ids))) return LSB_EINVAL;
+  for (size_t i = 0; i < count; ++i) {
+    const int rc = check_job(As[i], ns[i], key_bits[i], key_flags[i]);
+    if (rc) return rc;
+  }
+  std::lock_guard<std::mutex> lock(s->mu);
+  if (s->stop) return LSB_EINVAL;
+  if (s->pending.size() + count > s->cfg.max_pending) return LSB_EBUSY;
+  for (size_t i = 0; i < count; ++i)
+    if (32ull * ns[i] > s->cfg.memory_budget) return LSB_ENOMEM;
+  const double t = now_s();
+  for (size_t i = 0; i < count; ++i)
+    job_ids[i] = enqueue_job(s, As[i], ns[i], key_bits[i], key_flags[i], t);
+  if (count) s->cv_submit.notify_one();
   return LSB_OK;
 }
 

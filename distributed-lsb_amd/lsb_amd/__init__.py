@@ -111,6 +111,7 @@ SIGNATURES = {
     "lsb_server_create": (_I, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
     "lsb_server_destroy": (_I, [_P]),
     "lsb_server_submit": (_I, [_P, _P, _SZ, _U32, _U32, ctypes.POINTER(_U64)]),
+    "lsb_server_submit_batch": (_I, [_P, _P, _P, _P, _P, _SZ, _P]),
     "lsb_server_poll": (_I, [_P, _U64]),
     "lsb_server_wait": (_I, [_P, _U64, ctypes.c_double]),
     "lsb_server_drain": (_I, [_P]),
@@ -403,6 +404,24 @@ class Server:
                                        ctypes.byref(jid)),
                "lsb_server_submit")
         return jid.value
+
+    def submit_many(self, arrays, key_bits=64, key_order="unsigned"):
+        """Queues every array at once (lsb_server_submit_batch, all or
+        nothing); key_bits / key_order are one value or one per array.
+        Returns the job ids in order."""
+        m = len(arrays)
+        kb = list(key_bits) if isinstance(key_bits, (list, tuple)) else [key_bits] * m
+        ko = list(key_order) if isinstance(key_order, (list, tuple)) else [key_order] * m
+        if len(kb) != m or len(ko) != m:
+            raise ValueError("one key_bits and key_order per array")
+        As = (ctypes.c_void_p * max(m, 1))(*[_ptr(a) for a in arrays])
+        ns = (ctypes.c_size_t * max(m, 1))(*[_nelem(a) for a in arrays])
+        bits = (ctypes.c_uint32 * max(m, 1))(*kb)
+        flags = (ctypes.c_uint32 * max(m, 1))(*[_key_flags(o) for o in ko])
+        ids = (ctypes.c_uint64 * max(m, 1))()
+        _check(lib().lsb_server_submit_batch(self._h, As, ns, bits, flags, m, ids),
+               "lsb_server_submit_batch")
+        return list(ids[:m])
 
     def poll(self, job):
         rc = lib().lsb_server_poll(self._h, job)

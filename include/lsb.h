@@ -216,6 +216,13 @@ int lsb_server_destroy(lsb_server* srv);
  * the budget. */
 int lsb_server_submit(lsb_server* srv, lsb_elem* A, size_t n, uint32_t key_bits,
                       uint32_t key_flags, uint64_t* job_id);
+/* Queues count jobs at once, in order, as count lsb_server_submit calls
+ * would (job i: As[i], ns[i], key_bits[i], key_flags[i]; its id in
+ * job_ids[i]) but under one lock, so the scheduler sees them together.  All
+ * or nothing: on any error no job is queued. */
+int lsb_server_submit_batch(lsb_server* srv, lsb_elem* const* As, const size_t* ns,
+                            const uint32_t* key_bits, const uint32_t* key_flags, size_t count,
+                            uint64_t* job_ids);
 /* 1 = done, 0 = not yet, negative = the job failed with that code. */
 int lsb_server_poll(lsb_server* srv, uint64_t job_id);
 /* Waits up to timeout_s (< 0: forever); LSB_EBUSY on timeout. */

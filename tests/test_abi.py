@@ -177,6 +177,7 @@ def test_server_rejects_bad_config_without_gpu():
     assert L.lsb_server_create(ctypes.byref(c), None) == lsb_amd.LSB_EINVAL
     jid = ctypes.c_uint64()
     assert L.lsb_server_submit(None, None, 5, 64, 0, ctypes.byref(jid)) == lsb_amd.LSB_EINVAL
+    assert L.lsb_server_submit_batch(None, None, None, None, None, 1, None) == lsb_amd.LSB_EINVAL
 
 
 def test_tune_table_drives_auto_choice(tmp_path):

@@ -72,6 +72,30 @@ def test_each_job_same_alone_and_in_the_mix():
         assert np.array_equal(got, alone(a, kb))
 
 
+def test_batch_submission():
+    """submit_many queues a whole mix under one lock: every job still comes
+    out as its lone sort; a bad entry (a misaligned array) refuses the whole
+    call and queues nothing."""
+    jobs = make_jobs(60, seed=3)
+    with lsb_amd.Server(streams=4) as srv:
+        bufs = [to_dev(a) for a, _ in jobs]
+        torch.cuda.synchronize()
+        ids = srv.submit_many(bufs, key_bits=[kb for _, kb in jobs])
+        assert len(ids) == len(jobs) and ids == sorted(ids)
+        srv.drain()
+        assert srv.stats()["jobs_done"] == len(jobs)
+        for jid in ids:
+            assert srv.poll(jid)
+        spare = torch.empty((1001, 2), dtype=torch.int64, device=DEV)
+        misaligned = spare.view(-1)[1:2001]  # 8-byte offset from a 16-byte element
+        with pytest.raises(lsb_amd.LsbError):
+            srv.submit_many([bufs[0], misaligned])
+        srv.drain()
+        assert srv.stats()["jobs_done"] == len(jobs)
+    for (a, kb), buf in zip(jobs, bufs):
+        assert np.array_equal(from_dev(buf), expected(a, kb))
+
+
 def test_key_orders_in_the_mix():
     """Jobs of different key orders are never batched together: signed,
     float and unsigned jobs of equal width submitted at once each come back
