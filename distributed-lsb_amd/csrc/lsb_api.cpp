@@ -79,44 +79,44 @@ std::vector<lsb_tune_entry> g_tune;
 // made by lsb_autotune or lsb_tune_load takes precedence; lsb_tune_clear
 // returns to this one.
 const lsb_tune_entry kBuiltinTune[] = {
-    {12, 64, 8, 2048, 0, 0, 0.0680f, 0.0732f},
-    {13, 64, 8, 2048, 0, 0, 0.0746f, 0.0802f},
-    {14, 64, 8, 2048, 0, 0, 0.0764f, 0.0873f},
-    {15, 64, 8, 2048, 0, 0, 0.0844f, 0.0961f},
-    {16, 64, 8, 2048, 0, 0, 0.0956f, 0.1034f},
-    {17, 64, 8, 2048, 0, 0, 0.1168f, 0.1177f},
-    {18, 64, 8, 2048, 0, 0, 0.1323f, 0.1325f},
-    {19, 64, 8, 4096, 0, 0, 0.1600f, 0.1647f},
-    {20, 64, 8, 8192, 3, 0, 0.1894f, 0.1918f},
-    {21, 64, 8, 8192, 3, 0, 0.2087f, 0.2261f},
-    {22, 64, 8, 8192, 3, 0, 0.3191f, 0.3413f},
-    {23, 64, 8, 8192, 3, 0, 0.5221f, 0.5950f},
-    {24, 64, 8, 8192, 3, 0, 1.0284f, 1.2199f},
-    {25, 64, 8, 8192, 3, 0, 1.8990f, 2.2532f},
-    {26, 64, 8, 8192, 3, 0, 3.6444f, 4.3267f},
-    {27, 64, 8, 8192, 3, 0, 7.1660f, 8.4423f},
-    {28, 64, 8, 8192, 3, 0, 14.3547f, 17.4303f},
-    {29, 64, 8, 8192, 3, 0, 29.5093f, 34.8872f},
-    {30, 64, 8, 8192, 3, 0, 60.7763f, 69.5869f},
-    {12, 32, 8, 2048, 0, 0, 0.0424f, 0.0427f},
-    {13, 32, 8, 2048, 0, 0, 0.0421f, 0.0458f},
-    {14, 32, 8, 2048, 0, 0, 0.0448f, 0.0496f},
-    {15, 32, 8, 2048, 0, 0, 0.0492f, 0.0562f},
-    {16, 32, 8, 2048, 0, 0, 0.0548f, 0.0574f},
-    {17, 32, 8, 2048, 0, 0, 0.0660f, 0.0667f},
-    {18, 32, 8, 2048, 3, 0, 0.0734f, 0.0744f},
-    {19, 32, 8, 2048, 3, 0, 0.0888f, 0.0893f},
-    {20, 32, 8, 8192, 3, 0, 0.1017f, 0.1043f},
-    {21, 32, 8, 8192, 3, 0, 0.1117f, 0.1200f},
-    {22, 32, 8, 8192, 3, 0, 0.1655f, 0.1769f},
-    {23, 32, 8, 8192, 3, 0, 0.2732f, 0.3023f},
-    {24, 32, 8, 8192, 3, 0, 0.5281f, 0.6223f},
-    {25, 32, 8, 8192, 3, 0, 0.9889f, 1.1742f},
-    {26, 32, 8, 8192, 3, 0, 1.9002f, 2.2507f},
-    {27, 32, 8, 8192, 3, 0, 3.6999f, 4.3865f},
-    {28, 32, 8, 8192, 3, 0, 7.4515f, 8.9221f},
-    {29, 32, 8, 8192, 3, 0, 15.1107f, 17.7994f},
-    {30, 32, 8, 8192, 3, 0, 30.6882f, 35.5001f},
+    {12, 64, 8, 2048, 3, 0, 0.0708f, 0.0708f},
+    {13, 64, 8, 2048, 0, 0, 0.0696f, 0.0696f},
+    {14, 64, 8, 2048, 3, 0, 0.0739f, 0.0758f},
+    {15, 64, 8, 2048, 0, 0, 0.0834f, 0.0966f},
+    {16, 64, 8, 2048, 3, 0, 0.0869f, 0.0945f},
+    {17, 64, 8, 2048, 3, 0, 0.0982f, 0.1098f},
+    {18, 64, 8, 2048, 3, 0, 0.1120f, 0.1230f},
+    {19, 64, 8, 4096, 3, 0, 0.1468f, 0.1485f},
+    {20, 64, 8, 4096, 3, 0, 0.1778f, 0.1809f},
+    {21, 64, 8, 8192, 3, 0, 0.1963f, 0.2224f},
+    {22, 64, 8, 8192, 3, 0, 0.3023f, 0.3309f},
+    {23, 64, 8, 8192, 3, 0, 0.5193f, 0.6012f},
+    {24, 64, 8, 8192, 3, 0, 1.0241f, 1.2055f},
+    {25, 64, 8, 8192, 3, 0, 1.9244f, 2.2476f},
+    {26, 64, 8, 8192, 3, 0, 3.6850f, 4.3229f},
+    {27, 64, 8, 8192, 3, 0, 7.2261f, 8.4240f},
+    {28, 64, 8, 8192, 3, 0, 14.4495f, 17.4637f},
+    {29, 64, 8, 8192, 3, 0, 29.0067f, 35.1728f},
+    {30, 64, 8, 8192, 3, 0, 58.6729f, 71.0898f},
+    {12, 32, 8, 2048, 0, 0, 0.0405f, 0.0415f},
+    {13, 32, 8, 2048, 3, 0, 0.0418f, 0.0423f},
+    {14, 32, 8, 2048, 3, 0, 0.0437f, 0.0440f},
+    {15, 32, 8, 2048, 0, 0, 0.0474f, 0.0546f},
+    {16, 32, 8, 2048, 3, 0, 0.0510f, 0.0536f},
+    {17, 32, 8, 2048, 3, 0, 0.0584f, 0.0648f},
+    {18, 32, 8, 2048, 3, 0, 0.0687f, 0.0722f},
+    {19, 32, 8, 4096, 3, 0, 0.0845f, 0.0850f},
+    {20, 32, 8, 8192, 3, 0, 0.0982f, 0.0987f},
+    {21, 32, 8, 8192, 3, 0, 0.1095f, 0.1192f},
+    {22, 32, 8, 8192, 3, 0, 0.1647f, 0.1737f},
+    {23, 32, 8, 8192, 3, 0, 0.2695f, 0.3015f},
+    {24, 32, 8, 8192, 3, 0, 0.5139f, 0.6119f},
+    {25, 32, 8, 8192, 3, 0, 0.9806f, 1.1642f},
+    {26, 32, 8, 8192, 3, 0, 1.8870f, 2.2409f},
+    {27, 32, 8, 8192, 3, 0, 3.6775f, 4.3568f},
+    {28, 32, 8, 8192, 3, 0, 7.3839f, 8.8683f},
+    {29, 32, 8, 8192, 3, 0, 14.8838f, 17.7564f},
+    {30, 32, 8, 8192, 3, 0, 29.8698f, 35.4811f},
 };
 
 uint32_t variant_id(uint32_t d, uint32_t tile, uint32_t persistent) {
