@@ -291,23 +291,31 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
                                                    const uint32_t* __restrict__ starts,
                                                    uint32_t shift, uint32_t mask,
                                                    const KeyOrder& ko,
-                                                   uint32_t* next_ticket = nullptr) {
+                                                   uint32_t* next_ticket = nullptr,
+                                                   uint32_t* own = nullptr) {
   using S = TileSmem<D, BLOCK, ITEMS>;
   constexpr int NB = S::NB, W = S::W, T = S::T;
   constexpr int DPT = (NB + BLOCK - 1) / BLOCK;  // digits per thread
   const uint32_t t = threadIdx.x;
   const uint32_t lane = t & (kWave - 1);
   const uint32_t wave = t / kWave;
-  uint32_t* whist = reinterpret_cast<uint32_t*>(sm.elems);
-
-  for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
+  // Per-wave counters: in `own` (W * NB words of their own) each wave clears
+  // its row and ranks at once -- since the last tile's staging barrier only
+  // that wave has touched the row; else in the staging buffer, cleared by
+  // all threads behind a barrier.
+  uint32_t* whist = own ? own : reinterpret_cast<uint32_t*>(sm.elems);
+  if (own) {
+    for (uint32_t i = lane; i < (uint32_t)NB; i += kWave) whist[wave * NB + i] = 0u;
+  } else {
+    for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
+  }
   for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) st_next[(size_t)tile * NB + b] = 0u;
 
   const uint32_t tile_base = tile * (uint32_t)T;
   const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
   const uint32_t wrun = wave * (kWave * ITEMS);
   const uint32_t lim = tile_n > wrun ? tile_n - wrun : 0u;
-  lds_barrier();
+  if (!own) lds_barrier();
 
   uint32_t dg[ITEMS], rk[ITEMS];
   uint32_t* wh = whist + wave * NB;
@@ -535,12 +543,13 @@ __device__ __forceinline__ void onesweep_tile(TileSmem<D, BLOCK, ITEMS>& sm,
                                               const uint32_t* __restrict__ starts,
                                               uint32_t shift, uint32_t mask,
                                               const KeyOrder& ko,
-                                              uint32_t* next_ticket = nullptr) {
+                                              uint32_t* next_ticket = nullptr,
+                                              uint32_t* own = nullptr) {
   elem_t e[ITEMS];
   tile_load<BLOCK, ITEMS>(e, src, n, tile);
   keys_fwd<ITEMS>(e, ko);
   onesweep_tile_regs<D, BLOCK, ITEMS>(sm, e, dst, n, tile, st_cur, st_next, starts, shift, mask,
-                                      ko, next_ticket);
+                                      ko, next_ticket, own);
 }
 
 // ----------------------------------------------------------- small_sort

@@ -194,6 +194,11 @@ __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(SegPassArgs args) {
   constexpr int NB = 1 << D;
   constexpr int MAXP = (64 + D - 1) / D;
   __shared__ TileSmem<D, BLOCK, ITEMS> sm;
+  // Blocks that fill a CU alone (512+ threads) keep their per-wave counters
+  // in LDS of their own, so a tile starts ranking without a barrier (8-bit
+  // digits: the 11-bit counters would not fit beside the tile).
+  constexpr bool kOwn = BLOCK >= 512 && D <= 8;
+  __shared__ uint32_t s_whist[kOwn ? (BLOCK / kWave) * NB : 1];
   const uint32_t total = args.hdr->total_tiles;
   const uint32_t pass = args.pass;
   const size_t slice = (size_t)args.max_tiles * NB;
@@ -219,7 +224,8 @@ __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(SegPassArgs args) {
           args.status + (size_t)((ord + 1u) & 1u) * slice + (size_t)L.tile_begin * NB;
       const uint32_t* starts = args.counts + (size_t)seg * MAXP * NB + (size_t)pass * NB;
       onesweep_tile<D, BLOCK, ITEMS>(sm, src, dst, L.n, tile - L.tile_begin, st_cur, st_next,
-                                     starts, args.shift, args.mask, args.ko, ticket);
+                                     starts, args.shift, args.mask, args.ko, ticket,
+                                     kOwn ? s_whist : nullptr);
     } else if (threadIdx.x == 0) {
       sm.scalar[3] = atomicAdd(ticket, 1u);  // this segment skips the pass
     }
@@ -264,6 +270,7 @@ static uint32_t seg_onesweep_occupancy() {
 }
 
 static const SegVariant kSegVariants[] = {
+    {8, 1024 * 8, 1024, &seg_onesweep_launch<8, 1024, 8>, &seg_onesweep_occupancy<8, 1024, 8>},
     {8, 256 * 16, 256, &seg_onesweep_launch<8, 256, 16>, &seg_onesweep_occupancy<8, 256, 16>},
     {8, 512 * 16, 512, &seg_onesweep_launch<8, 512, 16>, &seg_onesweep_occupancy<8, 512, 16>},
     {8, 256 * 8, 256, &seg_onesweep_launch<8, 256, 8>, &seg_onesweep_occupancy<8, 256, 8>},
