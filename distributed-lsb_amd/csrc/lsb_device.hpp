@@ -156,22 +156,44 @@ __device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d, bool valid) {
   }
 }
 
-// hist_add for one round of U keys per lane of a wave whose 64 lanes are all
-// active and valid: d[u] is key u's digit.  When all 64*U digits are equal
-// (sorted keys, zero high digits, heavy duplicates) one lane adds the whole
-// count -- a wave-uniform branch, tested once per round -- otherwise every
-// digit gets its own atomic.
-template <int U>
-__device__ __forceinline__ void hist_add_round(uint32_t* h, const uint32_t (&d)[U]) {
-  const uint32_t first = __builtin_amdgcn_readfirstlane(d[0]);
-  bool diff = false;
+// Counts digits 0 .. L-1 of one key per lane, all 64 lanes active and valid.
+// A digit every lane of the wave shares (zero high digits, sorted or
+// duplicate keys) is added once by lane 0 instead of by a 64-way
+// same-address atomic.
+template <int D, int L>
+__device__ __forceinline__ void count_row(uint32_t* h, uint64_t key, const uint32_t (&mask)[L]) {
+  constexpr int NB = 1 << D;
+  if constexpr (D == 8) {
+    // 8-bit digits never straddle the key's 32-bit halves: one field
+    // extract each; the digits lane 0 shares with everyone are read from
+    // the XOR with lane 0's key.
+    const uint32_t lo = (uint32_t)key, hi = (uint32_t)(key >> 32);
+    const uint32_t f_lo = __builtin_amdgcn_readfirstlane(lo);
+    const uint32_t f_hi = __builtin_amdgcn_readfirstlane(hi);
+    const uint32_t y_lo = lo ^ f_lo, y_hi = hi ^ f_hi;
 #pragma unroll
-  for (int u = 0; u < U; ++u) diff |= d[u] != first;
-  if (__ballot(diff) == 0) {
-    if (lane_id() == 0) atomicAdd(&h[first], (uint32_t)(kWave * U));
+    for (int p = 0; p < L; ++p) {
+      const uint32_t sh = (uint32_t)(p * 8) & 31u;
+      const uint32_t w = (uint32_t)__popc(mask[p]);
+      if (__ballot(__builtin_amdgcn_ubfe(p < 4 ? y_lo : y_hi, sh, w) != 0u) == 0) {
+        if (lane_id() == 0)
+          atomicAdd(&h[p * NB + __builtin_amdgcn_ubfe(p < 4 ? f_lo : f_hi, sh, w)],
+                    (uint32_t)kWave);
+      } else {
+        atomicAdd(&h[p * NB + __builtin_amdgcn_ubfe(p < 4 ? lo : hi, sh, w)], 1u);
+      }
+    }
   } else {
 #pragma unroll
-    for (int u = 0; u < U; ++u) atomicAdd(&h[d[u]], 1u);
+    for (int p = 0; p < L; ++p) {
+      const uint32_t d = digit_of(key, p * D, mask[p]);
+      const uint32_t first = __builtin_amdgcn_readfirstlane(d);
+      if (__ballot(d != first) == 0) {
+        if (lane_id() == 0) atomicAdd(&h[p * NB + first], (uint32_t)kWave);
+      } else {
+        atomicAdd(&h[p * NB + d], 1u);
+      }
+    }
   }
 }
 

@@ -49,47 +49,6 @@ void launch_zero(uint32_t* p, size_t words, hipStream_t s) {
 // themselves) the kernel also ORs every key's XOR with key 0 into orr[0..1]:
 // digit p is single-bucket exactly when its bits of that OR are zero.
 
-// Counts digits 0 .. L-1 of one key per lane, all 64 lanes active and valid.
-// A digit every lane of the wave shares (zero high digits, sorted or
-// duplicate keys) is added once by lane 0 instead of by a 64-way
-// same-address atomic.
-template <int D, int L>
-__device__ __forceinline__ void count_row(uint32_t* h, uint64_t key, const uint32_t (&mask)[L]) {
-  constexpr int NB = 1 << D;
-  if constexpr (D == 8) {
-    // 8-bit digits never straddle the key's 32-bit halves: one field
-    // extract each; the digits lane 0 shares with everyone are read from
-    // the XOR with lane 0's key.
-    const uint32_t lo = (uint32_t)key, hi = (uint32_t)(key >> 32);
-    const uint32_t f_lo = __builtin_amdgcn_readfirstlane(lo);
-    const uint32_t f_hi = __builtin_amdgcn_readfirstlane(hi);
-    const uint32_t y_lo = lo ^ f_lo, y_hi = hi ^ f_hi;
-#pragma unroll
-    for (int p = 0; p < L; ++p) {
-      const uint32_t sh = (uint32_t)(p * 8) & 31u;
-      const uint32_t w = (uint32_t)__popc(mask[p]);
-      if (__ballot(__builtin_amdgcn_ubfe(p < 4 ? y_lo : y_hi, sh, w) != 0u) == 0) {
-        if (lane_id() == 0)
-          atomicAdd(&h[p * NB + __builtin_amdgcn_ubfe(p < 4 ? f_lo : f_hi, sh, w)],
-                    (uint32_t)kWave);
-      } else {
-        atomicAdd(&h[p * NB + __builtin_amdgcn_ubfe(p < 4 ? lo : hi, sh, w)], 1u);
-      }
-    }
-  } else {
-#pragma unroll
-    for (int p = 0; p < L; ++p) {
-      const uint32_t d = digit_of(key, p * D, mask[p]);
-      const uint32_t first = __builtin_amdgcn_readfirstlane(d);
-      if (__ballot(d != first) == 0) {
-        if (lane_id() == 0) atomicAdd(&h[p * NB + first], (uint32_t)kWave);
-      } else {
-        atomicAdd(&h[p * NB + d], 1u);
-      }
-    }
-  }
-}
-
 template <int D, int P, int L>
 __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A, uint32_t n,
                                                    uint32_t key_bits, KeyOrder ko,

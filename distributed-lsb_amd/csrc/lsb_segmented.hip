@@ -23,18 +23,6 @@
 
 namespace lsb {
 
-// Binary search: the large segment holding global tile `tile`.
-__device__ __forceinline__ uint32_t find_segment(const SegLarge* __restrict__ large,
-                                                 uint32_t nlarge, uint32_t tile) {
-  uint32_t lo = 0, hi = nlarge - 1;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1) / 2;
-    if (large[mid].tile_begin <= tile) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
 // ----------------------------------------------------------- small segments
 template <int D, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_seg_small(const SegHeader* __restrict__ hdr,
@@ -76,21 +64,25 @@ __global__ __launch_bounds__(256) void k_seg_zero(const SegHeader* __restrict__ 
     counts[i] = 0u;
 }
 
-// Each block owns a contiguous range of tiles, accumulates the digit counts
-// of consecutive tiles of one segment in LDS and flushes them when the
-// segment changes.
-template <int D>
+// Each block owns a contiguous range of tiles, accumulates the counts of
+// digits 0 .. P-1 over consecutive tiles of one segment in LDS and flushes
+// them when the segment changes (tile_seg: the tile-to-segment map).  P is a
+// template parameter so the digit loop unrolls; full tiles are read in
+// unchecked rounds of 256 x 8 keys counted by count_row, a segment's last
+// tile key by key.
+template <int D, int P>
 __global__ __launch_bounds__(256) void k_seg_hist(const SegHeader* __restrict__ hdr,
                                                   const SegLarge* __restrict__ large,
                                                   uint32_t tile, uint32_t key_bits, KeyOrder ko,
-                                                  uint32_t passes, uint32_t* __restrict__ counts,
-                                                  uint32_t* __restrict__ status0) {
+                                                  uint32_t* __restrict__ counts,
+                                                  uint32_t* __restrict__ status0,
+                                                  const uint32_t* __restrict__ tile_seg) {
   constexpr int NB = 1 << D;
   constexpr int MAXP = (64 + D - 1) / D;
-  __shared__ uint32_t h[MAXP * NB];
+  constexpr int U = 8;
+  __shared__ uint32_t h[P * NB];
   const uint32_t t = threadIdx.x;
   const uint32_t total = hdr->total_tiles;
-  const uint32_t nlarge = hdr->n_large;
   const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
   const uint32_t t0 = blockIdx.x * per;
   const uint32_t t1 = t0 + per < total ? t0 + per : total;
@@ -102,47 +94,57 @@ __global__ __launch_bounds__(256) void k_seg_hist(const SegHeader* __restrict__ 
     const size_t z4 = (size_t)(t1 - t0) * NB / 4;
     for (size_t i = t; i < z4; i += 256) z[i] = make_uint4(0u, 0u, 0u, 0u);
   }
-  for (uint32_t i = t; i < passes * NB; i += 256) h[i] = 0u;
+  for (uint32_t i = t; i < (uint32_t)(P * NB); i += 256) h[i] = 0u;
+  uint32_t mask[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const uint32_t w = key_bits - p * D < (uint32_t)D ? key_bits - p * D : (uint32_t)D;
+    mask[p] = (1u << w) - 1u;
+  }
   __syncthreads();
 
-  uint32_t seg = find_segment(large, nlarge, t0);
+  uint32_t seg = tile_seg[t0];
   for (uint32_t tl = t0; tl < t1; ++tl) {
-    if (seg + 1 < nlarge && large[seg + 1].tile_begin <= tl) {
+    const uint32_t next = tile_seg[tl];
+    if (next != seg) {
       // flush the finished segment
       __syncthreads();
-      for (uint32_t i = t; i < passes * NB; i += 256) {
+      for (uint32_t i = t; i < (uint32_t)(P * NB); i += 256) {
         const uint32_t c = h[i];
         if (c) atomicAdd(&counts[(size_t)seg * MAXP * NB + i], c);
         h[i] = 0u;
       }
       __syncthreads();
-      seg = find_segment(large, nlarge, tl);
+      seg = next;
     }
     const SegLarge L = large[seg];
     const uint32_t first = (tl - L.tile_begin) * tile;
     const uint32_t cnt = L.n - first < tile ? L.n - first : tile;
     const elem_t* a = reinterpret_cast<const elem_t*>(L.a) + first;
-    for (uint32_t i0 = 0; i0 < cnt; i0 += 256 * 4) {
-      uint64_t k[4];
-      bool v[4];
+    if (cnt == tile && tile % (256 * U) == 0) {
+      for (uint32_t i0 = 0; i0 < cnt; i0 += 256 * U) {
+        uint64_t k[U];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t i = i0 + u * 256 + t;
-        v[u] = i < cnt;
-        k[u] = v[u] ? key_fwd(a[i].key, ko) : 0ull;
-      }
+        for (int u = 0; u < U; ++u) k[u] = a[i0 + u * 256 + t].key;
+        if (ko.sign) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        for (uint32_t p = 0; p < passes; ++p) {
-          const uint32_t shift = p * D;
-          const uint32_t w = key_bits - shift < (uint32_t)D ? key_bits - shift : (uint32_t)D;
-          hist_add(h + p * NB, digit_of(k[u], shift, (1u << w) - 1u), v[u]);
+          for (int u = 0; u < U; ++u) k[u] = key_fwd(k[u], ko);
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u) count_row<D, P>(h, k[u], mask);
+      }
+    } else {
+      for (uint32_t i0 = 0; i0 < cnt; i0 += 256) {
+        const uint32_t i = i0 + t;
+        const bool v = i < cnt;
+        const uint64_t k = v ? key_fwd(a[i].key, ko) : 0ull;
+#pragma unroll
+        for (int p = 0; p < P; ++p) hist_add(h + p * NB, digit_of(k, p * D, mask[p]), v);
       }
     }
   }
   __syncthreads();
-  for (uint32_t i = t; i < passes * NB; i += 256) {
+  for (uint32_t i = t; i < (uint32_t)(P * NB); i += 256) {
     const uint32_t c = h[i];
     if (c) atomicAdd(&counts[(size_t)seg * MAXP * NB + i], c);
   }
@@ -303,15 +305,33 @@ hipError_t launch_seg_prepare(uint32_t d, const SegHeader* hdr,
   if (d == 8) {
     constexpr int MAXP = 8;
     hipLaunchKernelGGL(k_seg_zero, dim3(grid), dim3(256), 0, s, hdr, counts, MAXP * 256u);
-    hipLaunchKernelGGL(k_seg_hist<8>, dim3(grid), dim3(256), 0, s, hdr, large, tile,
-                       key_bits, ko, passes, counts, status0);
+#define LSB_SEG_HIST(PP)                                                                   \
+  case PP:                                                                                 \
+    hipLaunchKernelGGL((k_seg_hist<8, PP>), dim3(grid), dim3(256), 0, s, hdr, large, tile, \
+                       key_bits, ko, counts, status0, tile_seg);                           \
+    break;
+    switch (passes) {
+      LSB_SEG_HIST(1) LSB_SEG_HIST(2) LSB_SEG_HIST(3) LSB_SEG_HIST(4)
+      LSB_SEG_HIST(5) LSB_SEG_HIST(6) LSB_SEG_HIST(7) LSB_SEG_HIST(8)
+      default: return hipErrorInvalidValue;
+    }
+#undef LSB_SEG_HIST
     hipLaunchKernelGGL(k_seg_scan<8>, dim3(grid), dim3(1024), 0, s, hdr, large, counts,
                        seg_plan, tickets, passes, flags);
   } else if (d == 11) {
     constexpr int MAXP = 6;
     hipLaunchKernelGGL(k_seg_zero, dim3(grid), dim3(256), 0, s, hdr, counts, MAXP * 2048u);
-    hipLaunchKernelGGL(k_seg_hist<11>, dim3(grid), dim3(256), 0, s, hdr, large, tile,
-                       key_bits, ko, passes, counts, status0);
+#define LSB_SEG_HIST(PP)                                                                    \
+  case PP:                                                                                  \
+    hipLaunchKernelGGL((k_seg_hist<11, PP>), dim3(grid), dim3(256), 0, s, hdr, large, tile, \
+                       key_bits, ko, counts, status0, tile_seg);                            \
+    break;
+    switch (passes) {
+      LSB_SEG_HIST(1) LSB_SEG_HIST(2) LSB_SEG_HIST(3)
+      LSB_SEG_HIST(4) LSB_SEG_HIST(5) LSB_SEG_HIST(6)
+      default: return hipErrorInvalidValue;
+    }
+#undef LSB_SEG_HIST
     hipLaunchKernelGGL(k_seg_scan<11>, dim3(grid), dim3(1024), 0, s, hdr, large, counts,
                        seg_plan, tickets, passes, flags);
   } else {
