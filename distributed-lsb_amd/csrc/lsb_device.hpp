@@ -71,13 +71,18 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
 // so callers may write a[] right before and read it right after.  With
 // ROWS > 0 the result is also added into ROWS rows of N words at `rows`
 // (rows[r*N + i] += scan[i]) before the closing barrier.
-template <int N, int BLOCK, int ROWS = 0>
+//
+// OWN: the caller wrote a[i] (and rows[r*N + i]) with thread i itself and
+// N <= BLOCK, so each thread reads back only what it wrote and the opening
+// barrier is skipped.
+template <int N, int BLOCK, int ROWS = 0, bool OWN = false>
 __device__ __forceinline__ void block_exclusive_scan(uint32_t* a, uint32_t* wave_sums,
                                                      uint32_t* rows = nullptr) {
   constexpr int PER = (N + BLOCK - 1) / BLOCK;
   constexpr int W = BLOCK / kWave;
+  static_assert(!OWN || PER == 1, "OWN needs one element per thread");
   const int t = threadIdx.x;
-  lds_barrier();
+  if (!OWN) lds_barrier();
   uint32_t v[PER];
   uint32_t s = 0;
 #pragma unroll

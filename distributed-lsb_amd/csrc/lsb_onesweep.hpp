@@ -355,7 +355,7 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
   }
   // The scan also adds each bucket's tile start into every wave's row, so
   // an element's slot is one LDS read: wh[digit] + rank.
-  block_exclusive_scan<NB, BLOCK, W>(sm.start, sm.wsum, whist);
+  block_exclusive_scan<NB, BLOCK, W, DPT == 1>(sm.start, sm.wsum, whist);
 
   uint32_t pos[ITEMS];
 #pragma unroll
@@ -472,7 +472,7 @@ __device__ __forceinline__ void onesweep_tile_region(
   }
   // The scan also adds each bucket's tile start into every wave's row, so
   // an element's slot is one LDS read: wh[digit] + rank.
-  block_exclusive_scan<NB, BLOCK, W>(sm.start, sm.wsum, whist);
+  block_exclusive_scan<NB, BLOCK, W, DPT == 1>(sm.start, sm.wsum, whist);
 
   uint32_t pos[ITEMS];
 #pragma unroll
