@@ -677,8 +677,9 @@ int lsb_sort(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg) {
     int dev = 0;
     LSB_HIP(hipGetDevice(&dev));
     const std::vector<uint64_t> key = {0u, (uint64_t)(uintptr_t)A, (uint64_t)(uintptr_t)B, n,
-                                       (uint64_t)(uintptr_t)ws, p.d, p.tile, p.key_bits,
-                                       (uint64_t)(c.flags & ~LSB_SYNC), (uint64_t)dev};
+                                       (uint64_t)(uintptr_t)ws, p.d, p.tile, p.v->persistent,
+                                       p.key_bits, (uint64_t)(c.flags & ~LSB_SYNC),
+                                       (uint64_t)dev};
     char* w = static_cast<char*>(ws);
     rc = launch_graph(key, s, [&](hipStream_t cs) {
       return enqueue_sort(A, B, (uint32_t)n, p, w, c.flags, cs);

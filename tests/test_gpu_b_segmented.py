@@ -185,6 +185,38 @@ def test_plain_sort_graph_replay():
         assert np.array_equal(from_dev(A), want), rep
 
 
+@pytest.mark.parametrize("variant", [0, 1])
+def test_plain_sort_graph_replay_changing_skips(variant):
+    """One captured sequence replayed over inputs whose single-bucket digits
+    differ (none, the low two, all but the top one, all): the plan, and for
+    the region form the digit counted by the extra histogram read, are
+    decided on the device at every replay."""
+    n = 250_000
+    base = O.gen_uniform(n, ranks=2)
+    inputs = []
+    inputs.append(base.copy())
+    low = base.copy()
+    low["key"] = (low["key"] << np.uint64(16)) | np.uint64(0x1234)
+    inputs.append(low)
+    top = base.copy()
+    top["key"] &= np.uint64(0xFF00000000000000)
+    inputs.append(top)
+    same = base.copy()
+    same["key"] = np.uint64(0xDEADBEEF)
+    inputs.append(same)
+    A = torch.empty((n, 2), dtype=torch.int64, device=DEV)
+    B = torch.empty_like(A)
+    for rep in range(2):
+        for i, a in enumerate(inputs):
+            A.copy_(to_dev(a))
+            lsb_amd.sort(A, B, graph=True, variant=variant)
+            torch.cuda.synchronize()
+            want = a[np.argsort(a["key"], kind="stable")]
+            assert np.array_equal(from_dev(A), want), (rep, i, variant)
+            x = torch.arange(1000, device=DEV).sum()  # eager work between replays
+            assert int(x) == 499500
+
+
 def test_segmented_rejects_bad_offsets():
     A = torch.empty((100, 2), dtype=torch.int64, device=DEV)
     B = torch.empty_like(A)
