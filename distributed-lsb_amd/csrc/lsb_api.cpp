@@ -370,11 +370,12 @@ int enqueue_sort(lsb_elem* A, lsb_elem* B, uint32_t n, const Plan& p, char* ws,
   LSB_HIP(lsb::launch_histogram(p.d, A, n, p.key_bits, p.ko, p.passes, counted, counts,
                                 p.region_tiles * p.tile, status, slice, orr, s));
   if ((rc = prof_mark(prof, 1, s))) return rc;
-  LSB_HIP(lsb::launch_scan_plan(p.d, counts, starts, plan, tickets, regnext, n, p.passes, flags,
-                                orr, p.key_bits, s));
   if (lean)
-    LSB_HIP(lsb::launch_hist_first(p.d, A, n, p.key_bits, p.ko, counts, p.region_tiles * p.tile,
-                                   plan, p.passes, counted, s));
+    LSB_HIP(lsb::launch_plan_first(p.d, A, n, p.key_bits, p.ko, counts, p.region_tiles * p.tile,
+                                   orr, plan, tickets, regnext, p.passes, counted, flags, s));
+  else
+    LSB_HIP(lsb::launch_scan_plan(p.d, counts, starts, plan, tickets, regnext, n, p.passes,
+                                  flags, s));
   if ((rc = prof_mark(prof, 2, s))) return rc;
   for (uint32_t q = 0; q < p.passes; ++q) {
     lsb::PassArgs a;

@@ -80,20 +80,19 @@ hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_
                             const KeyOrder& ko, uint32_t passes, uint32_t counted,
                             uint32_t* regcount, uint32_t region_elems, uint32_t* zero,
                             size_t zero_words, uint32_t* orr, hipStream_t s);
-// Lean form, after the plan: the per-region counts of the first executed
-// pass when it is not one of the `counted` digits (else every block returns
-// at once).
-hipError_t launch_hist_first(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
+// Lean form, after the histogram: skip detection from orr, the plan, the
+// cleared tickets[P][R] and regnext[P][R][2^d], and the per-region counts of
+// the first executed pass when it is not one of the `counted` digits (d = 8).
+hipError_t launch_plan_first(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
                              const KeyOrder& ko, uint32_t* regcount, uint32_t region_elems,
-                             const uint32_t* plan, uint32_t passes, uint32_t counted,
-                             hipStream_t s);
+                             const uint32_t* orr, uint32_t* plan, uint32_t* tickets,
+                             uint32_t* regnext, uint32_t passes, uint32_t counted,
+                             uint32_t flags, hipStream_t s);
 // Global counts (sum of regions), exclusive scans, skip detection, plan;
-// clears tickets[P][R] and regnext[P][R][2^d].  With orr (lean form) skip
-// detection reads orr and no starts are written.
+// clears tickets[P][R] and regnext[P][R][2^d] (all-digit histogram).
 hipError_t launch_scan_plan(uint32_t d, const uint32_t* regcount, uint32_t* starts,
                             uint32_t* plan, uint32_t* tickets, uint32_t* regnext, uint32_t n,
-                            uint32_t passes, uint32_t flags, const uint32_t* orr,
-                            uint32_t key_bits, hipStream_t s);
+                            uint32_t passes, uint32_t flags, hipStream_t s);
 // Whole sort of one array of n <= small_sort_capacity() elements inside one
 // workgroup (one HBM read and write per element).
 uint32_t small_sort_capacity();

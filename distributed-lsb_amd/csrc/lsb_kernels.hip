@@ -193,29 +193,60 @@ hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_
   return hipGetLastError();
 }
 
-// ----------------------------------------------------------- k_hist_first
-// Lean form only: when the digits counted up front are all single-bucket,
-// the first executed pass is a later one and its per-region counts come from
-// here (one more read of the keys).  Otherwise every block returns at once.
+// ----------------------------------------------------------- k_plan_first
+// Lean form, after the histogram: the pass plan and, when the digits counted
+// up front are all single-bucket, the first executed digit's per-region
+// counts (one more read of the keys).  Every block derives the plan itself
+// from the two OR words (digit p runs unless its bits of the OR are zero,
+// or always with LSB_NO_SKIP); block 0 writes it, all blocks clear the
+// tickets and the per-region counters the scatter passes fill, and the
+// extra count runs only when it is needed.  One launch does what k_scan_plan
+// and a separate count would.
 template <int D>
-__global__ __launch_bounds__(256) void k_hist_first(const elem_t* __restrict__ A, uint32_t n,
+__global__ __launch_bounds__(256) void k_plan_first(const elem_t* __restrict__ A, uint32_t n,
                                                     uint32_t key_bits, KeyOrder ko,
                                                     uint32_t* __restrict__ regcount,
                                                     uint32_t region_elems,
                                                     uint32_t blocks_per_region,
-                                                    const uint32_t* __restrict__ plan,
-                                                    uint32_t passes, uint32_t counted) {
+                                                    const uint32_t* __restrict__ orr,
+                                                    uint32_t* __restrict__ plan,
+                                                    uint32_t* __restrict__ tickets,
+                                                    uint32_t* __restrict__ regnext,
+                                                    uint32_t passes, uint32_t counted,
+                                                    uint32_t flags) {
   constexpr int NB = 1 << D;
   constexpr int U = 8;
-  uint32_t f = passes;
-  for (uint32_t p = 0; p < passes; ++p)
-    if (plan[4 * p]) {
-      f = p;
-      break;
-    }
-  if (f >= passes || f < counted) return;
-  __shared__ uint32_t h[NB];
   const uint32_t t = threadIdx.x;
+  const uint64_t x = (uint64_t)orr[0] | ((uint64_t)orr[1] << 32);
+  uint32_t f = passes;
+  for (uint32_t p = passes; p-- > 0;) {
+    const uint32_t shift = p * D;
+    const uint32_t w = key_bits - shift < (uint32_t)D ? key_bits - shift : (uint32_t)D;
+    if ((flags & 2u) || ((x >> shift) & ((1ull << w) - 1ull)) != 0) f = p;
+  }
+  if (blockIdx.x == 0 && t == 0) {
+    uint32_t ord = 0;
+    for (uint32_t p = 0; p < passes; ++p) {
+      const uint32_t shift = p * D;
+      const uint32_t w = key_bits - shift < (uint32_t)D ? key_bits - shift : (uint32_t)D;
+      const uint32_t run =
+          ((flags & 2u) || ((x >> shift) & ((1ull << w) - 1ull)) != 0) ? 1u : 0u;
+      plan[4 * p + 0] = run;
+      plan[4 * p + 1] = ord;
+      plan[4 * p + 2] = ord & 1u;
+      plan[4 * p + 3] = 0u;
+      ord += run;
+    }
+    plan[4 * passes] = ord & 1u;
+  }
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + t; i < (size_t)passes * kRegions * NB; i += stride)
+    regnext[i] = 0u;
+  for (size_t i = (size_t)blockIdx.x * 256 + t; i < (size_t)passes * kRegions; i += stride)
+    tickets[i] = 0u;
+  if (f >= passes || f < counted) return;
+
+  __shared__ uint32_t h[NB];
   for (uint32_t i = t; i < (uint32_t)NB; i += 256) h[i] = 0u;
   __syncthreads();
   const uint32_t shift = f * D;
@@ -258,51 +289,38 @@ __global__ __launch_bounds__(256) void k_hist_first(const elem_t* __restrict__ A
   }
 }
 
-hipError_t launch_hist_first(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
+hipError_t launch_plan_first(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
                              const KeyOrder& ko, uint32_t* regcount, uint32_t region_elems,
-                             const uint32_t* plan, uint32_t passes, uint32_t counted,
-                             hipStream_t s) {
+                             const uint32_t* orr, uint32_t* plan, uint32_t* tickets,
+                             uint32_t* regnext, uint32_t passes, uint32_t counted,
+                             uint32_t flags, hipStream_t s) {
   if (d != 8) return hipErrorInvalidValue;
   const uint32_t pr = hist_blocks_per_region(region_elems);
-  hipLaunchKernelGGL(k_hist_first<8>, dim3(pr * kRegions), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_plan_first<8>, dim3(pr * kRegions), dim3(256), 0, s,
                      static_cast<const elem_t*>(A), n, key_bits, ko, regcount, region_elems, pr,
-                     plan, passes, counted);
+                     orr, plan, tickets, regnext, passes, counted, flags);
   return hipGetLastError();
 }
 
 // ------------------------------------------------------------ k_scan_plan
 // Global count of bucket b = sum over regions; exclusive scan per digit;
 // single-bucket detection; plan; clears the tickets and the per-region
-// counters the scatter passes fill for their successors.  With orr (the
-// lean histogram) a digit is single-bucket when its bits of orr are zero,
-// and no starts are made: the region passes scan their own counts.
+// counters the scatter passes fill for their successors.  The lean form
+// (region passes) plans with k_plan_first instead.
 template <int NB>
 __global__ __launch_bounds__(1024) void k_scan_plan(const uint32_t* __restrict__ regcount,
                                                     uint32_t* __restrict__ starts,
                                                     uint32_t* __restrict__ plan,
                                                     uint32_t* __restrict__ tickets,
                                                     uint32_t* __restrict__ regnext, uint32_t n,
-                                                    uint32_t passes, uint32_t flags,
-                                                    const uint32_t* __restrict__ orr,
-                                                    uint32_t key_bits, uint32_t d) {
+                                                    uint32_t passes, uint32_t flags) {
   __shared__ uint32_t a[NB];
   __shared__ uint32_t wsum[16];
   __shared__ uint32_t trivial[16];
   const uint32_t t = threadIdx.x;
   for (uint32_t i = t; i < passes * kRegions * NB; i += 1024) regnext[i] = 0u;
   for (uint32_t i = t; i < passes * kRegions; i += 1024) tickets[i] = 0u;
-  if (orr) {
-    if (t == 0) {
-      const uint64_t x = (uint64_t)orr[0] | ((uint64_t)orr[1] << 32);
-      for (uint32_t p = 0; p < passes; ++p) {
-        const uint32_t shift = p * d;
-        const uint32_t w = key_bits - shift < d ? key_bits - shift : d;
-        trivial[p] = ((x >> shift) & ((1ull << w) - 1ull)) == 0 ? 1u : 0u;
-      }
-    }
-    __syncthreads();
-  }
-  for (uint32_t p = 0; p < passes && !orr; ++p) {
+  for (uint32_t p = 0; p < passes; ++p) {
     int triv = 0;
     for (uint32_t i = t; i < NB; i += 1024) {
       uint32_t c = 0;
@@ -333,14 +351,13 @@ __global__ __launch_bounds__(1024) void k_scan_plan(const uint32_t* __restrict__
 
 hipError_t launch_scan_plan(uint32_t d, const uint32_t* regcount, uint32_t* starts,
                             uint32_t* plan, uint32_t* tickets, uint32_t* regnext, uint32_t n,
-                            uint32_t passes, uint32_t flags, const uint32_t* orr,
-                            uint32_t key_bits, hipStream_t s) {
+                            uint32_t passes, uint32_t flags, hipStream_t s) {
   if (d == 8)
     hipLaunchKernelGGL(k_scan_plan<256>, dim3(1), dim3(1024), 0, s, regcount, starts, plan,
-                       tickets, regnext, n, passes, flags, orr, key_bits, d);
+                       tickets, regnext, n, passes, flags);
   else if (d == 11)
     hipLaunchKernelGGL(k_scan_plan<2048>, dim3(1), dim3(1024), 0, s, regcount, starts, plan,
-                       tickets, regnext, n, passes, flags, orr, key_bits, d);
+                       tickets, regnext, n, passes, flags);
   else
     return hipErrorInvalidValue;
   return hipSuccess;
