@@ -48,7 +48,6 @@ void launch_zero(uint32_t* p, size_t words, hipStream_t s) {
 // With orr given (L < P: the region passes count each later digit
 // themselves) the kernel also ORs every key's XOR with key 0 into orr[0..1]:
 // digit p is single-bucket exactly when its bits of that OR are zero.
-
 template <int D, int P, int L>
 __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A, uint32_t n,
                                                    uint32_t key_bits, KeyOrder ko,
