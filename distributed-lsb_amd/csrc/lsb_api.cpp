@@ -469,7 +469,7 @@ struct SegLayout {
   lsb::KeyOrder ko;
   size_t cap_n;
   uint32_t cap_segs, max_tiles;
-  size_t hdr, meta, counts, plan, tickets, status, tileseg, total;
+  size_t hdr, meta, counts, plan, tickets, status, tileseg, perm, total;
 };
 
 int make_seg_layout(size_t cap_n, uint32_t cap_segs, const lsb_config& c, SegLayout* L,
@@ -520,9 +520,10 @@ int make_seg_layout(size_t cap_n, uint32_t cap_segs, const lsb_config& c, SegLay
   L->meta = off;    off += align_up((size_t)L->cap_segs * sizeof(lsb::SegLarge));
   L->counts = off;  off += align_up((size_t)L->cap_segs * L->maxp * L->nb * 4);
   L->plan = off;    off += align_up((size_t)L->cap_segs * 8);
-  L->tickets = off; off += align_up(16 * 4);
+  L->tickets = off; off += align_up(8 * lsb::kRegions * 4);
   L->status = off;  off += align_up(2 * (size_t)L->max_tiles * L->nb * 4);
   L->tileseg = off; off += align_up((size_t)L->max_tiles * 4);
+  L->perm = off;    off += align_up((size_t)L->max_tiles * 4);
   L->total = off;
   return LSB_OK;
 }
@@ -564,17 +565,21 @@ int enqueue_segmented(const SegLayout& L, char* ws, uint32_t flags, hipStream_t 
   uint32_t* tickets = reinterpret_cast<uint32_t*>(ws + L.tickets);
   uint32_t* status = reinterpret_cast<uint32_t*>(ws + L.status);
   uint32_t* tileseg = reinterpret_cast<uint32_t*>(ws + L.tileseg);
+  uint32_t* perm = reinterpret_cast<uint32_t*>(ws + L.perm);
   const uint32_t cus = device_cus();
   LSB_HIP(lsb::launch_seg_prepare(L.d, hdr, large, L.tile, L.key_bits, L.ko, L.passes, counts,
-                                  plan, tickets, status, tileseg, flags, 4 * cus, s));
+                                  plan, tickets, status, tileseg, perm, flags, 4 * cus, s));
   LSB_HIP(lsb::launch_seg_small(hdr, small, L.key_bits, L.ko, 4 * cus, s));
-  const uint32_t grid = L.v->blocks_per_cu() * cus;
+  // whole groups of blocks (kRegions of them, each taking its own tickets)
+  uint32_t grid = L.v->blocks_per_cu() * cus;
+  grid = grid < (uint32_t)lsb::kRegions ? (uint32_t)lsb::kRegions : grid - grid % lsb::kRegions;
   for (uint32_t q = 0; q < L.passes; ++q) {
     lsb::SegPassArgs a;
     a.hdr = hdr;
     a.large = large;
     a.seg_plan = plan;
     a.tile_seg = tileseg;
+    a.perm = perm;
     a.counts = counts;
     a.tickets = tickets;
     a.status = status;
@@ -799,7 +804,7 @@ int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_c
     } else if (e.n <= lsb::seg_small_capacity()) {
       sml[is++] = lsb::SegSmall{a, (uint32_t)e.n, 0u};
     } else {
-      lg[il++] = lsb::SegLarge{a, b, (uint32_t)e.n, tiles};
+      lg[il++] = lsb::SegLarge{a, b, (uint32_t)e.n, tiles, 0u, 0u};
       tiles += (uint32_t)((e.n + L.tile - 1) / L.tile);
     }
   }
@@ -808,6 +813,35 @@ int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_c
   hdr->n_small = n_small;
   hdr->total_tiles = tiles;
   if (tiles > L.max_tiles) return LSB_ENOMEM;
+  // The scatter passes give each group of blocks whole segments: largest
+  // first onto the least-loaded group; inside a group the segments keep
+  // their order.
+  {
+    const auto ntl = [&](uint32_t i) { return (uint32_t)((lg[i].n + L.tile - 1) / L.tile); };
+    std::vector<uint32_t> order(n_large), group(n_large);
+    for (uint32_t i = 0; i < n_large; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return ntl(x) > ntl(y); });
+    uint32_t load[lsb::kRegions] = {};
+    for (uint32_t i : order) {
+      int g = 0;
+      for (int k = 1; k < lsb::kRegions; ++k)
+        if (load[k] < load[g]) g = k;
+      group[i] = (uint32_t)g;
+      load[g] += ntl(i);
+    }
+    uint32_t next[lsb::kRegions];
+    uint32_t off = 0;
+    for (int g = 0; g < lsb::kRegions; ++g) {
+      hdr->group_off[g] = next[g] = off;
+      hdr->group_tiles[g] = load[g];
+      off += load[g];
+    }
+    for (uint32_t i = 0; i < n_large; ++i) {
+      lg[i].perm_pos = next[group[i]];
+      next[group[i]] += ntl(i);
+    }
+  }
   LSB_HIP(hipMemcpyAsync(ws + L.hdr, h, meta_bytes, hipMemcpyHostToDevice, s));
   LSB_HIP(hipEventRecord(W.staged, s));
 

@@ -110,6 +110,11 @@ struct SegHeader {
   uint32_t n_tiny;       // segments <= seg_tiny_capacity()
   uint32_t n_small;      // segments in (tiny, seg_small_capacity()]
   uint32_t total_tiles;  // tiles over all large segments
+  // The scatter passes split the large segments, whole, over kRegions
+  // groups of blocks (one per XCD); group g takes the tiles
+  // perm[group_off[g] .. group_off[g] + group_tiles[g]) in ticket order.
+  uint32_t group_off[kRegions];
+  uint32_t group_tiles[kRegions];
 };
 // Segment tables hold device addresses, so one launch sequence serves
 // segments carved out of one array (lsb_sort_segmented) or scattered over
@@ -120,6 +125,8 @@ struct SegLarge {
   uint64_t b;           // its scratch of n elements
   uint32_t n;
   uint32_t tile_begin;  // global index of the segment's first tile
+  uint32_t perm_pos;    // where its tiles start in its group's tile order
+  uint32_t pad;
 };
 struct SegSmall {
   uint64_t a;
@@ -131,8 +138,9 @@ struct SegPassArgs {
   const SegLarge* large;
   const uint32_t* seg_plan;  // [2*s] executed-pass mask, [2*s+1] result-in-B flag
   const uint32_t* tile_seg;  // large segment of every global tile
+  const uint32_t* perm;      // global tiles in the groups' ticket order
   const uint32_t* counts;    // per segment [MAXP][2^d] starts (after the scan)
-  uint32_t* tickets;
+  uint32_t* tickets;         // [pass][group]
   uint32_t* status;          // two slices of max_tiles * 2^d words
   uint32_t max_tiles;
   uint32_t pass;
@@ -157,7 +165,7 @@ hipError_t launch_seg_prepare(uint32_t d, const SegHeader* hdr,
                               const SegLarge* large, uint32_t tile, uint32_t key_bits,
                               const KeyOrder& ko, uint32_t passes, uint32_t* counts, uint32_t* seg_plan,
                               uint32_t* tickets, uint32_t* status0, uint32_t* tile_seg,
-                              uint32_t flags, uint32_t grid, hipStream_t s);
+                              uint32_t* perm, uint32_t flags, uint32_t grid, hipStream_t s);
 hipError_t launch_seg_copy(const SegHeader* hdr, const SegLarge* large,
                            const uint32_t* seg_plan, const uint32_t* tile_seg, uint32_t tile,
                            uint32_t grid,

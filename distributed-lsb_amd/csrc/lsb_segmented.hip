@@ -12,9 +12,10 @@
 //   k_seg_hist      per-segment all-digit histograms over the tiles of the
 //                   large segments (+ clears look-back slice 0).
 //   k_seg_scan      per-segment exclusive scans, skip mask, pass parity.
-//   k_seg_onesweep  one launch per digit pass, persistent: blocks take tile
-//                   tickets across ALL large segments; look-back stays
-//                   inside a segment (its first tile publishes inclusive).
+//   k_seg_onesweep  one launch per digit pass, persistent: the large
+//                   segments are split whole over kRegions groups of blocks,
+//                   each with its own tile tickets; look-back stays inside a
+//                   segment (its first tile publishes inclusive).
 //   k_seg_copy      moves a segment from B to A when it ran an odd number
 //                   of passes.
 #include "lsb_device.hpp"
@@ -43,15 +44,21 @@ __global__ __launch_bounds__(BLOCK) void k_seg_small(const SegHeader* __restrict
 }
 
 // ------------------------------------------------------------ tile map
-// tile_seg[t] = the large segment that global tile t belongs to.
+// tile_seg[t] = the large segment that global tile t belongs to; perm lists
+// the global tiles in the groups' ticket order (each segment's tiles
+// consecutive and in order from its perm_pos).
 __global__ __launch_bounds__(256) void k_seg_tilemap(const SegHeader* __restrict__ hdr,
                                                      const SegLarge* __restrict__ large,
-                                                     uint32_t tile, uint32_t* __restrict__ tile_seg) {
+                                                     uint32_t tile, uint32_t* __restrict__ tile_seg,
+                                                     uint32_t* __restrict__ perm) {
   const uint32_t nl = hdr->n_large;
   for (uint32_t s = blockIdx.x; s < nl; s += gridDim.x) {
     const SegLarge L = large[s];
     const uint32_t nt = (L.n + tile - 1) / tile;
-    for (uint32_t i = threadIdx.x; i < nt; i += 256) tile_seg[L.tile_begin + i] = s;
+    for (uint32_t i = threadIdx.x; i < nt; i += 256) {
+      tile_seg[L.tile_begin + i] = s;
+      perm[L.perm_pos + i] = L.tile_begin + i;
+    }
   }
 }
 
@@ -164,7 +171,7 @@ __global__ __launch_bounds__(1024) void k_seg_scan(const SegHeader* __restrict__
   __shared__ uint32_t a[NB];
   __shared__ uint32_t wsum[16];
   const uint32_t t = threadIdx.x;
-  if (blockIdx.x == 0 && t < 16) tickets[t] = 0u;
+  if (blockIdx.x == 0 && t < 8 * kRegions) tickets[t] = 0u;
   const uint32_t nlarge = hdr->n_large;
   for (uint32_t s = blockIdx.x; s < nlarge; s += gridDim.x) {
     const uint32_t n = large[s].n;
@@ -201,17 +208,23 @@ __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(SegPassArgs args) {
   // digits: the 11-bit counters would not fit beside the tile).
   constexpr bool kOwn = BLOCK >= 512 && D <= 8;
   __shared__ uint32_t s_whist[kOwn ? (BLOCK / kWave) * NB : 1];
-  const uint32_t total = args.hdr->total_tiles;
   const uint32_t pass = args.pass;
   const size_t slice = (size_t)args.max_tiles * NB;
-  uint32_t* ticket = &args.tickets[pass];
+  // Group g = blockIdx % kRegions (one per XCD when blocks are dispatched
+  // round-robin) works through its own segments, so a segment's look-back
+  // chain and the partial lines between its tiles' runs stay in one L2.
+  const uint32_t group = blockIdx.x % kRegions;
+  const uint32_t g_off = args.hdr->group_off[group];
+  const uint32_t g_tiles = args.hdr->group_tiles[group];
+  uint32_t* ticket = &args.tickets[pass * kRegions + group];
   if (threadIdx.x == 0) sm.scalar[3] = atomicAdd(ticket, 1u);
   __syncthreads();
   while (true) {
     // The next tile comes from the previous tile's early ticket; its segment
     // from the tile map (uniform loads).
-    const uint32_t tile = sm.scalar[3];
-    if (tile >= total) return;
+    const uint32_t t = sm.scalar[3];
+    if (t >= g_tiles) return;
+    const uint32_t tile = args.perm[g_off + t];
     const uint32_t seg = args.tile_seg[tile];
     const uint32_t mask = args.seg_plan[2 * seg];
     if (mask & (1u << pass)) {
@@ -300,8 +313,9 @@ hipError_t launch_seg_prepare(uint32_t d, const SegHeader* hdr,
                               const SegLarge* large, uint32_t tile, uint32_t key_bits,
                               const KeyOrder& ko, uint32_t passes, uint32_t* counts, uint32_t* seg_plan,
                               uint32_t* tickets, uint32_t* status0, uint32_t* tile_seg,
-                              uint32_t flags, uint32_t grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_seg_tilemap, dim3(grid), dim3(256), 0, s, hdr, large, tile, tile_seg);
+                              uint32_t* perm, uint32_t flags, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_tilemap, dim3(grid), dim3(256), 0, s, hdr, large, tile, tile_seg,
+                     perm);
   if (d == 8) {
     constexpr int MAXP = 8;
     hipLaunchKernelGGL(k_seg_zero, dim3(grid), dim3(256), 0, s, hdr, counts, MAXP * 256u);
