@@ -815,7 +815,9 @@ int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_c
   if (tiles > L.max_tiles) return LSB_ENOMEM;
   // The scatter passes give each group of blocks whole segments: largest
   // first onto the least-loaded group; inside a group the segments keep
-  // their order.
+  // their order.  When that leaves one group with much more than its share
+  // (few segments, or one dominant one), all blocks share one group instead,
+  // as a group's blocks cannot help with another group's segments.
   {
     const auto ntl = [&](uint32_t i) { return (uint32_t)((lg[i].n + L.tile - 1) / L.tile); };
     std::vector<uint32_t> order(n_large), group(n_large);
@@ -830,16 +832,28 @@ int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_c
       group[i] = (uint32_t)g;
       load[g] += ntl(i);
     }
-    uint32_t next[lsb::kRegions];
-    uint32_t off = 0;
-    for (int g = 0; g < lsb::kRegions; ++g) {
-      hdr->group_off[g] = next[g] = off;
-      hdr->group_tiles[g] = load[g];
-      off += load[g];
-    }
-    for (uint32_t i = 0; i < n_large; ++i) {
-      lg[i].perm_pos = next[group[i]];
-      next[group[i]] += ntl(i);
+    uint32_t most = 0;
+    for (int g = 0; g < lsb::kRegions; ++g) most = std::max(most, load[g]);
+    // grouped only while the busiest group stays within 1/16 of an even split
+    const bool grouped = (uint64_t)most * lsb::kRegions * 16 <= (uint64_t)tiles * 17;
+    for (int g = 0; g < lsb::kRegions; ++g) hdr->group_off[g] = hdr->group_tiles[g] = 0;
+    if (grouped) {
+      hdr->groups = lsb::kRegions;
+      uint32_t next[lsb::kRegions];
+      uint32_t off = 0;
+      for (int g = 0; g < lsb::kRegions; ++g) {
+        hdr->group_off[g] = next[g] = off;
+        hdr->group_tiles[g] = load[g];
+        off += load[g];
+      }
+      for (uint32_t i = 0; i < n_large; ++i) {
+        lg[i].perm_pos = next[group[i]];
+        next[group[i]] += ntl(i);
+      }
+    } else {
+      hdr->groups = 1;
+      hdr->group_tiles[0] = tiles;
+      for (uint32_t i = 0; i < n_large; ++i) lg[i].perm_pos = lg[i].tile_begin;
     }
   }
   LSB_HIP(hipMemcpyAsync(ws + L.hdr, h, meta_bytes, hipMemcpyHostToDevice, s));

@@ -110,9 +110,11 @@ struct SegHeader {
   uint32_t n_tiny;       // segments <= seg_tiny_capacity()
   uint32_t n_small;      // segments in (tiny, seg_small_capacity()]
   uint32_t total_tiles;  // tiles over all large segments
-  // The scatter passes split the large segments, whole, over kRegions
-  // groups of blocks (one per XCD); group g takes the tiles
+  // The scatter passes split the large segments, whole, over `groups`
+  // groups of blocks (kRegions, one per XCD, when that balances; else 1);
+  // block b serves group b % groups, which takes the tiles
   // perm[group_off[g] .. group_off[g] + group_tiles[g]) in ticket order.
+  uint32_t groups;
   uint32_t group_off[kRegions];
   uint32_t group_tiles[kRegions];
 };

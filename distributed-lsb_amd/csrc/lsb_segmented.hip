@@ -210,10 +210,11 @@ __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(SegPassArgs args) {
   __shared__ uint32_t s_whist[kOwn ? (BLOCK / kWave) * NB : 1];
   const uint32_t pass = args.pass;
   const size_t slice = (size_t)args.max_tiles * NB;
-  // Group g = blockIdx % kRegions (one per XCD when blocks are dispatched
-  // round-robin) works through its own segments, so a segment's look-back
-  // chain and the partial lines between its tiles' runs stay in one L2.
-  const uint32_t group = blockIdx.x % kRegions;
+  // Group g = blockIdx % groups (with kRegions groups, one per XCD when
+  // blocks are dispatched round-robin) works through its own segments, so a
+  // segment's look-back chain and the partial lines between its tiles' runs
+  // stay in one L2.
+  const uint32_t group = blockIdx.x % args.hdr->groups;
   const uint32_t g_off = args.hdr->group_off[group];
   const uint32_t g_tiles = args.hdr->group_tiles[group];
   uint32_t* ticket = &args.tickets[pass * kRegions + group];

@@ -151,6 +151,28 @@ def test_segment_alone_equals_segment_in_batch():
         assert np.array_equal(from_dev(A1), batched[lo:hi])
 
 
+@pytest.mark.parametrize("shape", ["dominant", "even", "two"])
+def test_group_assignment_shapes(shape):
+    """The scatter passes split large segments over 8 groups of blocks only
+    when that balances; one dominant segment, two segments, or many even
+    ones all sort every segment exactly."""
+    rng = np.random.default_rng(5)
+    if shape == "dominant":
+        sizes = [600_000] + list(rng.integers(9000, 30000, size=20))
+    elif shape == "even":
+        sizes = [40_000] * 64
+    else:
+        sizes = [300_000, 250_000]
+    a, off = batch(sizes, seed=len(sizes))
+    n = int(off[-1])
+    A = to_dev(a)
+    B = torch.empty_like(A)
+    for graph in (False, True):
+        A.copy_(to_dev(a))
+        lsb_amd.sort_segmented(A, B, off, graph=graph, sync=True)
+        assert np.array_equal(from_dev(A[:n]), expected(a, off)), (shape, graph)
+
+
 def test_graph_replay_many_batches():
     """One captured graph per buffer set, replayed for batches of different
     shapes, with eager work and synchronisations between replays."""
