@@ -9,7 +9,8 @@
 //     batch sorted in place by one batched launch sequence (lsb_sort_batch,
 //     replayed from a captured hipGraph) with a per-stream scratch slab --
 //     no staging copies;
-//   * launches larger jobs as plain sorts with server-allocated scratch;
+//   * launches larger jobs as plain sorts, with a free slab as scratch when
+//     the job fits one (else scratch from the library's memory pool);
 //   * spreads launches over several streams so jobs that arrive while others
 //     run overlap on the device;
 //   * records each job's latency (completion observed - submission) for
@@ -64,7 +65,7 @@ struct Launch {
 struct StreamSlot {
   hipStream_t s = nullptr;
   lsb_elem* slab_b = nullptr;  // scratch of coalesced batches
-  hipEvent_t slab_free = nullptr;  // recorded after the last batch on this slot
+  hipEvent_t slab_free = nullptr;  // recorded after the last job that used the slab
   int inflight = 0;
 };
 
@@ -127,7 +128,13 @@ int lsb_server::pick_slot(bool need_slab) {
 }
 
 int lsb_server::launch_plain(Job& j) {
-  const int si = pick_slot(false);
+  // Scratch: a free slab when the job fits one, else the memory pool.  A job
+  // that fits a slab waits for one (LSB_EBUSY, nothing changed) rather than
+  // allocate: fresh device allocations of gigabytes have taken from 10 ms to
+  // over half a second and stall the scheduler meanwhile.
+  const bool use_slab = j.n > 1 && j.n <= cfg.batch_max_total;
+  int si = pick_slot(use_slab);
+  if (si < 0) return LSB_EBUSY;
   StreamSlot& sl = slots[si];
   Launch L;
   L.stream = si;
@@ -135,7 +142,10 @@ int lsb_server::launch_plain(Job& j) {
   L.jobs.push_back(j.id);
   int rc = LSB_OK;
   if (j.n > 1) {
-    rc = hip_rc(lsb::pool_alloc(&L.scratch, j.n * sizeof(lsb_elem), sl.s));
+    if (use_slab)
+      L.scratch = sl.slab_b;
+    else
+      rc = hip_rc(lsb::pool_alloc(&L.scratch, j.n * sizeof(lsb_elem), sl.s));
     if (rc == LSB_OK) {
       lsb_config c;
       lsb_config_init(&c);
@@ -143,7 +153,13 @@ int lsb_server::launch_plain(Job& j) {
       c.flags = j.key_flags;
       c.stream = sl.s;
       rc = lsb_sort(j.A, static_cast<lsb_elem*>(L.scratch), j.n, &c);
-      (void)hipFreeAsync(L.scratch, sl.s);
+      if (use_slab) {
+        // the slab is busy until this sort is done
+        const int er = hip_rc(hipEventRecord(sl.slab_free, sl.s));
+        if (rc == LSB_OK) rc = er;
+      } else {
+        (void)hipFreeAsync(L.scratch, sl.s);
+      }
       L.scratch = nullptr;
     }
   }
@@ -267,10 +283,14 @@ bool lsb_server::admit() {
       b.push_back(j.id);
       tot += j.n;
     } else {
-      pending.pop_front();
       used += charge;
       j.state = 1;
-      (void)launch_plain(j);
+      if (launch_plain(j) == LSB_EBUSY) {  // every slab busy: retry later
+        used -= charge;
+        j.state = 0;
+        break;
+      }
+      pending.pop_front();
     }
     any = true;
   }
@@ -292,7 +312,7 @@ bool lsb_server::flush_batches() {
                      t - batch_opened[kv.first] >= cfg.batch_window_us * 1e-6;
     if (!due) continue;
     if (b.size() == 1) {
-      (void)launch_plain(jobs[b[0]]);
+      if (launch_plain(jobs[b[0]]) == LSB_EBUSY) continue;
     } else if (launch_batch(kv.first, b) == LSB_EBUSY) {
       continue;
     }
