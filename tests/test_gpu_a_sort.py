@@ -288,6 +288,41 @@ def test_full_2p30_uniform():
     torch.cuda.empty_cache()
 
 
+def test_max_n_uniform():
+    """The largest n the ABI accepts (LSB_MAX_N = 2^31 - 2, 32 GiB per
+    buffer): the last tile is partial and ends one short of 2^31."""
+    n = lsb_amd.LSB_MAX_N
+    A = torch.empty((n, 2), dtype=torch.int64, device=DEV)
+    lsb_amd.generate(A, kind=lsb_amd.GEN_UNIFORM, ranks=256)
+    keys = torch.empty(n, dtype=torch.int64, device=DEV)
+    lsb_amd.extract_keys(A, keys)
+    B = torch.empty_like(A)
+    lsb_amd.sort(A, B, sync=True)
+    del B
+    assert _device_check(A, keys, n) == (0, -1)
+    del A, keys
+    torch.cuda.empty_cache()
+
+
+def test_max_n_one_bucket_no_skip():
+    """LSB_MAX_N equal keys, every pass run (LSB_NO_SKIP) by the one-tile
+    form, whose look-back prefixes are global: the last tile publishes the
+    largest status word there is, (n + 1) | inclusive flag = 0xFFFFFFFF."""
+    n = lsb_amd.LSB_MAX_N
+    v = [v for v, d, t, pf in variants() if d == 8 and pf == 0 and t == 4096][0]
+    A = torch.empty((n, 2), dtype=torch.int64, device=DEV)
+    lsb_amd.generate(A, kind=lsb_amd.GEN_UNIFORM, ranks=256)  # val = index
+    A[:, 0].fill_(0x5A5A5A5A5A5A5A5A)
+    keys = torch.empty(n, dtype=torch.int64, device=DEV)
+    lsb_amd.extract_keys(A, keys)
+    B = torch.empty_like(A)
+    lsb_amd.sort(A, B, sync=True, no_skip=True, variant=v)
+    del B
+    assert _device_check(A, keys, n) == (0, -1)
+    del A, keys
+    torch.cuda.empty_cache()
+
+
 def test_autotune_table_and_sorts_after_tuning():
     """The autotuner times every variant per class; sorts that follow use the
     tuned choice and stay bit-exact."""

@@ -276,3 +276,33 @@ def test_c4_batched_4096_segments_full_scale():
         assert np.array_equal(from_dev(A[lo:lo + n]), want)
     del A, keys
     torch.cuda.empty_cache()
+
+
+def test_max_size_segment_between_small_ones():
+    """A segment of LSB_MAX_N elements between a one-block segment and a
+    multi-tile one: 2^31 + 74998 elements in all, so the last segment starts
+    past element 2^31."""
+    sizes = [5000, lsb_amd.LSB_MAX_N, 70000]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    total = int(off[-1])
+    A = torch.empty((total, 2), dtype=torch.int64, device=DEV)
+    for s, n in enumerate(sizes):
+        lo = int(off[s])
+        lsb_amd.generate(A[lo:lo + n], kind=lsb_amd.GEN_UNIFORM, ranks=64 if s == 1 else 1,
+                         seed_base=7000 + s, val_base=0)
+    keys = torch.empty(total, dtype=torch.int64, device=DEV)
+    lsb_amd.extract_keys(A, keys)
+    B = torch.empty_like(A)
+    lsb_amd.sort_segmented(A, B, off, sync=True)
+    del B
+    for s, n in enumerate(sizes):
+        lo = int(off[s])
+        bad, first = lsb_amd.verify(A[lo:lo + n], keys[lo:lo + n], n=n, val_base=0)
+        assert bad == 0, (s, n, first)
+    for s in (0, 2):
+        lo, n = int(off[s]), sizes[s]
+        want = O.gen_uniform(n, ranks=1, seed_base=7000 + s)
+        want = want[np.argsort(want["key"], kind="stable")]
+        assert np.array_equal(from_dev(A[lo:lo + n]), want)
+    del A, keys
+    torch.cuda.empty_cache()
