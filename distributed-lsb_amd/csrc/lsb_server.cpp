@@ -71,6 +71,10 @@ struct StreamSlot {
 
 }  // namespace
 
+// Finished jobs (done or failed) kept for lsb_server_poll / wait / latency,
+// and latencies kept for the percentiles: the most recent kRetain of each.
+constexpr size_t kRetain = size_t(1) << 18;
+
 struct lsb_server {
   lsb_server_config cfg;
   std::mutex mu;
@@ -90,8 +94,10 @@ struct lsb_server {
   std::thread worker;
   int device = 0;
   int error = 0;
+  std::deque<uint64_t> finished;  // finished job ids, oldest first
   // stats
-  std::vector<double> lat_us;
+  std::vector<double> lat_us;  // ring of the latest kRetain latencies
+  size_t lat_next = 0;
   uint64_t done_jobs = 0, done_elems = 0, batches = 0, batched_jobs = 0, plain_jobs = 0;
   double first_submit = 0, last_done = 0;
 
@@ -102,6 +108,8 @@ struct lsb_server {
   int launch_plain(Job& j);
   int launch_batch(uint32_t key_class, std::vector<uint64_t>& ids);
   int pick_slot(bool need_slab);
+  void retire(uint64_t id);
+  void record_latency(double us);
 };
 
 namespace {
@@ -113,6 +121,23 @@ int hip_rc(hipError_t e) {
 }
 
 }  // namespace
+
+void lsb_server::retire(uint64_t id) {
+  finished.push_back(id);
+  if (finished.size() > kRetain) {
+    jobs.erase(finished.front());
+    finished.pop_front();
+  }
+}
+
+void lsb_server::record_latency(double us) {
+  if (lat_us.size() < kRetain) {
+    lat_us.push_back(us);
+  } else {
+    lat_us[lat_next] = us;
+    lat_next = (lat_next + 1) % kRetain;
+  }
+}
 
 int lsb_server::pick_slot(bool need_slab) {
   // Least-loaded stream; a batch also needs the slot's scratch slab free.
@@ -170,6 +195,7 @@ int lsb_server::launch_plain(Job& j) {
     j.done_at = now_s();
     used -= L.charge;
     if (L.ev) (void)hipEventDestroy(L.ev);
+    retire(j.id);
     return rc;
   }
   ++sl.inflight;
@@ -217,6 +243,7 @@ int lsb_server::launch_batch(uint32_t key_class, std::vector<uint64_t>& ids) {
     }
     used -= L.charge;
     if (L.ev) (void)hipEventDestroy(L.ev);
+    for (uint64_t id : ids) retire(id);
     return rc;
   }
   ++sl.inflight;
@@ -243,12 +270,13 @@ bool lsb_server::poll_completions() {
       j.state = rc == LSB_OK ? 2 : rc;
       j.done_at = t;
       if (rc == LSB_OK) {
-        lat_us.push_back((t - j.submitted) * 1e6);
+        record_latency((t - j.submitted) * 1e6);
         ++done_jobs;
         done_elems += j.n;
         last_done = std::max(last_done, t);
       }
     }
+    for (uint64_t id : L.jobs) retire(id);
     used -= L.charge;
     --slots[L.stream].inflight;
     (void)hipEventDestroy(L.ev);
@@ -415,16 +443,6 @@ int lsb_server_create(const lsb_server_config* cfg, lsb_server** out) {
   lsb_server* s = new lsb_server();
   s->cfg = c;
   int rc = hip_rc(hipGetDevice(&s->device));
-  if (!rc) {
-    // Keep freed scratch in the stream-ordered pool instead of returning it
-    // to the driver after every job.
-    hipMemPool_t pool = nullptr;
-    if (hipDeviceGetDefaultMemPool(&pool, s->device) == hipSuccess && pool) {
-      uint64_t keep = ~0ull;
-      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-    }
-    (void)hipGetLastError();
-  }
   s->slots.resize(c.streams);
   for (auto& sl : s->slots) {
     if (rc) break;
@@ -559,12 +577,11 @@ int lsb_server_reset_stats(lsb_server* s) {
   if (!s) return LSB_EINVAL;
   std::lock_guard<std::mutex> lock(s->mu);
   s->lat_us.clear();
+  s->lat_next = 0;
   s->done_jobs = s->done_elems = s->batches = s->batched_jobs = s->plain_jobs = 0;
   s->first_submit = s->last_done = 0;
-  for (auto it = s->jobs.begin(); it != s->jobs.end();) {
-    if (it->second.state == 2 || it->second.state < 0) it = s->jobs.erase(it);
-    else ++it;
-  }
+  for (uint64_t id : s->finished) s->jobs.erase(id);
+  s->finished.clear();
   return LSB_OK;
 }
 

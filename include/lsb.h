@@ -223,13 +223,17 @@ int lsb_server_submit(lsb_server* srv, lsb_elem* A, size_t n, uint32_t key_bits,
 int lsb_server_submit_batch(lsb_server* srv, lsb_elem* const* As, const size_t* ns,
                             const uint32_t* key_bits, const uint32_t* key_flags, size_t count,
                             uint64_t* job_ids);
-/* 1 = done, 0 = not yet, negative = the job failed with that code. */
+/* 1 = done, 0 = not yet, negative = the job failed with that code.  A
+ * finished job stays known until 2^18 later jobs have finished or the
+ * statistics are reset; after that LSB_ENOTFOUND. */
 int lsb_server_poll(lsb_server* srv, uint64_t job_id);
 /* Waits up to timeout_s (< 0: forever); LSB_EBUSY on timeout. */
 int lsb_server_wait(lsb_server* srv, uint64_t job_id, double timeout_s);
 /* Waits until every submitted job has completed. */
 int lsb_server_drain(lsb_server* srv);
 int lsb_server_latency(lsb_server* srv, uint64_t job_id, double* us);
+/* Counts since the last reset; the latency figures cover the most recent
+ * 2^18 completed jobs. */
 int lsb_server_stats_get(lsb_server* srv, lsb_server_stats* out);
 /* Clears the statistics and forgets completed jobs. */
 int lsb_server_reset_stats(lsb_server* srv);

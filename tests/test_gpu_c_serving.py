@@ -174,3 +174,44 @@ def test_concurrent_submitters():
             t.join()
     for (a, kb), b in zip(jobs, bufs):
         assert np.array_equal(from_dev(b), expected(a, kb))
+
+
+def test_finished_jobs_retained_up_to_limit():
+    """A long-running server keeps the most recent 2^18 finished jobs (and
+    latencies) only: a job that finished 2^18 + 1000 jobs ago is forgotten
+    (LSB_ENOTFOUND) while the latest ones still answer poll / latency."""
+    import ctypes
+    retain = 1 << 18
+    a0 = O.gen_uniform(1000, ranks=1, seed_base=77)
+    buf0 = to_dev(a0)
+    one = torch.zeros((1, 2), dtype=torch.int64, device=DEV)
+    torch.cuda.synchronize()
+    L = lsb_amd.lib()
+    with lsb_amd.Server(streams=2, use_graphs=False) as srv:
+        first = srv.submit(buf0)
+        srv.drain()
+        assert srv.poll(first) and srv.latency_us(first) > 0
+        total, chunk, last = retain + 1000, 32768, []
+        done = 0
+        while done < total:
+            m = min(chunk, total - done)
+            As = (ctypes.c_void_p * m)(*([one.data_ptr()] * m))
+            ns = (ctypes.c_size_t * m)(*([1] * m))
+            bits = (ctypes.c_uint32 * m)(*([64] * m))
+            flags = (ctypes.c_uint32 * m)()
+            ids = (ctypes.c_uint64 * m)()
+            assert L.lsb_server_submit_batch(srv._h, As, ns, bits, flags, m, ids) == 0
+            srv.drain()
+            last = list(ids[max(0, m - 16):m])
+            done += m
+        with pytest.raises(lsb_amd.LsbError) as err:
+            srv.poll(first)
+        assert err.value.code == lsb_amd.LSB_ENOTFOUND
+        for jid in last:
+            assert srv.poll(jid) and srv.latency_us(jid) >= 0
+        st = srv.stats()
+        assert st["jobs_done"] == total + 1 and st["p99_us"] >= st["p50_us"] > 0
+        srv.reset_stats()
+        with pytest.raises(lsb_amd.LsbError):
+            srv.poll(last[-1])
+    assert np.array_equal(from_dev(buf0), expected(a0, 64))
