@@ -252,9 +252,12 @@ namespace lsb {
 // for reuse instead of returning it at every synchronisation, and being the
 // library's own pool it leaves the device's default pool as the caller set
 // it.
+static std::mutex g_pool_mu;
+static std::map<int, hipMemPool_t> g_pools;
+
 hipError_t pool_alloc(void** p, size_t bytes, hipStream_t s) {
-  static std::mutex mu;
-  static std::map<int, hipMemPool_t> pools;
+  std::mutex& mu = g_pool_mu;
+  std::map<int, hipMemPool_t>& pools = g_pools;
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
@@ -282,6 +285,13 @@ hipError_t pool_alloc(void** p, size_t bytes, hipStream_t s) {
     }
   }
   return hipMallocFromPoolAsync(p, bytes, pool, s);
+}
+
+// Returns the pool's unused memory on device dev to the device.
+hipError_t pool_trim(int dev) {
+  std::lock_guard<std::mutex> lock(g_pool_mu);
+  auto it = g_pools.find(dev);
+  return it == g_pools.end() ? hipSuccess : hipMemPoolTrimTo(it->second, 0);
 }
 
 }  // namespace lsb
@@ -927,6 +937,62 @@ int lsb_sort_batch(lsb_elem* const* As, lsb_elem* const* Bs, const uint64_t* ns,
     total_n += n;
   }
   return sort_entries(ent, total_n, c);
+}
+
+int lsb_release(void* stream) {
+  int dev = 0;
+  LSB_HIP(hipGetDevice(&dev));
+  const hipStream_t s = (hipStream_t)stream;
+  LSB_HIP(hipStreamSynchronize(s));  // nothing queued on it may still use them
+  const auto key = std::make_pair(dev, stream);
+  std::vector<uint64_t> freed;
+  int rc = LSB_OK;
+  auto note = [&](hipError_t e) {
+    if (e != hipSuccess && rc == LSB_OK) {
+      g_last_hip = (int)e;
+      rc = LSB_EHIP;
+    }
+  };
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  auto w = g_ws.find(key);
+  if (w != g_ws.end()) {
+    if (w->second.ptr) {
+      freed.push_back((uint64_t)(uintptr_t)w->second.ptr);
+      note(hipFreeAsync(w->second.ptr, s));
+    }
+    g_ws.erase(w);
+  }
+  auto sw = g_seg_ws.find(key);
+  if (sw != g_seg_ws.end()) {
+    SegWorkspace& W = sw->second;
+    if (W.ptr) {
+      freed.push_back((uint64_t)(uintptr_t)W.ptr);
+      note(hipFreeAsync(W.ptr, s));
+    }
+    if (W.staged) note(hipEventDestroy(W.staged));
+    if (W.stage) note(hipHostFree(W.stage));
+    g_seg_ws.erase(sw);
+  }
+  if (!freed.empty()) {
+    // Graphs captured against a freed workspace (its address is part of
+    // the key: entry 4 of a plain sort's key, entry 1 of a segmented one's)
+    // were only ever launched on this stream.
+    std::lock_guard<std::mutex> glock(g_graph_mu);
+    for (auto it = g_graphs.begin(); it != g_graphs.end();) {
+      const std::vector<uint64_t>& k = it->first;
+      const uint64_t ws = k[0] == 0u ? k[4] : k[1];
+      if (std::find(freed.begin(), freed.end(), ws) != freed.end()) {
+        note(hipGraphExecDestroy(it->second));
+        it = g_graphs.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  note(hipStreamSynchronize(s));
+  note(lsb::pool_trim(dev));
+  if (rc != LSB_OK) (void)hipGetLastError();
+  return rc;
 }
 
 // ------------------------------------------------------------- autotuner

@@ -65,6 +65,8 @@ const SortVariant* sort_variants(int* count);
 // current device (freed with hipFreeAsync); the pool keeps freed memory for
 // reuse (lsb_api.cpp).
 hipError_t pool_alloc(void** p, size_t bytes, hipStream_t s);
+// Returns the unused memory of device dev's library pool to the device.
+hipError_t pool_trim(int dev);
 
 // Zeroes p[0..words), launched before the histogram.
 void launch_zero(uint32_t* p, size_t words, hipStream_t s);

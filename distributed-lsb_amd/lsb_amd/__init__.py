@@ -107,6 +107,7 @@ SIGNATURES = {
     "lsb_sort_segmented": (_I, [_P, _P, ctypes.POINTER(_U64), _U32, ctypes.POINTER(Config)]),
     "lsb_segmented_workspace_size": (_SZ, [_SZ, _U32, ctypes.POINTER(Config)]),
     "lsb_sort_batch": (_I, [_P, _P, _P, _U32, ctypes.POINTER(Config)]),
+    "lsb_release": (_I, [_P]),
     "lsb_server_config_init": (None, [ctypes.POINTER(ServerConfig)]),
     "lsb_server_create": (_I, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
     "lsb_server_destroy": (_I, [_P]),
@@ -275,6 +276,13 @@ def sort_segmented(A, B, offsets, key_bits=64, digit_bits=0, tile=0, stream=None
     _check(lib().lsb_sort_segmented(_ptr(A), _ptr(B),
                                     off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                                     max(nseg, 0), ctypes.byref(c)), "lsb_sort_segmented")
+
+
+def release(stream=None):
+    """Frees the library's cached workspaces and graphs for sorts on
+    `stream` (default: torch's current stream) after its queued work, and
+    returns the library pool's unused memory to the device."""
+    _check(lib().lsb_release(_stream_ptr(stream)), "lsb_release")
 
 
 def sort_batch(As, Bs, ns=None, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,

@@ -171,6 +171,17 @@ size_t lsb_segmented_workspace_size(size_t total_n, uint32_t nseg, const lsb_con
 int lsb_sort_batch(lsb_elem* const* As, lsb_elem* const* Bs, const uint64_t* ns, uint32_t nseg,
                    const lsb_config* cfg);
 
+/* Frees what the library keeps for sorts on `stream` (a hipStream_t; NULL
+ * = the default stream) of the current device: the workspaces it allocated
+ * for lsb_sort / lsb_sort_segmented / lsb_sort_batch calls made without a
+ * caller workspace, and the graphs captured against them; then returns the
+ * unused memory of the library's pool to the device.  Waits for the work
+ * queued on `stream` first.  Call it before destroying a stream that sorts
+ * ran on, with no other thread sorting on that stream; later sorts on it
+ * allocate again.  No counterpart in the reference, whose temporaries are
+ * per-call vectors (mpi/mpi_lsbsort.cpp:523-528, 550-555). */
+int lsb_release(void* stream);
+
 /* ---------------------------------------------------------------- serving */
 
 /* A sort service over jobs resident in HBM.  Jobs are admitted in arrival

@@ -217,6 +217,50 @@ def test_rejects_host_pointer():
         lsb_amd.sort(host.ctypes.data + (16 - host.ctypes.data % 16) % 16, B.data_ptr(), n=n)
 
 
+def test_release_frees_stream_state_and_sorts_again():
+    """lsb_release gives back a stream's cached workspaces (plain and
+    segmented) and the graphs captured against them; sorts on that stream
+    afterwards, graph-replayed ones included, allocate and capture again and
+    stay bit-exact."""
+    n = 1 << 26
+    a = O.gen_uniform(1 << 20, ranks=2, seed_base=31)
+    want = stable(a)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        big = torch.empty((n, 2), dtype=torch.int64, device=DEV)
+        lsb_amd.generate(big, kind=lsb_amd.GEN_UNIFORM, ranks=4, stream=s)
+        big_b = torch.empty_like(big)
+        lsb_amd.sort(big, big_b, stream=s)  # a workspace of about 20 MiB
+        for graph in (False, True, True):
+            A = to_dev(a)
+            B = torch.empty_like(A)
+            lsb_amd.sort(A, B, stream=s, graph=graph)
+            s.synchronize()
+            assert np.array_equal(from_dev(A), want), graph
+        off = [0, 3000, 300000, len(a)]
+        A = to_dev(a)
+        B = torch.empty_like(A)
+        lsb_amd.sort_segmented(A, B, off, stream=s, sync=True)
+        got = from_dev(A)
+        for lo, hi in zip(off[:-1], off[1:]):
+            assert np.array_equal(got[lo:hi], stable(a[lo:hi]))
+        s.synchronize()
+        del big, big_b
+        torch.cuda.empty_cache()
+        free0 = torch.cuda.mem_get_info()[0]
+        lsb_amd.release(s)
+        free1 = torch.cuda.mem_get_info()[0]
+        assert free1 - free0 >= 8 << 20, (free0, free1)
+        for graph in (True, False):
+            A = to_dev(a)
+            B = torch.empty_like(A)
+            lsb_amd.sort(A, B, stream=s, graph=graph)
+            s.synchronize()
+            assert np.array_equal(from_dev(A), want), graph
+        lsb_amd.release(s)
+    lsb_amd.release(torch.cuda.Stream())  # a stream the library never saw
+
+
 def test_repeated_sorts_reuse_workspace():
     """Back-to-back sorts of different sizes on one stream share the cached
     workspace; stale look-back state must never leak between them."""
