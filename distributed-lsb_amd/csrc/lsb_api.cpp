@@ -552,15 +552,16 @@ struct SegWorkspace {
 std::map<std::pair<int, void*>, SegWorkspace> g_seg_ws;
 
 uint32_t device_cus() {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
+  static const int cus = [] {
+    int dev = 0, c = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        c <= 0) {
       (void)hipGetLastError();
-      cus = 256;
+      c = 256;
     }
-  }
+    return c;
+  }();
   return (uint32_t)cus;
 }
 

@@ -150,8 +150,7 @@ static void launch_hist(const elem_t* a, uint32_t n, uint32_t key_bits, const Ke
 // 2^30-key histogram on 256 CUs takes 2.82 ms with two long-running blocks
 // per CU, 2.90 ms with four and 3.01 ms with eight (one per CU: 2.80 ms).
 static uint32_t hist_blocks_per_region(uint32_t region_elems) {
-  static uint32_t cap = 0;
-  if (!cap) {
+  static const uint32_t cap = [] {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -159,9 +158,9 @@ static uint32_t hist_blocks_per_region(uint32_t region_elems) {
       (void)hipGetLastError();
       cus = 256;
     }
-    cap = (uint32_t)(2 * cus / kRegions);
-    if (cap < 1) cap = 1;
-  }
+    const uint32_t c = (uint32_t)(2 * cus / kRegions);
+    return c < 1 ? 1u : c;
+  }();
   size_t per_region = ((size_t)region_elems + 256 * 8 * 8 - 1) / (256 * 8 * 8);
   if (per_region < 1) per_region = 1;
   if (per_region > cap) per_region = cap;
@@ -514,8 +513,8 @@ __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(Pass
 
 template <int D, int BLOCK, int ITEMS>
 static void launch_onesweep_rg(const PassArgs& a, hipStream_t s) {
-  static uint32_t per_region = 0;
-  if (!per_region) {
+  // resident blocks per region, found once (thread-safe static init)
+  static const uint32_t per_region = [] {
     int dev = 0, cus = 0, occ = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -525,9 +524,9 @@ static void launch_onesweep_rg(const PassArgs& a, hipStream_t s) {
       cus = 256;
       occ = 1;
     }
-    per_region = (uint32_t)((cus * (occ > 0 ? occ : 1)) / kRegions);
-    if (per_region < 1) per_region = 1;
-  }
+    const uint32_t pr = (uint32_t)((cus * (occ > 0 ? occ : 1)) / kRegions);
+    return pr < 1 ? 1u : pr;
+  }();
   const uint32_t pr = a.region_tiles < per_region ? a.region_tiles : per_region;
   hipLaunchKernelGGL((k_onesweep_rg<D, BLOCK, ITEMS>), dim3(pr * kRegions), dim3(BLOCK), 0, s, a);
 }

@@ -261,6 +261,63 @@ def test_release_frees_stream_state_and_sorts_again():
     lsb_amd.release(torch.cuda.Stream())  # a stream the library never saw
 
 
+def test_concurrent_host_threads():
+    """lsb_sort is reentrant across streams: four host threads, each on its
+    own stream, run plain (eager and graph-replayed), segmented and batched
+    sorts at the same time (the binding releases the GIL in every call), and
+    every result is the stable sort."""
+    import threading
+    cases = []
+    for tid in range(4):
+        plain = [O.gen_uniform(n, ranks=2, seed_base=500 + 10 * tid + k)
+                 for k, n in enumerate((5000, 200_000, (1 << 20) + 3))]
+        seg = O.gen_zipf(300_000, ranks=2, m=1 << 10, seed_base=600 + tid)
+        off = [0, 7, 40_000, 41_000, 300_000]
+        batch = [O.gen_uniform(n, ranks=1, seed_base=700 + 10 * tid + k)
+                 for k, n in enumerate((3000, 90_000, 8192))]
+        cases.append((plain, seg, off, batch))
+    streams = [torch.cuda.Stream() for _ in range(4)]
+    errors = []
+
+    def work(tid):
+        plain, seg, off, batch = cases[tid]
+        s = streams[tid]
+        try:
+            with torch.cuda.stream(s):
+                for rep in range(3):
+                    for a in plain:
+                        A = to_dev(a)
+                        B = torch.empty_like(A)
+                        lsb_amd.sort(A, B, stream=s, graph=rep == 2)
+                        s.synchronize()
+                        if not np.array_equal(from_dev(A), stable(a)):
+                            errors.append(("plain", tid, rep, len(a)))
+                    A = to_dev(seg)
+                    B = torch.empty_like(A)
+                    lsb_amd.sort_segmented(A, B, off, stream=s, sync=True, graph=rep == 2)
+                    got = from_dev(A)
+                    for lo, hi in zip(off[:-1], off[1:]):
+                        if not np.array_equal(got[lo:hi], stable(seg[lo:hi])):
+                            errors.append(("segmented", tid, rep, lo))
+                    As = [to_dev(x) for x in batch]
+                    Bs = [torch.empty_like(x) if x.shape[0] > 8192 else None for x in As]
+                    lsb_amd.sort_batch(As, Bs, stream=s, sync=True, graph=rep == 2)
+                    for x, X in zip(batch, As):
+                        if not np.array_equal(from_dev(X), stable(x)):
+                            errors.append(("batch", tid, rep, len(x)))
+        except Exception as e:  # reported below, not lost in the thread
+            errors.append(("raised", tid, repr(e)))
+
+    threads = [threading.Thread(target=work, args=(t,)) for t in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert errors == []
+    for s in streams:
+        lsb_amd.release(s)
+
+
 def test_repeated_sorts_reuse_workspace():
     """Back-to-back sorts of different sizes on one stream share the cached
     workspace; stale look-back state must never leak between them."""
