@@ -180,7 +180,7 @@ def test_constant_low_digits(low_bits):
             assert list(executed) == [0] * skipped + [1] * (8 - skipped), (low_bits, v, executed)
 
 
-@pytest.mark.parametrize("key_bits", [1, 8, 16, 21, 32, 33, 40])
+@pytest.mark.parametrize("key_bits", [1, 8, 16, 21, 32, 33, 40, 57, 63])
 def test_key_bits_and_per_pass_prefix(key_bits):
     """Sorting by the low key_bits is what the first passes leave behind:
     after pass p the array is stably sorted by the low (p+1)d bits."""
