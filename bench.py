@@ -175,6 +175,21 @@ def main():
         if not verified:
             log(f"VERIFY FAILED: {bad} bad positions, first {first}")
 
+    # Context for the roofline: the HIP runtime's own device-to-device copy of
+    # the same array (16 B read + 16 B written per element, as one scatter
+    # pass), timed with events on the stream it runs on; mean of 3.
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    copy_ms = []
+    for i in range(4):
+        e0.record(stream)
+        A.copy_(pristine)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if i:
+            copy_ms.append(e0.elapsed_time(e1))
+    copy_avg = sum(copy_ms) / len(copy_ms)
+    copy_gbs = 32 * n / (copy_avg * 1e-3) / 1e9
+
     traffic = traffic_from_profile(args.log2n, d, tile)
 
     value = n / (ms_wall * 1e-3) / 1e6
@@ -203,6 +218,10 @@ def main():
         "sort_roofline": {"model_bytes": model_bytes, "achieved": round(sort_gbs, 1),
                           "frac": round(sort_gbs / HBM_PEAK_GBS, 4),
                           "ms_event": round(ms_event, 4), "histogram_ms": round(hist_ms, 4)},
+        "copy_ceiling": {"achieved": round(copy_gbs, 1), "unit": "GB/s", "ms": round(copy_avg, 4),
+                         "frac": round(copy_gbs / HBM_PEAK_GBS, 4),
+                         "what": f"device-to-device copy of the 2^{args.log2n}-element input "
+                                 "(torch copy_, 32 B moved per element), for context"},
         "verified_bit_exact": verified,
         "cpu_baseline": base,
     }
