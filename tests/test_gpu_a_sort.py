@@ -511,3 +511,42 @@ def test_key_order_verifier_and_skip():
     U = to_dev(a)
     lsb_amd.sort(U, B, sync=True)
     assert lsb_amd.verify(U, keys, n=n, key_order="signed")[0] > 0
+
+
+def test_randomized_configurations():
+    """Forty seeded draws of size (small, mid, or within two of a multiple of
+    the tile), key width, key order, distribution, variant (or the automatic
+    choice) and pass skipping, each bit-exact against the oracle's stable
+    sort in that key order."""
+    rng = np.random.default_rng(20261203)
+    vs = variants()
+    for case in range(40):
+        sizes = [int(rng.integers(2, 70_000)), int(rng.integers(70_000, 400_000)),
+                 8192 * int(rng.integers(1, 40)) + int(rng.integers(-2, 3))]
+        n = sizes[int(rng.integers(0, 3))]
+        order = ["unsigned", "unsigned", "signed", "float"][int(rng.integers(0, 4))]
+        key_bits = [32, 64][int(rng.integers(0, 2))] if order == "float" else int(rng.integers(1, 65))
+        dist = ["uniform", "zipf", "few", "ordered"][int(rng.integers(0, 4))]
+        seed = int(rng.integers(0, 1 << 30))
+        if order != "unsigned":
+            n = max(n, 16)  # the float input plants three special values
+        if order == "signed":
+            a = _signed_input(n, key_bits, seed)
+        elif order == "float":
+            a = _float_input(n, key_bits, seed)
+        elif dist == "uniform":
+            a = O.gen_uniform(n, ranks=3, seed_base=seed)
+        elif dist == "zipf":
+            a = O.gen_zipf(n, ranks=2, m=1 << 12, seed_base=seed)
+        elif dist == "few":
+            r2 = np.random.default_rng(seed)
+            a = np.zeros(n, dtype=O.ELEM)
+            a["key"] = r2.integers(0, 5, n).astype(np.uint64) << np.uint64(int(r2.integers(0, 60)))
+            a["val"] = np.arange(n, dtype=np.uint64)
+        else:
+            a = O.gen_ordered(n, reverse=bool(seed & 1))
+        v = vs[int(rng.integers(0, len(vs)))][0] if rng.random() < 0.7 else 0
+        no_skip = bool(rng.random() < 0.3)
+        want = O.stable_sort_order(a, key_bits, order)
+        got = gpu_sort(a, key_bits=key_bits, key_order=order, variant=v, no_skip=no_skip)
+        assert np.array_equal(got, want), (case, n, order, key_bits, dist, v, no_skip)
