@@ -306,3 +306,48 @@ def test_max_size_segment_between_small_ones():
         assert np.array_equal(from_dev(A[lo:lo + n]), want)
     del A, keys
     torch.cuda.empty_cache()
+
+
+def test_randomized_batches():
+    """Sixteen seeded batches: 1-150 segments of log-uniform sizes from 0 to
+    2^18 (empty, single, LDS-sized and multi-tile ones mixed), random key
+    width and key order, through lsb_sort_segmented (one buffer) or
+    lsb_sort_batch (separate buffers), eager or graph-replayed; every segment
+    equals the oracle's stable sort of that segment in that order."""
+    rng = np.random.default_rng(777)
+    for case in range(16):
+        nseg = int(rng.integers(1, 151))
+        sizes = [0 if rng.random() < 0.05 else int(2 ** rng.uniform(0, 18)) for _ in range(nseg)]
+        order = ["unsigned", "unsigned", "signed", "float"][int(rng.integers(0, 4))]
+        key_bits = [32, 64][int(rng.integers(0, 2))] if order == "float" else int(rng.integers(1, 65))
+        graph = bool(rng.random() < 0.5)
+        segs = []
+        for s, n in enumerate(sizes):
+            a = O.gen_uniform(n, ranks=1, seed_base=10_000 * case + s, key_bits=64)
+            if n and rng.random() < 0.3:
+                a["key"] = a["key"] % np.uint64(7)  # heavy duplicates: stability
+            if order == "float" and key_bits == 32:
+                a["key"] &= np.uint64(0xFFFFFFFF)
+            segs.append(a)
+        wants = [O.stable_sort_order(a, key_bits, order) for a in segs]
+        if rng.random() < 0.5:
+            off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+            total = int(off[-1])
+            A = to_dev(np.concatenate(segs)) if total else torch.zeros((1, 2), dtype=torch.int64,
+                                                                        device=DEV)
+            B = torch.empty_like(A)
+            lsb_amd.sort_segmented(A, B, off, key_bits=key_bits, key_order=order, sync=True,
+                                   graph=graph)
+            got = from_dev(A)[:total]
+            for s in range(nseg):
+                lo, hi = int(off[s]), int(off[s + 1])
+                assert np.array_equal(got[lo:hi], wants[s]), (case, "segmented", s, hi - lo)
+        else:
+            As = [to_dev(a) if len(a) else torch.zeros((1, 2), dtype=torch.int64, device=DEV)
+                  for a in segs]
+            Bs = [torch.empty_like(x) if n > 8192 else None for x, n in zip(As, sizes)]
+            lsb_amd.sort_batch(As, Bs, ns=sizes, key_bits=key_bits, key_order=order, sync=True,
+                               graph=graph)
+            for s in range(nseg):
+                got = from_dev(As[s])[:sizes[s]]
+                assert np.array_equal(got, wants[s]), (case, "batch", s, sizes[s])
