@@ -40,58 +40,215 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sample_n, max_cores, reps):
-    """Reference mpi_lsbsort on the host (kind=reference), else the oracle port."""
+def host_info():
+    """Host CPU and memory facts for the baseline legs (lscpu, /proc/meminfo)."""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        f = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        get = lambda k: f.get(k, "").strip()
+        info["model"] = get("Model name")
+        sockets = int(get("Socket(s)") or 0)
+        cores = int(get("Core(s) per socket") or 0)
+        info.update({"sockets": sockets, "cores_per_socket": cores,
+                     "threads_per_core": int(get("Thread(s) per core") or 0),
+                     "physical_cores": sockets * cores})
+    except Exception as e:
+        log("lscpu failed:", e)
+    try:
+        mem = dict(line.split(":", 1) for line in open("/proc/meminfo"))
+        info["mem_available_gib"] = round(int(mem["MemAvailable"].split()[0]) / 2 ** 20, 1)
+    except Exception:
+        pass
+    return info
+
+
+# At most this many ranks: a GPU job's share of the host's CPUs on the
+# pooled boxes this runs on (the rest belong to other jobs), whatever nproc
+# says.
+CPU_SHARE = 16
+
+
+def _ref_leg(mpirun, ref, n, ranks, reps):
+    rates, secs = [], []
+    for r in range(reps):
+        log(f"  reference mpi_lsbsort: n=2^{n.bit_length() - 1}, {ranks} ranks, rep {r + 1}/{reps}")
+        p = subprocess.run([mpirun, "-n", str(ranks), "-bind-to", "core", ref, "--n", str(n),
+                            "--no-verify"], capture_output=True, text=True, timeout=300)
+        m = re.search(r"That's ([0-9.eE+]+) M elements sorted / s", p.stdout)
+        t = re.search(r"Sorted \d+ values in ([0-9.eE+]+)", p.stdout)
+        if p.returncode != 0 or not m:
+            raise RuntimeError(p.stdout[-500:] + p.stderr[-500:])
+        rates.append(float(m.group(1)))
+        if t:
+            secs.append(float(t.group(1)))
+    rates.sort()
+    secs.sort()
+    return {"n": n, "ranks": ranks, "reps": reps, "melem_s": rates[len(rates) // 2],
+            "sort_s": secs[len(secs) // 2] if secs else None, "all_melem_s": rates}
+
+
+def cpu_baseline(reps=5, full=False):
+    """The reference's own mpi_lsbsort (oracle/_ref, built unmodified from its
+    sources with -DNDEBUG -O3) under mpirun -bind-to core on this host, before
+    the GPU is touched: 4 ranks at 2^24 (the oracle config), all usable
+    physical cores at 2^24 and at 2^28 (when 24 GiB of RAM are free; 2^30 with
+    --cpu-full and 96 GiB free), median of `reps` each.  `value` is the
+    largest leg.  If the reference cannot run, the oracle's single-thread C
+    LSD sort (a port) is timed instead."""
     ref = os.path.join(ROOT, "oracle", "_ref", "mpi_lsbsort")
     mpirun = shutil.which("mpirun") or "/opt/conda/bin/mpirun"
-    cores = max(1, min(max_cores, len(os.sched_getaffinity(0))))
+    info = host_info()
+    phys = info.get("physical_cores") or info["affinity_cpus"]
+    ranks = max(1, min(phys, info["affinity_cpus"], CPU_SHARE))
     if os.path.exists(ref) and os.path.exists(mpirun):
-        rates = []
         try:
             probe = subprocess.run([mpirun, "-n", "2", ref, "--n", "1000", "--no-verify"],
                                    capture_output=True, text=True, timeout=60)
             if probe.returncode != 0:
                 raise RuntimeError("mpirun probe failed: " + probe.stderr[-300:])
-            for r in range(reps):
-                log(f"  reference mpi_lsbsort rep {r + 1}/{reps}: n={sample_n}, {cores} ranks")
-                p = subprocess.run([mpirun, "-n", str(cores), ref, "--n", str(sample_n),
-                                    "--no-verify"], capture_output=True, text=True, timeout=150)
-                m = re.search(r"That's ([0-9.eE+]+) M elements sorted / s", p.stdout)
-                if p.returncode != 0 or not m:
-                    raise RuntimeError(p.stdout[-500:] + p.stderr[-500:])
-                rates.append(float(m.group(1)))
-            rates.sort()
-            return {"value": rates[len(rates) // 2], "unit": "Melem/s", "cores": cores,
+            legs = [_ref_leg(mpirun, ref, 1 << 24, 4, reps),
+                    _ref_leg(mpirun, ref, 1 << 24, ranks, reps)]
+            free = info.get("mem_available_gib", 0)
+            if free >= 24:
+                legs.append(_ref_leg(mpirun, ref, 1 << 28, ranks, reps))
+            if full and free >= 96:
+                legs.append(_ref_leg(mpirun, ref, 1 << 30, ranks, 3))
+            top = legs[-1]
+            return {"value": top["melem_s"], "unit": "Melem/s", "cores": top["ranks"],
                     "kind": "reference",
-                    "sample": f"mpi_lsbsort --n {sample_n} --no-verify, mpirun -n {cores}, "
-                              f"median of {reps} (reference keygen pcg64(rank), 64-bit keys)"}
+                    "sample": f"mpi_lsbsort --n 2^{top['n'].bit_length() - 1} --no-verify, "
+                              f"mpirun -n {top['ranks']} -bind-to core, median of "
+                              f"{top['reps']} (reference keygen pcg64(rank), 64-bit keys); "
+                              f"ranks = min(physical cores, usable CPUs, {CPU_SHARE}-CPU share)",
+                    "legs": legs, "host": info}
         except Exception as e:  # fall through to the port
             log("reference CPU baseline failed:", e)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
     import oracle as O
-    n = min(sample_n, 1 << 24)
+    n = 1 << 24
     a = O.gen_uniform(n, ranks=4)
     t = time.perf_counter()
     O.lsd_sort(a, 64, 16)
     dt = time.perf_counter() - t
     return {"value": n / dt / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
-            "sample": f"oracle C LSD sort (16-bit digits, one thread), n={n}"}
+            "sample": f"oracle C LSD sort (16-bit digits, one thread), n={n}", "host": info}
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_onesweep_rg_2p30_uniform_traffic.md")
+
+
+# The sources the plain sort's kernels are compiled from, and the build file.
+PLAIN_SORT_SOURCES = ["csrc/lsb_kernels.hip", "csrc/lsb_onesweep.hpp", "csrc/lsb_device.hpp",
+                      "csrc/lsb_internal.hpp", "Makefile"]
+
+
+def source_digest():
+    """SHA-256 over the sources of the plain sort's kernels and the build
+    file: a committed counter summary is used only for the code it was taken
+    on."""
+    import hashlib
+    h = hashlib.sha256()
+    pkg = os.path.join(ROOT, "distributed-lsb_amd")
+    for rel in PLAIN_SORT_SOURCES:
+        h.update(rel.encode())
+        h.update(open(os.path.join(pkg, rel), "rb").read())
+    return h.hexdigest()[:16]
 
 
 def traffic_from_profile(log2n, d, tile):
     """HBM bytes per scatter launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary, when it was taken on this workload and layout."""
-    path = os.path.join(ROOT, "profiles", "k_onesweep_rg_2p30_uniform_traffic.md")
+    WRITE_SIZE summary, when it was taken on this workload, this layout and
+    these exact sources; else None."""
     try:
-        text = open(path).read()
+        text = open(TRAFFIC_PROFILE).read()
     except OSError:
         return None
     m = re.search(r"^traffic_bytes_per_launch: (\d+)$", text, re.M)
     c = re.search(r"^traffic_config: log2n=(\d+) digit_bits=(\d+) tile=(\d+)$", text, re.M)
+    h = re.search(r"^traffic_sources: ([0-9a-f]+)$", text, re.M)
     if not m or not c or tuple(map(int, c.groups())) != (log2n, d, tile):
         return None
+    if not h or h.group(1) != source_digest():
+        log("traffic profile was taken on other sources: traffic reported as null")
+        return None
     return int(m.group(1))
+
+
+def _counter_rows(csv_dir):
+    """(kernel name, dispatch id, counter value) rows of every rocprofv3
+    counter_collection.csv under csv_dir, summed per dispatch."""
+    import csv
+    per = {}
+    for root, _, files in os.walk(csv_dir):
+        for f in files:
+            if not f.endswith("counter_collection.csv"):
+                continue
+            for row in csv.DictReader(open(os.path.join(root, f))):
+                key = (row["Kernel_Name"], int(row["Dispatch_Id"]))
+                per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+    return per
+
+
+def traffic_summary(fetch_dir, write_dir, log2n, d, tile, cmd_fetch, cmd_write):
+    """Writes the traffic profile from two rocprofv3 --pmc runs of
+    `bench.py --traffic-probe` (FETCH_SIZE, then WRITE_SIZE, KiB per
+    dispatch).  Scale factors come from the probe's plain device copy of the
+    same 2^log2n-element array (16 B read + 16 B written per element)."""
+    fetch, write = _counter_rows(fetch_dir), _counter_rows(write_dir)
+    n = 1 << log2n
+    kib = 16 * n / 1024.0
+
+    def pick(per, pat):
+        return [v for (k, _), v in sorted(per.items(), key=lambda kv: kv[0][1]) if pat in k]
+
+    def copy_of(per):
+        vals = [v for v in pick(per, "copyBuffer") if v > 0.5 * kib]
+        return vals[-1] if vals else None
+
+    cf, cw = copy_of(fetch), copy_of(write)
+    sf, sw = kib / cf, kib / cw
+    pf, pw = pick(fetch, "k_onesweep_rg"), pick(write, "k_onesweep_rg")
+    hf = pick(fetch, "k_histogram")
+    rd = sf * sum(pf) / len(pf) * 1024
+    wr = sw * sum(pw) / len(pw) * 1024
+    alg = 16 * n
+    lines = [
+        f"# HBM traffic per launch, 2^{log2n} uniform 16-byte pairs, d = {d}, tile {tile}",
+        "",
+        "Commands (two passes, one counter each; repo root = /root/repo):",
+        "",
+        "    " + cmd_fetch,
+        "    " + cmd_write,
+        "",
+        "Summary written by `python3 bench.py --traffic-summary <fetch dir> <write dir>`.",
+        "",
+        "Raw counters, KiB per dispatch:",
+        "",
+        "| kernel | dispatches | FETCH_SIZE | WRITE_SIZE |",
+        "|---|---|---|---|",
+        f"| plain copy (`copyBuffer`, 2^{log2n} x 16 B) | 1 | {cf:.0f} | {cw:.0f} |",
+        f"| `k_histogram` | {len(hf)} | {' '.join(f'{v:.0f}' for v in hf)} | - |",
+        f"| `k_onesweep_rg` | {len(pf)} | {' '.join(f'{v:.0f}' for v in pf)} | "
+        f"{' '.join(f'{v:.0f}' for v in pw)} |",
+        "",
+        f"Calibration on the plain copy ({kib:.0f} KiB read and written): FETCH_SIZE scale "
+        f"{sf:.3f}, WRITE_SIZE scale {sw:.3f}.",
+        "",
+        "Scatter pass, mean over its dispatches, scaled:",
+        "",
+        f"- read  = {rd / 1e9:.3f} GB (algorithmic {alg / 1e9:.3f} GB, {100 * (rd / alg - 1):+.1f} %)",
+        f"- write = {wr / 1e9:.3f} GB (algorithmic {alg / 1e9:.3f} GB, {100 * (wr / alg - 1):+.1f} %)",
+        f"- traffic per launch = {(rd + wr) / 1e9:.3f} GB = {(rd + wr) / n:.2f} B per element "
+        "(algorithmic 32 B)",
+        "",
+        f"traffic_bytes_per_launch: {int(round(rd + wr))}",
+        f"traffic_config: log2n={log2n} digit_bits={d} tile={tile}",
+        f"traffic_sources: {source_digest()}",
+        "",
+    ]
+    open(TRAFFIC_PROFILE, "w").write("\n".join(lines))
+    print("\n".join(lines))
 
 
 def main():
@@ -103,13 +260,33 @@ def main():
     ap.add_argument("--digit-bits", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-log2n", type=int, default=26)
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also time the reference at 2^30 (needs 96 GiB of free RAM)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only the timed sorts and one plain copy of the array, for "
+                         "rocprofv3 --pmc runs")
+    ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write the traffic profile from two rocprofv3 --pmc output dirs")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-segments", type=int, default=4096)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serving-jobs", type=int, default=96)
+    ap.add_argument("--serving-jobs", type=int, default=500)
+    ap.add_argument("--serving-wide-jobs", type=int, default=96)
     args = ap.parse_args()
+
+    if args.traffic_summary:
+        probe = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 120 rocprofv3 --pmc {c} "
+                 "--output-format csv -d /root/repo/gpurun_out/pmc_{l} -o {l} -- python3 "
+                 "/root/repo/bench.py --traffic-probe --steps 1 --warmup 0")
+        d, tile = args.digit_bits or 8, args.tile or 8192
+        traffic_summary(args.traffic_summary[0], args.traffic_summary[1], args.log2n, d, tile,
+                        probe.format(c="FETCH_SIZE", l="fetch"),
+                        probe.format(c="WRITE_SIZE", l="write"))
+        return
+    if args.traffic_probe:
+        args.no_cpu_baseline = args.no_verify = args.no_batched = args.no_serving = True
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the sort is one rank's share on one MI355X "
@@ -119,7 +296,7 @@ def main():
     base = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        base = cpu_baseline(1 << args.cpu_sample_log2n, 16, 3)
+        base = cpu_baseline(args.cpu_reps, args.cpu_full)
         log("cpu baseline:", base)
 
     import torch
@@ -236,7 +413,7 @@ def main():
         out["serving"] = bench_serving(args.serving_jobs, span_log2=12)
         torch.cuda.empty_cache()
         log("serving stream, sizes 2^12..2^28 ...")
-        out["serving_wide"] = bench_serving(args.serving_jobs, span_log2=16)
+        out["serving_wide"] = bench_serving(args.serving_wide_jobs, span_log2=16)
         torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
 
@@ -341,6 +518,7 @@ def bench_serving(jobs, seed=77, lo_log2=12, span_log2=16):
     torch.cuda.empty_cache()
     out = {"workload": f"{jobs} jobs, sizes log-uniform [2^{lo_log2}, 2^{lo_log2 + span_log2}], "
                        "32/64-bit keys, uniform/Zipf(1.1), HBM-resident",
+           "jobs": jobs,
            "elements": total, "plain_melem_s": round(total / t_plain / 1e6, 1)}
     with lsb_amd.Server() as srv:
         # warm the server's streams, workspaces and graphs
@@ -376,7 +554,11 @@ def bench_serving(jobs, seed=77, lo_log2=12, span_log2=16):
         st = srv.stats()
         out.update({"poisson_rate_jobs_s": round(lam, 2),
                     "poisson_melem_s": round(total / st["span_s"] / 1e6, 1),
-                    "p50_ms": round(st["p50_us"] / 1e3, 3), "p99_ms": round(st["p99_us"] / 1e3, 3)})
+                    "p50_ms": round(st["p50_us"] / 1e3, 3), "p99_ms": round(st["p99_us"] / 1e3, 3),
+                    "max_ms": round(st["max_us"] / 1e3, 3),
+                    "latency_samples": int(st["jobs_done"]),
+                    "latency": "p50/p99/max of completion - submission over every job of the "
+                               "Poisson run (arrivals at 0.8 of the saturated capacity)"})
     # every job's result is the stable sort of its input
     ok = True
     keys = torch.empty(max(j["n"] for j in trace), dtype=torch.int64, device=dev)

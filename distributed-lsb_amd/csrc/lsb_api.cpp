@@ -294,22 +294,45 @@ hipError_t pool_trim(int dev) {
   return it == g_pools.end() ? hipSuccess : hipMemPoolTrimTo(it->second, 0);
 }
 
+int check_sort_args(const void* A, const void* B, size_t n) {
+  return check_host_args(static_cast<const lsb_elem*>(A), static_cast<const lsb_elem*>(B), n);
+}
+
+void note_hip_error(hipError_t e) {
+  g_last_hip = (int)e;
+  (void)hipGetLastError();
+}
+
 }  // namespace lsb
 
 namespace {
 
+// Each entry has its own mutex, held by a sort from the moment it takes the
+// workspace until its launches are enqueued: a second thread on the same
+// stream that grows the workspace then frees the old buffer (stream-ordered)
+// only after those launches.  g_ws_mu guards the map alone; entries are
+// never erased (lsb_release empties them), so a reference stays valid.
 struct Workspace {
+  std::mutex mu;
   void* ptr = nullptr;
   size_t bytes = 0;
 };
 std::mutex g_ws_mu;
 std::map<std::pair<int, void*>, Workspace> g_ws;
 
-int get_workspace(void* stream, size_t need, void** out) {
+template <class W>
+W& cache_entry(std::map<std::pair<int, void*>, W>& m, int dev, void* stream) {
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  return m[std::make_pair(dev, stream)];
+}
+
+// Takes the (device, stream) workspace, grown to `need` bytes, and returns
+// it with its entry locked in *hold.
+int get_workspace(void* stream, size_t need, void** out, std::unique_lock<std::mutex>* hold) {
   int dev = 0;
   LSB_HIP(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lock(g_ws_mu);
-  Workspace& w = g_ws[std::make_pair(dev, stream)];
+  Workspace& w = cache_entry(g_ws, dev, stream);
+  *hold = std::unique_lock<std::mutex>(w.mu);
   if (w.bytes < need) {
     // Stream-ordered: the old buffer is released after the sorts already
     // queued on this stream, so growing waits for nothing.
@@ -542,6 +565,7 @@ int make_seg_layout(size_t cap_n, uint32_t cap_segs, const lsb_config& c, SegLay
 // capacity their layout was made for, and a pinned staging buffer for the
 // per-call batch shape.
 struct SegWorkspace {
+  std::mutex mu;  // held by a batched sort from staging until its launches are enqueued
   void* ptr = nullptr;
   size_t bytes = 0, cap_n = 0;
   uint32_t cap_segs = 0;
@@ -684,9 +708,10 @@ int lsb_sort(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg) {
     return LSB_OK;
   }
   void* ws = c.workspace;
+  std::unique_lock<std::mutex> hold;  // the library workspace, until enqueued
   if (ws) {
     if (c.workspace_bytes < p.total || ((uintptr_t)ws & 255u)) return LSB_ENOMEM;
-  } else if ((rc = get_workspace(c.stream, p.total, &ws))) {
+  } else if ((rc = get_workspace(c.stream, p.total, &ws, &hold))) {
     return rc;
   }
   if (c.flags & LSB_GRAPH) {
@@ -704,17 +729,32 @@ int lsb_sort(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg) {
   } else {
     rc = enqueue_sort(A, B, (uint32_t)n, p, static_cast<char*>(ws), c.flags, s);
   }
+  if (hold.owns_lock()) hold.unlock();
   if (rc) return rc;
   if (c.flags & LSB_SYNC) LSB_HIP(hipStreamSynchronize(s));
   return LSB_OK;
 }
 
 
+// The largest layout over the compiled segmented variants of the digit
+// width (and tile, when the config names one): the tuned choice for a batch
+// depends on its segment sizes, and any of them must fit.
 size_t lsb_segmented_workspace_size(size_t total_n, uint32_t nseg, const lsb_config* cfg) {
   const lsb_config c = cfg ? *cfg : defaults();
   SegLayout L;
   if (make_seg_layout(total_n, nseg, c, &L)) return 0;
-  return L.total;
+  size_t most = L.total;
+  int nv = 0;
+  const lsb::SegVariant* vs = lsb::seg_variants(&nv);
+  for (int i = 0; i < nv; ++i) {
+    if (vs[i].digit_bits != L.d || (c.tile && vs[i].tile != c.tile)) continue;
+    lsb_config v = c;
+    v.digit_bits = vs[i].digit_bits;
+    v.tile = vs[i].tile;
+    SegLayout M;
+    if (make_seg_layout(total_n, nseg, v, &M) == LSB_OK) most = std::max(most, M.total);
+  }
+  return most;
 }
 
 }  // extern "C"
@@ -751,10 +791,14 @@ int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_c
   int dev = 0;
   LSB_HIP(hipGetDevice(&dev));
 
-  std::lock_guard<std::mutex> lock(g_ws_mu);
-  SegWorkspace& W = g_seg_ws[std::make_pair(dev, c.stream)];
+  SegWorkspace& W = cache_entry(g_seg_ws, dev, c.stream);
+  std::unique_lock<std::mutex> hold(W.mu);
   char* ws = static_cast<char*>(c.workspace);
   if (ws) {
+    // A caller workspace sized by lsb_segmented_workspace_size holds the
+    // largest layout of the digit width; if the tuned tile needs more than
+    // the caller gave, the default tile's layout is used instead.
+    if (L.total > c.workspace_bytes && (rc = make_seg_layout(total_n, nseg, c, &L))) return rc;
     if (c.workspace_bytes < L.total || ((uintptr_t)ws & 255u)) return LSB_ENOMEM;
   } else {
     // The layout of a library workspace follows its capacity, not the batch,
@@ -880,6 +924,7 @@ int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_c
   } else {
     rc = enqueue_segmented(L, ws, c.flags, s);
   }
+  hold.unlock();
   if (rc) return rc;
   if (c.flags & LSB_SYNC) LSB_HIP(hipStreamSynchronize(s));
   return LSB_OK;
@@ -945,7 +990,6 @@ int lsb_release(void* stream) {
   LSB_HIP(hipGetDevice(&dev));
   const hipStream_t s = (hipStream_t)stream;
   LSB_HIP(hipStreamSynchronize(s));  // nothing queued on it may still use them
-  const auto key = std::make_pair(dev, stream);
   std::vector<uint64_t> freed;
   int rc = LSB_OK;
   auto note = [&](hipError_t e) {
@@ -954,25 +998,33 @@ int lsb_release(void* stream) {
       rc = LSB_EHIP;
     }
   };
-  std::lock_guard<std::mutex> lock(g_ws_mu);
-  auto w = g_ws.find(key);
-  if (w != g_ws.end()) {
-    if (w->second.ptr) {
-      freed.push_back((uint64_t)(uintptr_t)w->second.ptr);
-      note(hipFreeAsync(w->second.ptr, s));
+  // Entries stay in the maps, emptied, so no other thread can hold a
+  // reference to a destroyed one.
+  {
+    Workspace& w = cache_entry(g_ws, dev, stream);
+    std::lock_guard<std::mutex> lock(w.mu);
+    if (w.ptr) {
+      freed.push_back((uint64_t)(uintptr_t)w.ptr);
+      note(hipFreeAsync(w.ptr, s));
     }
-    g_ws.erase(w);
+    w.ptr = nullptr;
+    w.bytes = 0;
   }
-  auto sw = g_seg_ws.find(key);
-  if (sw != g_seg_ws.end()) {
-    SegWorkspace& W = sw->second;
+  {
+    SegWorkspace& W = cache_entry(g_seg_ws, dev, stream);
+    std::lock_guard<std::mutex> lock(W.mu);
     if (W.ptr) {
       freed.push_back((uint64_t)(uintptr_t)W.ptr);
       note(hipFreeAsync(W.ptr, s));
     }
     if (W.staged) note(hipEventDestroy(W.staged));
     if (W.stage) note(hipHostFree(W.stage));
-    g_seg_ws.erase(sw);
+    W.ptr = nullptr;
+    W.bytes = W.cap_n = 0;
+    W.cap_segs = 0;
+    W.stage = nullptr;
+    W.stage_bytes = 0;
+    W.staged = nullptr;
   }
   if (!freed.empty()) {
     // Graphs captured against a freed workspace (its address is part of

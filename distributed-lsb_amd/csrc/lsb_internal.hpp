@@ -67,6 +67,11 @@ const SortVariant* sort_variants(int* count);
 hipError_t pool_alloc(void** p, size_t bytes, hipStream_t s);
 // Returns the unused memory of device dev's library pool to the device.
 hipError_t pool_trim(int dev);
+// lsb_sort's host-side argument checks (null, alignment, overlap, n).
+int check_sort_args(const void* A, const void* B, size_t n);
+// Records e as this thread's lsb_last_hip_error() and clears HIP's sticky
+// last error.
+void note_hip_error(hipError_t e);
 
 // Zeroes p[0..words), launched before the histogram.
 void launch_zero(uint32_t* p, size_t words, hipStream_t s);

@@ -51,6 +51,7 @@ struct Job {
   double submitted, done_at = 0;
   int state = 0;  // 0 pending, 1 admitted/running, 2 done, <0 error code
   int rc = 0;
+  bool waited = false;  // held at the head of the queue for budget memory
 };
 
 struct Launch {
@@ -99,6 +100,7 @@ struct lsb_server {
   std::vector<double> lat_us;  // ring of the latest kRetain latencies
   size_t lat_next = 0;
   uint64_t done_jobs = 0, done_elems = 0, batches = 0, batched_jobs = 0, plain_jobs = 0;
+  uint64_t pool_plain_jobs = 0, admission_waits = 0;
   double first_submit = 0, last_done = 0;
 
   void run();
@@ -200,6 +202,7 @@ int lsb_server::launch_plain(Job& j) {
   }
   ++sl.inflight;
   ++plain_jobs;
+  if (j.n > 1 && !use_slab) ++pool_plain_jobs;
   inflight.push_back(L);
   return LSB_OK;
 }
@@ -293,7 +296,13 @@ bool lsb_server::admit() {
   while (!pending.empty()) {
     Job& j = jobs[pending.front()];
     const uint64_t charge = 32ull * j.n;
-    if (used + charge > cfg.memory_budget && used > 0) break;  // FIFO: wait for memory
+    if (used + charge > cfg.memory_budget && used > 0) {  // FIFO: wait for memory
+      if (!j.waited) {
+        j.waited = true;
+        ++admission_waits;
+      }
+      break;
+    }
     if (j.n <= cfg.batch_max_elems) {
       // jobs are batched with others of the same key width and key order
       const uint32_t cls = j.key_bits | (j.key_flags << 8);
@@ -554,6 +563,8 @@ int lsb_server_stats_get(lsb_server* s, lsb_server_stats* out) {
   out->batches = s->batches;
   out->batched_jobs = s->batched_jobs;
   out->plain_jobs = s->plain_jobs;
+  out->pool_plain_jobs = s->pool_plain_jobs;
+  out->admission_waits = s->admission_waits;
   out->memory_in_use = s->used;
   std::vector<double> v = s->lat_us;
   if (!v.empty()) {
@@ -579,6 +590,7 @@ int lsb_server_reset_stats(lsb_server* s) {
   s->lat_us.clear();
   s->lat_next = 0;
   s->done_jobs = s->done_elems = s->batches = s->batched_jobs = s->plain_jobs = 0;
+  s->pool_plain_jobs = s->admission_waits = 0;
   s->first_submit = s->last_done = 0;
   for (uint64_t id : s->finished) s->jobs.erase(id);
   s->finished.clear();

@@ -78,7 +78,8 @@ class ServerStats(ctypes.Structure):
                 ("plain_jobs", ctypes.c_uint64), ("memory_in_use", ctypes.c_uint64),
                 ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double),
                 ("max_us", ctypes.c_double), ("mean_us", ctypes.c_double),
-                ("span_s", ctypes.c_double)]
+                ("span_s", ctypes.c_double), ("pool_plain_jobs", ctypes.c_uint64),
+                ("admission_waits", ctypes.c_uint64)]
 
 
 class TuneEntry(ctypes.Structure):

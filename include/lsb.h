@@ -214,6 +214,8 @@ typedef struct lsb_server_stats {
   uint64_t memory_in_use;    /* budget bytes currently charged */
   double p50_us, p99_us, max_us, mean_us;  /* completion - submission */
   double span_s;             /* last completion - first submission */
+  uint64_t pool_plain_jobs;  /* plain jobs larger than a scratch slab (scratch from the pool) */
+  uint64_t admission_waits;  /* jobs that waited at the head of the queue for budget memory */
 } lsb_server_stats;
 
 void lsb_server_config_init(lsb_server_config* cfg);

@@ -31,6 +31,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(lsb_amd.Config) == 48
     assert lsb_amd.Config.stream.offset == 16
     assert ctypes.sizeof(lsb_amd.GenSpec) == 48
+    assert ctypes.sizeof(lsb_amd.ServerStats) == 104
 
 
 def test_version_and_strerror():

@@ -318,6 +318,43 @@ def test_concurrent_host_threads():
         lsb_amd.release(s)
 
 
+def test_two_threads_one_stream_growing_sizes():
+    """Two host threads sorting on the SAME stream with sizes that keep
+    growing, so one thread regrows the stream's cached workspace while the
+    other's sorts are queued: the old workspace must outlive them (the
+    entry's lock is held from lookup until the launches are enqueued)."""
+    import threading
+    s = torch.cuda.Stream()
+    sizes = [[(1 << k) + 3 * t + 1 for k in range(14, 22)] for t in range(2)]
+    inputs = [[O.gen_uniform(n, ranks=1, seed_base=900 + 50 * t + i)
+               for i, n in enumerate(sizes[t])] for t in range(2)]
+    errors = []
+    bufs = [[(to_dev(a), torch.empty((len(a), 2), dtype=torch.int64, device=DEV))
+             for a in inputs[t]] for t in range(2)]
+    torch.cuda.synchronize()
+    start = threading.Barrier(2)
+
+    def work(t):
+        try:
+            start.wait()
+            for A, B in bufs[t]:
+                lsb_amd.sort(A, B, stream=s)
+        except Exception as e:
+            errors.append(repr(e))
+
+    ths = [threading.Thread(target=work, args=(t,)) for t in range(2)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    s.synchronize()
+    assert errors == []
+    for t in range(2):
+        for a, (A, _) in zip(inputs[t], bufs[t]):
+            assert np.array_equal(from_dev(A), stable(a)), (t, len(a))
+    lsb_amd.release(s)
+
+
 def test_repeated_sorts_reuse_workspace():
     """Back-to-back sorts of different sizes on one stream share the cached
     workspace; stale look-back state must never leak between them."""
@@ -329,6 +366,20 @@ def test_repeated_sorts_reuse_workspace():
 def _device_check(A_sorted, keys, n, key_bits=64):
     bad, first = lsb_amd.verify(A_sorted, keys, n=n, key_bits=key_bits)
     return bad, first
+
+
+def _torch_stable_check(A_sorted, keys):
+    """Independent full-size check with torch's own stable GPU sort: the
+    output's keys must be the sorted keys and its vals (= input indices) the
+    stable permutation.  Unsigned order is signed order with the top bit
+    flipped."""
+    flip = torch.tensor(-(1 << 63), dtype=torch.int64, device=DEV)
+    sk, idx = torch.sort(torch.bitwise_xor(keys, flip), stable=True)
+    ok_keys = torch.equal(torch.bitwise_xor(sk, flip), A_sorted[:, 0])
+    del sk
+    ok_vals = torch.equal(idx, A_sorted[:, 1])
+    del idx
+    return ok_keys and ok_vals
 
 
 def test_device_verifier_detects_wrong_output():
@@ -366,8 +417,18 @@ def test_stress_2p28(cfg):
     B = torch.empty_like(A)
     lsb_amd.sort(A, B, sync=True)
     assert _device_check(A, keys, n) == (0, -1)
-    # spot-check against the oracle on a window of the input
     del B
+    kind = cfg["kind"]
+    if kind in (lsb_amd.GEN_SORTED, lsb_amd.GEN_REVERSE):
+        # closed form: sorted input is its own result; reversed input (key
+        # n-1-i at index i) comes out as key j carrying val n-1-j
+        j = torch.arange(n, dtype=torch.int64, device=DEV)
+        assert torch.equal(A[:, 0], j)
+        assert torch.equal(A[:, 1], j if kind == lsb_amd.GEN_SORTED else (n - 1) - j)
+        del j
+    else:
+        assert _torch_stable_check(A, keys)
+    del A, keys
     torch.cuda.empty_cache()
 
 
@@ -385,6 +446,7 @@ def test_full_2p30_uniform():
     lsb_amd.sort(A, B, sync=True)
     del B
     assert _device_check(A, keys, n) == (0, -1)
+    assert _torch_stable_check(A, keys)
     del A, keys
     torch.cuda.empty_cache()
 
@@ -401,6 +463,7 @@ def test_max_n_uniform():
     lsb_amd.sort(A, B, sync=True)
     del B
     assert _device_check(A, keys, n) == (0, -1)
+    assert _torch_stable_check(A, keys)
     del A, keys
     torch.cuda.empty_cache()
 

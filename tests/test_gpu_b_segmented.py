@@ -62,6 +62,31 @@ def test_segmented_mixed_sizes(kind):
         assert np.array_equal(from_dev(A), want), (kind, d, tile)
 
 
+@pytest.mark.parametrize("nseg", [64, 16])
+def test_caller_workspace_sized_by_segmented_workspace_size(nseg):
+    """A workspace of segmented_workspace_size(total, nseg) bytes holds the
+    layout of whatever tile the sort picks for the batch: nseg segments of
+    2^14 (small totals pick small tiles, whose layout is the larger one)."""
+    sizes = [1 << 14] * nseg
+    a, off = batch(sizes, seed=40 + nseg)
+    want = expected(a, off)
+    total = int(off[-1])
+    wbytes = lsb_amd.segmented_workspace_size(total, nseg)
+    assert wbytes > 0
+    ws = torch.empty(wbytes, dtype=torch.uint8, device=DEV)
+    for graph in (False, True):
+        A = to_dev(a)
+        B = torch.empty_like(A)
+        lsb_amd.sort_segmented(A, B, off, sync=True, graph=graph, workspace=ws,
+                               workspace_bytes=wbytes)
+        assert np.array_equal(from_dev(A), want), graph
+        As = [to_dev(a[int(off[i]):int(off[i + 1])]) for i in range(nseg)]
+        Bs = [torch.empty_like(x) for x in As]
+        lsb_amd.sort_batch(As, Bs, sync=True, graph=graph)
+        for i, X in enumerate(As):
+            assert np.array_equal(from_dev(X), want[int(off[i]):int(off[i + 1])])
+
+
 def test_segmented_skips_and_parity():
     """Segments whose keys leave some or all digits in one bucket run an odd
     or even number of passes; each must still end in A."""

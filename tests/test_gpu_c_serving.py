@@ -3,6 +3,7 @@ that job returns -- whether the job ran by itself, inside a coalesced batch,
 or overlapped with other jobs on other streams -- and its admission respects
 the memory budget."""
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -10,6 +11,7 @@ import torch
 
 import lsb_amd
 import oracle as O
+from lsb_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
@@ -215,3 +217,104 @@ def test_finished_jobs_retained_up_to_limit():
         with pytest.raises(lsb_amd.LsbError):
             srv.poll(last[-1])
     assert np.array_equal(from_dev(buf0), expected(a0, 64))
+
+
+def _torch_stable_ok(A_sorted, keys):
+    """torch's own stable GPU sort as an independent check of one job:
+    sorted keys and vals (= input indices) in stable order."""
+    flip = torch.tensor(-(1 << 63), dtype=torch.int64, device=DEV)
+    sk, idx = torch.sort(torch.bitwise_xor(keys, flip), stable=True)
+    return (torch.equal(torch.bitwise_xor(sk, flip), A_sorted[:, 0])
+            and torch.equal(idx, A_sorted[:, 1]))
+
+
+def _poisson_feed(srv, bufs, trace, rate):
+    """Submits job j at time sum(gap[0..j]) / rate from a thread of its own,
+    as arrivals that come while earlier jobs are on the GPU."""
+    ids = [None] * len(bufs)
+    err = []
+
+    def feed():
+        try:
+            t0, at = time.perf_counter(), 0.0
+            for j, (a, job) in enumerate(zip(bufs, trace)):
+                at += job["gap"] / rate
+                d = at - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                ids[j] = srv.submit(a, key_bits=job["key_bits"])
+        except Exception as e:
+            err.append(repr(e))
+
+    th = threading.Thread(target=feed)
+    th.start()
+    th.join()
+    assert err == []
+    srv.drain()
+    return ids
+
+
+def test_serving_stream_full_workload():
+    """The serving config at its own workload: 40 jobs of log-uniform size
+    over [2^12, 2^28] (1.15 G elements, largest 2.48e8), 32- or 64-bit keys,
+    uniform or Zipf(1.1) keys, arriving as a Poisson stream from another
+    thread.  Run twice: with the default server (200 GB budget; jobs up to
+    2^24 coalesced, larger ones plain with a scratch slab) and with a 10 GB
+    budget and 2^22-element slabs, where admission waits for memory and jobs
+    above 2^22 take their scratch from the memory pool.  Every job is proven
+    the stable sort of its input on the device; one batched job and one
+    slab-plain job are compared with numpy's stable sort, the largest with
+    torch's stable sort."""
+    trace = W.serving_trace(40, 79, lo_log2=12, span_log2=16)
+    bufs, pris, keys = [], [], []
+    for j, job in enumerate(trace):
+        a = torch.empty((job["n"], 2), dtype=torch.int64, device=DEV)
+        lsb_amd.generate(a, kind=lsb_amd.GEN_ZIPF if job["zipf"] else lsb_amd.GEN_UNIFORM,
+                         ranks=4, key_bits=job["key_bits"], seed_base=9100 + 16 * j)
+        k = torch.empty(job["n"], dtype=torch.int64, device=DEV)
+        lsb_amd.extract_keys(a, k)
+        bufs.append(a)
+        pris.append(a.clone())
+        keys.append(k)
+    torch.cuda.synchronize()
+    ns = [job["n"] for job in trace]
+    assert max(ns) > 2 ** 27 and sum(n > 2 ** 24 for n in ns) >= 5
+    small = next(j for j, n in enumerate(ns) if 2 ** 16 <= n <= 2 ** 20)
+    slab = min((j for j, n in enumerate(ns) if n > 2 ** 24), key=lambda j: ns[j])
+    largest = max(range(len(ns)), key=lambda j: ns[j])
+    host_in = {j: pris[j].cpu().numpy().reshape(-1).view(O.ELEM).copy() for j in (small, slab)}
+
+    def check_all(tag):
+        for j, (a, job) in enumerate(zip(bufs, trace)):
+            assert lsb_amd.verify(a, keys[j], key_bits=job["key_bits"]) == (0, -1), (tag, j)
+        for j in (small, slab):
+            got = from_dev(bufs[j])
+            assert np.array_equal(got, expected(host_in[j], trace[j]["key_bits"])), (tag, j)
+        assert _torch_stable_ok(bufs[largest], keys[largest]), (tag, largest)
+
+    print(f"serving full workload: {len(ns)} jobs, {sum(ns)} elements, default server",
+          flush=True)
+    with lsb_amd.Server() as srv:
+        _poisson_feed(srv, bufs, trace, rate=2000.0)
+        st = srv.stats()
+    assert st["jobs_done"] == len(ns)
+    # how many small jobs coalesce depends on how many arrive while the
+    # device is busy; at least one batch and the slab-plain jobs must run
+    assert st["batches"] >= 1 and st["batched_jobs"] >= 2, st
+    assert st["plain_jobs"] >= sum(n > 2 ** 24 for n in ns), st
+    assert st["pool_plain_jobs"] == 0 and st["memory_in_use"] == 0, st
+    check_all("default")
+
+    for a, p in zip(bufs, pris):
+        a.copy_(p)
+    torch.cuda.synchronize()
+    print("serving full workload: 10 GB budget, 2^22-element slabs", flush=True)
+    with lsb_amd.Server(memory_budget=10 * 10 ** 9, batch_max_elems=1 << 20,
+                        batch_max_total=1 << 22) as srv:
+        _poisson_feed(srv, bufs, trace, rate=2000.0)
+        st = srv.stats()
+    assert st["jobs_done"] == len(ns)
+    assert st["pool_plain_jobs"] >= sum(n > 2 ** 22 for n in ns), st
+    assert st["admission_waits"] >= 1 and st["batches"] >= 1, st
+    assert st["memory_in_use"] == 0
+    check_all("small budget")
