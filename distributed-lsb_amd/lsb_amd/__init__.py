@@ -25,6 +25,7 @@ LSB_KEYS_SIGNED, LSB_KEYS_FLOAT = 16, 32
 # key_order names accepted by the sort, verify and server calls
 KEY_ORDERS = {"unsigned": 0, "signed": LSB_KEYS_SIGNED, "float": LSB_KEYS_FLOAT}
 LSB_MAX_N = 0x7FFFFFFE
+LSB_NOALLOC_MAX_N = 1 << 27
 GEN_UNIFORM, GEN_SORTED, GEN_REVERSE, GEN_ZIPF = 0, 1, 2, 3
 
 
@@ -98,6 +99,8 @@ _P, _SZ, _U32, _U64, _I = (ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32,
 SIGNATURES = {
     "lsb_config_init": (None, [ctypes.POINTER(Config)]),
     "lsb_sort": (_I, [_P, _P, _SZ, ctypes.POINTER(Config)]),
+    "lsb_sort_noalloc": (_I, [_P, _P, _SZ, ctypes.POINTER(Config)]),
+    "lsb_noalloc_scratch_bytes": (_SZ, []),
     "lsb_workspace_size": (_SZ, [_SZ, ctypes.POINTER(Config)]),
     "lsb_plan": (_I, [_SZ, ctypes.POINTER(Config), ctypes.POINTER(_U32),
                       ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
@@ -260,6 +263,22 @@ def sort(A, B, n=None, key_bits=64, digit_bits=0, tile=0, stream=None, sync=Fals
     c = make_config(key_bits, digit_bits, tile, stream, flags, workspace, workspace_bytes,
                     variant)
     _check(lib().lsb_sort(_ptr(A), _ptr(B), n, ctypes.byref(c)), "lsb_sort")
+
+
+def sort_noalloc(A, B, n=None, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,
+                 no_skip=False, variant=0, key_order="unsigned"):
+    """sort() without any device allocation at call time (lsb_sort_noalloc):
+    the workspace is the library's fixed per-device scratch array, so
+    n <= LSB_NOALLOC_MAX_N and calls that overlap on one device race."""
+    n = _nelem(A) if n is None else n
+    flags = ((LSB_SYNC if sync else 0) | (LSB_NO_SKIP if no_skip else 0)
+             | _key_flags(key_order))
+    c = make_config(key_bits, digit_bits, tile, stream, flags, variant=variant)
+    _check(lib().lsb_sort_noalloc(_ptr(A), _ptr(B), n, ctypes.byref(c)), "lsb_sort_noalloc")
+
+
+def noalloc_scratch_bytes():
+    return lib().lsb_noalloc_scratch_bytes()
 
 
 def sort_segmented(A, B, offsets, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,

@@ -92,6 +92,21 @@ void lsb_config_init(lsb_config* cfg);
  * NULL.  Returns LSB_OK or a negative LSB_E* code. */
 int lsb_sort(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg);
 
+/* lsb_sort with no device allocation at call time, for callers that cannot
+ * allocate (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference, whose mySort
+ * allocates its temporaries per pass (mpi/mpi_lsbsort.cpp:523-528, 550-555).
+ * Same contract, launch sequence and results as lsb_sort, but the workspace
+ * is a fixed device array of lsb_noalloc_scratch_bytes() bytes that every
+ * GPU the library runs on holds until the process ends; the first call on a
+ * device looks up that device's copy.  Accepts n <= LSB_NOALLOC_MAX_N
+ * (LSB_ERANGE above); cfg->workspace must be NULL and LSB_GRAPH /
+ * LSB_PROFILE unset (LSB_EINVAL).  All calls on one device share the array:
+ * calls that overlap on one device (two streams, two threads) race. */
+#define LSB_NOALLOC_MAX_N ((size_t)1 << 27)
+int lsb_sort_noalloc(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg);
+size_t lsb_noalloc_scratch_bytes(void);
+
 /* Bytes of device workspace lsb_sort(n, cfg) needs (0 for n <= 1). */
 size_t lsb_workspace_size(size_t n, const lsb_config* cfg);
 
