@@ -498,18 +498,79 @@ int launch_graph(const std::vector<uint64_t>& key, hipStream_t s, Enqueue enqueu
 // ------------------------------------------------------- segmented sort
 struct SegLayout {
   const lsb::SegVariant* v;
+  bool msd;  // large segments sorted MSD-first (else by LSD passes)
   uint32_t key_bits, d, tile, passes, maxp, nb;
   lsb::KeyOrder ko;
   size_t cap_n;
   uint32_t cap_segs, max_tiles;
   size_t hdr, meta, counts, plan, tickets, status, tileseg, perm, total;
+  // MSD-first form
+  uint32_t cap_entries, cap_runs;
+  size_t ment[2], mcounts[2], morr[2], mtile[2], mstatus, mlvl, mruns, mrunctr;
 };
+
+// The MSD-first form sorts the large segments unless the caller asks for the
+// LSD passes (LSB_SEG_LSD), names a digit width other than 8 or a tile, or
+// sets LSB_NO_SKIP (run every digit pass).
+bool wants_msd(const lsb_config& c) {
+  return !(c.flags & (LSB_SEG_LSD | LSB_NO_SKIP)) && (c.digit_bits == 0 || c.digit_bits == 8) &&
+         c.tile == 0;
+}
+
+// Capacities of the MSD-first form for batches of at most cap_n elements.
+// Every range of every level holds more than kMsdFinishCap elements (the
+// large segments, then buckets too big to finish) and the ranges of a level
+// are disjoint, so a level has at most cap_n / (kMsdFinishCap + 1) of them,
+// and at most cap_n / kMsdTile + that many tiles.  Runs: along one range's
+// buckets, two consecutive runs not separated by a large bucket hold more
+// than kMsdFinishCap elements together, so runs <= 2 cap_n / kMsdFinishCap
+// + (number of partitioned ranges + large buckets) over all levels.
+int make_msd_layout(size_t cap_n, SegLayout* L) {
+  const uint64_t ents = cap_n / (lsb::kMsdFinishCap + 1) + 1;
+  const uint64_t tiles = cap_n / lsb::kMsdTile + ents + 1;
+  const uint64_t runs = 2 * (cap_n / lsb::kMsdFinishCap) + 2 * lsb::kMsdMaxLevels * ents + 64;
+  if (tiles > 0x7FFFFFFFull || runs > 0xFFFFFFFFull) return LSB_ERANGE;
+  L->cap_entries = (uint32_t)ents;
+  L->max_tiles = (uint32_t)tiles;
+  L->cap_runs = (uint32_t)runs;
+  size_t off = L->perm;  // after the header and the segment table
+  for (int i = 0; i < 2; ++i) {
+    L->ment[i] = off;    off += align_up(ents * sizeof(lsb::MsdEntry));
+    L->mcounts[i] = off; off += align_up(ents * 256 * 4);
+    L->morr[i] = off;    off += align_up(ents * 8);
+    L->mtile[i] = off;   off += align_up(tiles * 4);
+  }
+  L->mstatus = off; off += align_up(tiles * 256 * 4);
+  L->mlvl = off;    off += align_up(lsb::kMsdMaxLevels * sizeof(lsb::MsdLevel));
+  L->mruns = off;   off += align_up(runs * sizeof(lsb::MsdRun));
+  L->mrunctr = off; off += align_up(16);
+  L->total = off;
+  return LSB_OK;
+}
 
 int make_seg_layout(size_t cap_n, uint32_t cap_segs, const lsb_config& c, SegLayout* L,
                     size_t typical_n = 0) {
   L->key_bits = c.key_bits ? c.key_bits : 64;
   if (L->key_bits > 64) return LSB_EINVAL;
   if (make_key_order(c.flags, L->key_bits, &L->ko)) return LSB_EINVAL;
+  L->msd = wants_msd(c);
+  if (L->msd) {
+    int nv = 0;
+    const lsb::SegVariant* vs = lsb::seg_variants(&nv);
+    L->v = &vs[0];
+    L->d = 8;
+    L->tile = lsb::kMsdTile;
+    L->passes = (L->key_bits + 7) / 8;
+    L->maxp = 8;
+    L->nb = 256;
+    L->cap_n = cap_n;
+    L->cap_segs = cap_segs ? cap_segs : 1;
+    L->hdr = 0;
+    L->meta = align_up(sizeof(lsb::SegHeader));
+    L->perm = L->meta + align_up((size_t)L->cap_segs * sizeof(lsb::SegLarge));
+    L->counts = L->plan = L->tickets = L->status = L->tileseg = 0;
+    return make_msd_layout(cap_n, L);
+  }
   uint32_t d = c.digit_bits ? c.digit_bits : 8;
   uint32_t want_tile = c.tile;
   if (!c.digit_bits && !c.tile && typical_n) {
@@ -602,6 +663,36 @@ int enqueue_segmented(const SegLayout& L, char* ws, uint32_t flags, hipStream_t 
   uint32_t* tileseg = reinterpret_cast<uint32_t*>(ws + L.tileseg);
   uint32_t* perm = reinterpret_cast<uint32_t*>(ws + L.perm);
   const uint32_t cus = device_cus();
+  if (L.msd) {
+    LSB_HIP(lsb::launch_seg_small(hdr, small, L.key_bits, L.ko, 4 * cus, s));
+    lsb::MsdArgs m;
+    m.hdr = hdr;
+    m.large = large;
+    for (int i = 0; i < 2; ++i) {
+      m.ent[i] = reinterpret_cast<lsb::MsdEntry*>(ws + L.ment[i]);
+      m.counts[i] = reinterpret_cast<uint32_t*>(ws + L.mcounts[i]);
+      m.orr[i] = reinterpret_cast<uint32_t*>(ws + L.morr[i]);
+      m.tilemap[i] = reinterpret_cast<uint32_t*>(ws + L.mtile[i]);
+    }
+    m.status = reinterpret_cast<uint32_t*>(ws + L.mstatus);
+    m.lvl = reinterpret_cast<lsb::MsdLevel*>(ws + L.mlvl);
+    m.runs = reinterpret_cast<lsb::MsdRun*>(ws + L.mruns);
+    m.run_ctr = reinterpret_cast<uint32_t*>(ws + L.mrunctr);
+    m.cap_entries = L.cap_entries;
+    m.cap_tiles = L.max_tiles;
+    m.cap_runs = L.cap_runs;
+    m.key_bits = L.key_bits;
+    m.ko = L.ko;
+    m.level = 0;
+    LSB_HIP(lsb::launch_msd_init(m, 4 * cus, s));
+    // levels 0 .. P: the last one only copies ranges left in B
+    for (uint32_t k = 0; k <= L.passes && k < (uint32_t)lsb::kMsdMaxLevels; ++k) {
+      m.level = k;
+      LSB_HIP(lsb::launch_msd_level(m, cus, s));
+    }
+    LSB_HIP(lsb::launch_msd_finish(m, cus, s));
+    return LSB_OK;
+  }
   LSB_HIP(lsb::launch_seg_prepare(L.d, hdr, large, L.tile, L.key_bits, L.ko, L.passes, counts,
                                   plan, tickets, status, tileseg, perm, flags, 4 * cus, s));
   LSB_HIP(lsb::launch_seg_small(hdr, small, L.key_bits, L.ko, 4 * cus, s));
@@ -917,16 +1008,24 @@ int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_c
   if (c.flags & LSB_GRAPH) {
     const std::vector<uint64_t> key = {1u, (uint64_t)(uintptr_t)ws, L.cap_n, L.cap_segs, L.d,
                                        L.tile, L.key_bits, (uint64_t)(c.flags & ~LSB_SYNC),
-                                       (uint64_t)dev};
+                                       (uint64_t)dev, (uint64_t)L.msd};
     rc = launch_graph(key, s, [&](hipStream_t cs) {
       return enqueue_segmented(L, ws, c.flags, cs);
     });
   } else {
     rc = enqueue_segmented(L, ws, c.flags, s);
   }
-  hold.unlock();
   if (rc) return rc;
-  if (c.flags & LSB_SYNC) LSB_HIP(hipStreamSynchronize(s));
+  if (c.flags & LSB_SYNC) {
+    LSB_HIP(hipStreamSynchronize(s));
+    if (L.msd) {
+      // the MSD lists stayed within their capacities (a proven bound; a
+      // violation would mean wrong results, so it is reported)
+      uint32_t overflow = 0;
+      LSB_HIP(hipMemcpy(&overflow, ws + L.mrunctr + 8, 4, hipMemcpyDeviceToHost));
+      if (overflow) return LSB_ENOMEM;
+    }
+  }
   return LSB_OK;
 }
 

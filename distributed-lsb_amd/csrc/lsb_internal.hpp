@@ -180,6 +180,63 @@ hipError_t launch_seg_copy(const SegHeader* hdr, const SegLarge* large,
                            uint32_t grid,
                            hipStream_t s);
 
+// ------------------------------------------- MSD-first batched sort
+// The large segments of a batch are sorted most-significant digit first:
+// level k partitions every open range ("entry") on its highest digit that
+// still varies, with the same stable onesweep tile pass; children of at
+// most kMsdFinishCap elements are grouped into runs that one workgroup
+// sorts in LDS by their remaining digits (k_msd_finish), larger children
+// become the entries of level k+1.  Every list and count lives on the
+// device, so the launch sequence is fixed and can be captured.
+constexpr uint32_t kMsdTile = 8192;       // elements per onesweep tile
+constexpr uint32_t kMsdFinishCap = 8192;  // largest run one workgroup sorts in LDS
+constexpr int kMsdMaxLevels = 9;          // 8 digits of 8 bits + a copy-only level
+
+// flags of an entry
+constexpr uint32_t kMsdInB = 1u;   // its elements are in the scratch buffer
+constexpr uint32_t kMsdCopy = 2u;  // all keys equal and in B: copy to A at this level
+constexpr uint32_t kMsdMove = 4u;  // partitioned on `dig` at this level (set by the plan)
+
+struct MsdEntry {
+  uint64_t a;           // the range in the final buffer
+  uint64_t b;           // the same range in the scratch buffer
+  uint32_t n;
+  uint32_t tile_begin;  // first tile in its level's tile space
+  uint32_t dig;         // digit to partition on
+  uint32_t flags;
+};
+// A run: consecutive children of one partitioned entry, sorted in LDS by
+// the digits in `digits` (bit p = digit p), read from src, written to dst.
+struct MsdRun {
+  uint64_t src;
+  uint64_t dst;
+  uint32_t n;
+  uint32_t digits;
+};
+struct MsdLevel {
+  uint32_t entries, tiles, ticket, overflow;
+};
+struct MsdArgs {
+  const SegHeader* hdr;
+  const SegLarge* large;
+  MsdEntry* ent[2];        // entries of even / odd levels
+  uint32_t* counts[2];     // [entry][256] digit counts, then bucket starts
+  uint32_t* orr[2];        // [entry][2] OR over the entry of (key ^ first key)
+  uint32_t* tilemap[2];    // tile -> entry
+  uint32_t* status;        // [tile][256] look-back words of the current level
+  MsdLevel* lvl;           // [kMsdMaxLevels]
+  MsdRun* runs;
+  uint32_t* run_ctr;       // [0] runs emitted, [1] finish ticket, [2] overflow
+  uint32_t cap_entries, cap_tiles, cap_runs;
+  uint32_t key_bits;
+  KeyOrder ko;
+  uint32_t level;
+};
+// One kernel each; every launch reads its work counts from the device.
+hipError_t launch_msd_init(const MsdArgs& a, uint32_t grid, hipStream_t s);
+hipError_t launch_msd_level(const MsdArgs& a, uint32_t cus, hipStream_t s);
+hipError_t launch_msd_finish(const MsdArgs& a, uint32_t cus, hipStream_t s);
+
 // Workload generation and verification (lsb_workload.hip).
 hipError_t launch_generate(void* A, uint64_t n, uint32_t kind, uint64_t per,
                            uint64_t seed_base, uint64_t key_mask, uint64_t val_base,

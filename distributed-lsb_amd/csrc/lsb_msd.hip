@@ -1,0 +1,667 @@
+// MSD-first sort of the large segments of a batch.
+//
+// Each segment is one rank's mySort (mpi/mpi_lsbsort.cpp:591-597); its
+// result must be the stable sort by key, which any stable sorting order
+// produces.  Instead of the reference's fixed sequence of low-to-high digit
+// passes over the whole array, a range is partitioned on its highest digit
+// that still varies, with the same stable onesweep tile pass as the LSD
+// sort, and each resulting bucket is either small enough to be finished
+// inside one workgroup (all its lower digits sorted in LDS, one HBM read and
+// write) or partitioned again on its next varying digit.  Uniform 64-bit
+// keys in arrays of up to ~2^29 elements take two partition passes and the
+// in-LDS finish instead of eight passes.
+//
+// Per level k (launch_msd_level), over that level's list of open ranges:
+//   k_msd_hist     counts each range's partition digit and ORs together
+//                  (key ^ first key) over it, tile by tile; clears the
+//                  look-back words of the level's tiles.
+//   k_msd_plan     one block per range: a range whose keys are all equal is
+//                  done (or copied to A when it sits in B); one whose digit
+//                  is constant moves on to the next level with its next
+//                  varying digit; any other is partitioned (bucket starts =
+//                  exclusive scan of its counts) and its buckets become
+//                  next-level ranges (> kMsdFinishCap elements) or runs of
+//                  consecutive small buckets for the finish.
+//   k_msd_scatter  the partition: persistent onesweep over the level's tiles
+//                  with look-back inside each range; copies the tiles of
+//                  ranges marked for copying.
+// Then k_msd_finish sorts every run in LDS by the digits it still needs.
+#include "lsb_device.hpp"
+#include "lsb_internal.hpp"
+#include "lsb_onesweep.hpp"
+
+namespace lsb {
+
+namespace {
+
+constexpr int kNB = 256;  // 8-bit digits
+
+__device__ __forceinline__ uint64_t low_bits(uint32_t bits) {
+  return bits >= 64 ? ~0ull : ((1ull << bits) - 1ull);
+}
+
+// Mask of digit p (8 bits, the last one cut to the key width).
+__device__ __forceinline__ uint32_t digit_mask(uint32_t p, uint32_t key_bits) {
+  const uint32_t sh = p * 8u;
+  const uint32_t w = key_bits - sh < 8u ? key_bits - sh : 8u;
+  return (1u << w) - 1u;
+}
+
+// Highest digit below `below` with a set bit in x, or -1.
+__device__ __forceinline__ int top_varying_digit(uint64_t x, int below) {
+  for (int p = below - 1; p >= 0; --p)
+    if ((x >> (8 * p)) & 0xFFull) return p;
+  return -1;
+}
+
+__device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+// Appends one range to level k1's list (a single thread): its entry, its
+// zeroed counters and its tiles in the level's tile map.
+__device__ void msd_emit(const MsdArgs& a, uint32_t k1, uint64_t ea, uint64_t eb, uint32_t n,
+                         uint32_t dig, uint32_t flags) {
+  if (k1 >= (uint32_t)kMsdMaxLevels) {
+    atomicAdd(&a.run_ctr[2], 1u);
+    return;
+  }
+  const uint32_t N = k1 & 1u;
+  MsdLevel* lv = &a.lvl[k1];
+  const uint32_t nt = (n + kMsdTile - 1) / kMsdTile;
+  const uint32_t id = atomicAdd(&lv->entries, 1u);
+  const uint32_t tb = atomicAdd(&lv->tiles, nt);
+  if (id >= a.cap_entries || (uint64_t)tb + nt > a.cap_tiles) {
+    atomicAdd(&a.run_ctr[2], 1u);  // cannot happen within the workspace's capacities
+    if (id < a.cap_entries) {
+      MsdEntry dead = {ea, eb, 0u, 0xFFFFFFFFu, 0u, 0u};
+      a.ent[N][id] = dead;
+    }
+    return;
+  }
+  MsdEntry e = {ea, eb, n, tb, dig, flags};
+  a.ent[N][id] = e;
+  uint4* c = reinterpret_cast<uint4*>(a.counts[N] + (size_t)id * kNB);
+  for (int i = 0; i < kNB / 4; ++i) c[i] = make_uint4(0u, 0u, 0u, 0u);
+  a.orr[N][2 * (size_t)id] = 0u;
+  a.orr[N][2 * (size_t)id + 1] = 0u;
+  for (uint32_t i = 0; i < nt; ++i) a.tilemap[N][tb + i] = id;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------- k_msd_init
+// Level 0 = the batch's large segments (host table), partition digit = the
+// key's top digit; clears every level's counters and the run counters.
+__global__ __launch_bounds__(256) void k_msd_init(MsdArgs a) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t nl = umin(a.hdr->n_large, a.cap_entries);
+  const uint32_t P = (a.key_bits + 7u) / 8u;
+  if (blockIdx.x == 0) {
+    if (t < (uint32_t)kMsdMaxLevels) {
+      MsdLevel z = {0u, 0u, 0u, 0u};
+      if (t == 0) {
+        z.entries = nl;
+        z.tiles = umin(a.hdr->total_tiles, a.cap_tiles);
+      }
+      a.lvl[t] = z;
+    }
+    if (t < 4) a.run_ctr[t] = 0u;
+  }
+  for (uint32_t s = blockIdx.x; s < nl; s += gridDim.x) {
+    const SegLarge L = a.large[s];
+    const uint32_t nt = (L.n + kMsdTile - 1) / kMsdTile;
+    if (t == 0) {
+      MsdEntry e = {L.a, L.b, L.n, L.tile_begin, P - 1u, 0u};
+      a.ent[0][s] = e;
+    }
+    a.counts[0][(size_t)s * kNB + t] = 0u;
+    if (t < 2) a.orr[0][2 * (size_t)s + t] = 0u;
+    for (uint32_t i = t; i < nt; i += 256)
+      if (L.tile_begin + i < a.cap_tiles) a.tilemap[0][L.tile_begin + i] = s;
+  }
+}
+
+// ----------------------------------------------------------- k_msd_hist
+// Each block owns a contiguous range of the level's tiles, counts each
+// range's partition digit in LDS and ORs (key ^ the range's first key),
+// flushing both when the range changes.  A warp whose 64 keys share one
+// digit adds them from one lane.
+__global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
+  constexpr int U = 8;
+  const uint32_t k = a.level, L = k & 1u;
+  const uint32_t t = threadIdx.x;
+  const uint32_t tiles = umin(a.lvl[k].tiles, a.cap_tiles);
+  const uint32_t nent = umin(a.lvl[k].entries, a.cap_entries);
+  const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
+  const uint32_t t0 = blockIdx.x * per;
+  const uint32_t t1 = umin(t0 + per, tiles);
+  if (t0 >= t1) return;
+  __shared__ uint32_t h[kNB];
+  __shared__ uint32_t s_or[2];
+  {  // the level's scatter starts from clear look-back words
+    uint4* z = reinterpret_cast<uint4*>(a.status + (size_t)t0 * kNB);
+    const size_t z4 = (size_t)(t1 - t0) * kNB / 4;
+    for (size_t i = t; i < z4; i += 256) z[i] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  h[t] = 0u;
+  if (t < 2) s_or[t] = 0u;
+  __syncthreads();
+  const uint64_t kbm = low_bits(a.key_bits);
+  uint32_t cur = a.tilemap[L][t0];
+  uint64_t acc = 0;
+  auto flush = [&](uint32_t e) {
+    acc &= kbm;
+    if ((uint32_t)acc) atomicOr(&s_or[0], (uint32_t)acc);
+    if ((uint32_t)(acc >> 32)) atomicOr(&s_or[1], (uint32_t)(acc >> 32));
+    acc = 0;
+    __syncthreads();
+    const uint32_t c = h[t];
+    if (c && e < nent) atomicAdd(&a.counts[L][(size_t)e * kNB + t], c);
+    h[t] = 0u;
+    if (t < 2) {
+      if (s_or[t] && e < nent) atomicOr(&a.orr[L][2 * (size_t)e + t], s_or[t]);
+      s_or[t] = 0u;
+    }
+    __syncthreads();
+  };
+  for (uint32_t tl = t0; tl < t1; ++tl) {
+    const uint32_t e = a.tilemap[L][tl];
+    if (e != cur) {
+      flush(cur);
+      cur = e;
+    }
+    if (e >= nent) continue;
+    const MsdEntry E = a.ent[L][e];
+    if (E.flags & kMsdCopy) continue;
+    if (tl < E.tile_begin || (uint64_t)(tl - E.tile_begin) * kMsdTile >= E.n) continue;
+    const elem_t* src = reinterpret_cast<const elem_t*>((E.flags & kMsdInB) ? E.b : E.a);
+    const uint64_t ref = key_fwd(src[0].key, a.ko);
+    const uint32_t shift = E.dig * 8u, mask = digit_mask(E.dig, a.key_bits);
+    const uint32_t first = (tl - E.tile_begin) * kMsdTile;
+    const uint32_t cnt = umin(kMsdTile, E.n - first);
+    const elem_t* p = src + first;
+    if (cnt == kMsdTile) {
+      for (uint32_t i0 = 0; i0 < cnt; i0 += 256 * U) {
+        uint64_t kk[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) kk[u] = key_fwd(p[i0 + u * 256 + t].key, a.ko);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          acc |= kk[u] ^ ref;
+          const uint32_t d = digit_of(kk[u], shift, mask);
+          const uint32_t f = __builtin_amdgcn_readfirstlane(d);
+          if (__ballot(d != f) == 0) {
+            if (lane_id() == 0) atomicAdd(&h[f], (uint32_t)kWave);
+          } else {
+            atomicAdd(&h[d], 1u);
+          }
+        }
+      }
+    } else {
+      for (uint32_t i0 = 0; i0 < cnt; i0 += 256) {
+        const uint32_t i = i0 + t;
+        const bool v = i < cnt;
+        const uint64_t kk = v ? key_fwd(p[i].key, a.ko) : 0ull;
+        if (v) acc |= kk ^ ref;
+        hist_add(h, digit_of(kk, shift, mask), v);
+      }
+    }
+  }
+  flush(cur);
+}
+
+// ----------------------------------------------------------- k_msd_plan
+// One block per range; thread b owns bucket b of a partitioned range.
+__global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
+  constexpr uint32_t C = kMsdFinishCap;
+  const uint32_t k = a.level, L = k & 1u;
+  const uint32_t t = threadIdx.x;
+  const uint32_t nent = umin(a.lvl[k].entries, a.cap_entries);
+  const uint64_t kbm = low_bits(a.key_bits);
+  __shared__ uint32_t s_start[kNB];   // element offset of each bucket
+  __shared__ uint32_t s_q[kNB + 1];   // exclusive prefix of min(count, C + 1)
+  __shared__ uint32_t s_nl[kNB + 1];  // exclusive prefix of buckets of <= C elements
+  __shared__ uint32_t s_emit[kNB];    // exclusive prefix of (tiles << 9 | 1) of next-level buckets
+  __shared__ uint32_t s_next[kNB];    // end of a run that starts at b; for a large b, the next
+                                      // small bucket | 0x80000000
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_rlo[kNB], s_rn[kNB];
+  __shared__ uint32_t s_misc[4];
+  for (uint32_t e = blockIdx.x; e < nent; e += gridDim.x) {
+    const MsdEntry E = a.ent[L][e];
+    if ((E.flags & kMsdCopy) || E.n == 0) continue;
+    const uint32_t rem = umin(8u * (E.dig + 1u), a.key_bits);
+    const uint64_t x = ((uint64_t)a.orr[L][2 * (size_t)e] |
+                        ((uint64_t)a.orr[L][2 * (size_t)e + 1] << 32)) & kbm & low_bits(rem);
+    const bool in_b = (E.flags & kMsdInB) != 0;
+    if (x == 0) {
+      // every key equal: done where it is, or copied to A by this level's scatter
+      if (in_b && t == 0) a.ent[L][e].flags = E.flags | kMsdCopy;
+      continue;
+    }
+    if (((x >> (8u * E.dig)) & 0xFFull) == 0) {
+      // this digit is constant: the next level partitions on the next varying one
+      if (t == 0) msd_emit(a, k + 1, E.a, E.b, E.n, (uint32_t)top_varying_digit(x, (int)E.dig),
+                           E.flags & kMsdInB);
+      continue;
+    }
+    // Partition on E.dig.  Buckets of more than C elements become ranges of
+    // the next level (with the next varying digit, or, when no digit below
+    // varies and they land in B, as copies); the others are finished in runs.
+    const int cd = top_varying_digit(x, (int)E.dig);
+    const bool out_b = !in_b;
+    const uint32_t c = a.counts[L][(size_t)e * kNB + t];
+    const bool large = c > C;
+    const bool emit = large && (cd >= 0 || out_b);
+    const uint32_t nt = (c + kMsdTile - 1) / kMsdTile;
+    const uint32_t my_q = large ? C + 1 : c, my_nl = large ? 0u : 1u;
+    const uint32_t my_emit = emit ? ((nt << 9) | 1u) : 0u;
+    s_start[t] = c;
+    s_q[t] = my_q;
+    s_nl[t] = my_nl;
+    s_emit[t] = my_emit;
+    block_exclusive_scan<kNB, 256, 0, true>(s_start, s_wsum);
+    block_exclusive_scan<kNB, 256, 0, true>(s_q, s_wsum);
+    block_exclusive_scan<kNB, 256, 0, true>(s_nl, s_wsum);
+    block_exclusive_scan<kNB, 256, 0, true>(s_emit, s_wsum);
+    const uint32_t start = s_start[t];
+    a.counts[L][(size_t)e * kNB + t] = start;
+    int any_small = (!large && c > 0) ? 1 : 0;
+    if (t == kNB - 1) {
+      s_q[kNB] = s_q[t] + my_q;
+      s_nl[kNB] = s_nl[t] + my_nl;
+      const uint32_t tot = s_emit[t] + my_emit;
+      uint32_t be = 0, bt = 0;
+      if (tot && k + 1 < (uint32_t)kMsdMaxLevels) {
+        be = atomicAdd(&a.lvl[k + 1].entries, tot & 511u);
+        bt = atomicAdd(&a.lvl[k + 1].tiles, tot >> 9);
+      } else if (tot) {
+        atomicAdd(&a.run_ctr[2], 1u);
+      }
+      s_misc[0] = be;
+      s_misc[1] = bt;
+      a.ent[L][e].flags = E.flags | kMsdMove;
+    }
+    any_small = __syncthreads_or(any_small);
+    if (emit && k + 1 < (uint32_t)kMsdMaxLevels) {
+      const uint32_t N = (k + 1) & 1u;
+      const uint32_t id = s_misc[0] + (s_emit[t] & 511u);
+      const uint32_t tb = s_misc[1] + (s_emit[t] >> 9);
+      const uint64_t ca = E.a + 16ull * start, cb = E.b + 16ull * start;
+      if (id >= a.cap_entries || (uint64_t)tb + nt > a.cap_tiles) {
+        atomicAdd(&a.run_ctr[2], 1u);  // cannot happen within the workspace's capacities
+        if (id < a.cap_entries) {
+          MsdEntry dead = {ca, cb, 0u, 0xFFFFFFFFu, 0u, 0u};
+          a.ent[N][id] = dead;
+        }
+      } else {
+        MsdEntry ch = {ca, cb, c, tb, cd >= 0 ? (uint32_t)cd : 0u,
+                       (out_b ? kMsdInB : 0u) | (cd >= 0 ? 0u : kMsdCopy)};
+        a.ent[N][id] = ch;
+        uint4* z = reinterpret_cast<uint4*>(a.counts[N] + (size_t)id * kNB);
+        for (int i = 0; i < kNB / 4; ++i) z[i] = make_uint4(0u, 0u, 0u, 0u);
+        a.orr[N][2 * (size_t)id] = 0u;
+        a.orr[N][2 * (size_t)id + 1] = 0u;
+        for (uint32_t i = 0; i < nt; ++i) a.tilemap[N][tb + i] = id;
+      }
+    }
+    // Runs: consecutive buckets of at most C elements packed greedily up to
+    // C elements; a larger bucket ends a run.  Each run is sorted by E.dig
+    // (so its buckets keep their order) and the digits below it that vary
+    // in the range; with none varying its buckets are already sorted, and
+    // it is a copy when they sit in B.
+    if (!any_small || (cd < 0 && !out_b)) {
+      __syncthreads();  // the shared arrays are rewritten for the next range
+      continue;
+    }
+    uint32_t digits = 0;
+    if (cd >= 0) {
+      digits = 1u << E.dig;
+      for (int p = 0; p < (int)E.dig; ++p)
+        if ((x >> (8 * p)) & 0xFFull) digits |= 1u << p;
+    }
+    // Where a run starting at bucket t ends: the last bucket boundary j with
+    // at most C elements in [t, j) and no large bucket among them (a large
+    // bucket counts C + 1); for a large t, the next small bucket.
+    if (large) {
+      // smallest m in (t + 1, kNB] with s_nl[m] > s_nl[t + 1]: bucket m - 1
+      const uint32_t want = s_nl[t + 1] + 1u;
+      uint32_t lo = t + 2, hi = kNB + 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (s_nl[mid] >= want) hi = mid;
+        else lo = mid + 1;
+      }
+      s_next[t] = 0x80000000u | (lo <= (uint32_t)kNB ? lo - 1 : (uint32_t)kNB);
+    } else {
+      const uint32_t base = s_q[t];
+      uint32_t lo = t + 1, hi = kNB;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (s_q[mid] - base <= C) lo = mid;
+        else hi = mid - 1;
+      }
+      s_next[t] = lo;
+    }
+    __syncthreads();
+    if (t == 0) {
+      uint32_t R = 0;
+      for (uint32_t b = 0; b < (uint32_t)kNB;) {
+        const uint32_t v = s_next[b];
+        if (v & 0x80000000u) {
+          b = v & 0x1FFu;
+          continue;
+        }
+        const uint32_t rn = s_q[v] - s_q[b];
+        if (rn) {
+          s_rlo[R] = s_start[b];
+          s_rn[R] = rn;
+          ++R;
+        }
+        b = v;
+      }
+      uint32_t base = R ? atomicAdd(&a.run_ctr[0], R) : 0u;
+      if (R && (uint64_t)base + R > a.cap_runs) {
+        atomicAdd(&a.run_ctr[2], 1u);  // cannot happen within the workspace's capacities
+        R = 0;
+      }
+      s_misc[2] = R;
+      s_misc[3] = base;
+    }
+    __syncthreads();
+    const uint32_t R = s_misc[2], rbase = s_misc[3];
+    const uint64_t from = out_b ? E.b : E.a;
+    for (uint32_t r = t; r < R; r += 256) {
+      MsdRun run = {from + 16ull * s_rlo[r], E.a + 16ull * s_rlo[r], s_rn[r], digits};
+      a.runs[rbase + r] = run;
+    }
+    __syncthreads();
+  }
+}
+
+// -------------------------------------------------------- k_msd_scatter
+template <int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_msd_scatter(MsdArgs a) {
+  static_assert(BLOCK * ITEMS == (int)kMsdTile, "one onesweep tile per level tile");
+  __shared__ TileSmem<8, BLOCK, ITEMS> sm;
+  constexpr bool kOwn = BLOCK >= 512;
+  __shared__ uint32_t s_whist[kOwn ? (BLOCK / kWave) * kNB : 1];
+  const uint32_t k = a.level, L = k & 1u;
+  const uint32_t tiles = umin(a.lvl[k].tiles, a.cap_tiles);
+  const uint32_t nent = umin(a.lvl[k].entries, a.cap_entries);
+  uint32_t* ticket = &a.lvl[k].ticket;
+  if (threadIdx.x == 0) sm.scalar[3] = atomicAdd(ticket, 1u);
+  __syncthreads();
+  while (true) {
+    const uint32_t tl = sm.scalar[3];
+    if (tl >= tiles) return;
+    const uint32_t e = a.tilemap[L][tl];
+    bool took = false;
+    if (e < nent) {
+      const MsdEntry E = a.ent[L][e];
+      const uint32_t local = tl - E.tile_begin;
+      if (tl >= E.tile_begin && (uint64_t)local * kMsdTile < E.n) {
+        if (E.flags & kMsdMove) {
+          const bool in_b = (E.flags & kMsdInB) != 0;
+          const elem_t* src = reinterpret_cast<const elem_t*>(in_b ? E.b : E.a);
+          elem_t* dst = reinterpret_cast<elem_t*>(in_b ? E.a : E.b);
+          onesweep_tile<8, BLOCK, ITEMS>(sm, src, dst, E.n, local,
+                                         a.status + (size_t)E.tile_begin * kNB, nullptr,
+                                         a.counts[L] + (size_t)e * kNB, E.dig * 8u,
+                                         digit_mask(E.dig, a.key_bits), a.ko, ticket,
+                                         kOwn ? s_whist : nullptr);
+          took = true;
+        } else if (E.flags & kMsdCopy) {
+          const uint32_t first = local * kMsdTile;
+          const uint32_t cnt = umin(kMsdTile, E.n - first);
+          const elem_t* src = reinterpret_cast<const elem_t*>(E.b) + first;
+          elem_t* dst = reinterpret_cast<elem_t*>(E.a) + first;
+          for (uint32_t i = threadIdx.x; i < cnt; i += BLOCK) dst[i] = src[i];
+        }
+      }
+    }
+    if (!took && threadIdx.x == 0) sm.scalar[3] = atomicAdd(ticket, 1u);
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------- k_msd_finish
+// A run is sorted in LDS by LSD passes over the (at most) four highest of its
+// digits.  Keys that still tie on everything from the lowest sorted digit up
+// form groups, already in input order; a group whose lower bits are out of
+// order is put in order by one thread (insertion sort, stable) when it holds
+// at most kFixGroup elements.  For uniform keys a run's groups are a few
+// pairs at most; long groups are mostly equal keys, already in order.  A
+// descent inside a longer group sends the run through LDS passes over every
+// one of its digits instead.
+constexpr uint32_t kFixGroup = 32;
+
+// One stable LDS pass: ranks e[] by the digit at shift/mask and writes the
+// elements to sm.elems in digit order.  Returns false (nothing written) when
+// all n elements share one bucket.  whist: the waves' own counter rows.
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ bool lds_pass(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t* whist,
+                                         const elem_t (&e)[ITEMS], uint32_t lim, uint32_t n,
+                                         uint32_t shift, uint32_t mask) {
+  constexpr int W = BLOCK / kWave;
+  static_assert(BLOCK >= kNB, "one bucket per thread");
+  const uint32_t t = threadIdx.x;
+  const uint32_t lane = t & (kWave - 1);
+  const uint32_t wave = t / kWave;
+  uint32_t* wh = whist + wave * kNB;
+  // Only this wave has touched its row since the last pass's scan.
+  for (uint32_t i = lane; i < (uint32_t)kNB; i += kWave) wh[i] = 0u;
+  uint32_t dg[ITEMS], rk[ITEMS];
+  wave_rank<8, ITEMS>(e, lim, shift, mask, wh, dg, rk);
+  lds_barrier();
+  int one = 0;
+  if (t < (uint32_t)kNB) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const uint32_t c = whist[w * kNB + t];
+      whist[w * kNB + t] = s;
+      s += c;
+    }
+    sm.start[t] = s;
+    one = s == n;
+  }
+  one = __syncthreads_or(one);
+  // adds each bucket's start into every wave's row: slot = wh[digit] + rank
+  block_exclusive_scan<kNB, BLOCK, W, true>(sm.start, sm.wsum, whist);
+  if (one) return false;
+  uint32_t pos[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) pos[i] = wh[dg[i]] + rk[i];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if ((uint32_t)i * kWave + lane < lim) sm.elems[pos[i]] = e[i];
+  return true;
+}
+
+// LSD passes over the digits in `digits`, lowest first, from e[]; on return
+// sm.elems[0..n) holds the elements in the resulting order.
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ void lds_sort(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t* whist,
+                                         elem_t (&e)[ITEMS], uint32_t lim, uint32_t n,
+                                         uint32_t key_bits, uint32_t digits) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t wrun = (threadIdx.x / kWave) * (kWave * ITEMS);
+  bool in_lds = false;  // sm.elems holds the current order (else e[] does)
+  for (uint32_t p = 0; p < 8; ++p) {
+    if (!((digits >> p) & 1u)) continue;
+    if (in_lds) {
+      lds_barrier();
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i)
+        if ((uint32_t)i * kWave + lane < lim) e[i] = sm.elems[wrun + (uint32_t)i * kWave + lane];
+      in_lds = false;
+    }
+    if (lds_pass<BLOCK, ITEMS>(sm, whist, e, lim, n, p * 8u, digit_mask(p, key_bits)))
+      in_lds = true;
+  }
+  if (!in_lds) {
+    lds_barrier();
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i)
+      if ((uint32_t)i * kWave + lane < lim) sm.elems[wrun + (uint32_t)i * kWave + lane] = e[i];
+  }
+  lds_barrier();
+}
+
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ void finish_run(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t* whist,
+                                           const elem_t* src, elem_t* dst, uint32_t n,
+                                           uint32_t key_bits, uint32_t digits,
+                                           const KeyOrder& ko) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t lane = t & (kWave - 1);
+  const uint32_t wrun = (t / kWave) * (kWave * ITEMS);
+  const uint32_t lim = n > wrun ? n - wrun : 0u;
+  const uint64_t kbm = low_bits(key_bits);
+  elem_t e[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    if ((uint32_t)i * kWave + lane < lim) {
+      e[i] = src[wrun + (uint32_t)i * kWave + lane];
+    } else {
+      e[i].key = 0;
+      e[i].val = 0;
+    }
+  }
+  keys_fwd<ITEMS>(e, ko);
+  uint32_t top = 0, ntop = 0;
+  for (int p = 7; p >= 0 && ntop < 4; --p)
+    if ((digits >> p) & 1u) {
+      top |= 1u << p;
+      ++ntop;
+    }
+  lds_sort<BLOCK, ITEMS>(sm, whist, e, lim, n, key_bits, top);
+  if (digits & ~top) {
+    // Groups: neighbours equal from the lowest sorted digit up.
+    const uint32_t cut = 8u * (uint32_t)__builtin_ctz(top);
+    auto hi = [&](uint32_t i) { return (sm.elems[i].key & kbm) >> cut; };
+    int desc = 0;
+    for (uint32_t p = t + 1; p < n; p += BLOCK) {
+      const uint64_t ka = sm.elems[p - 1].key & kbm, kb = sm.elems[p].key & kbm;
+      if (((ka ^ kb) >> cut) == 0 && kb < ka) desc = 1;
+    }
+    if (__syncthreads_or(desc)) {
+      // a descent in a group longer than kFixGroup: sort every digit instead
+      int longd = 0;
+      for (uint32_t p = t + 1; p < n; p += BLOCK) {
+        const uint64_t ka = sm.elems[p - 1].key & kbm, kb = sm.elems[p].key & kbm;
+        if (((ka ^ kb) >> cut) != 0 || kb >= ka) continue;
+        const uint64_t g = kb >> cut;
+        uint32_t lo = p, hi2 = p + 1;
+        while (lo > 0 && p - lo <= kFixGroup && hi(lo - 1) == g) --lo;
+        while (hi2 < n && hi2 - lo <= kFixGroup && hi(hi2) == g) ++hi2;
+        if (hi2 - lo > kFixGroup) longd = 1;
+      }
+      if (__syncthreads_or(longd)) {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+          if ((uint32_t)i * kWave + lane < lim) e[i] = sm.elems[wrun + (uint32_t)i * kWave + lane];
+        lds_sort<BLOCK, ITEMS>(sm, whist, e, lim, n, key_bits, digits);
+      } else {
+        for (uint32_t p = t; p < n; p += BLOCK) {
+          const uint64_t g = hi(p);
+          if (p > 0 && hi(p - 1) == g) continue;      // not a group start
+          if (p + 1 >= n || hi(p + 1) != g) continue;  // a group of one
+          uint32_t q = p + 1;
+          while (q < n && q - p <= kFixGroup && hi(q) == g) ++q;
+          if (q - p > kFixGroup) continue;  // long: no descent inside, already in order
+          for (uint32_t i = p + 1; i < q; ++i) {
+            const elem_t x = sm.elems[i];
+            const uint64_t kx = x.key & kbm;
+            uint32_t j = i;
+            while (j > p && (sm.elems[j - 1].key & kbm) > kx) {
+              sm.elems[j] = sm.elems[j - 1];
+              --j;
+            }
+            sm.elems[j] = x;
+          }
+        }
+        lds_barrier();
+      }
+    }
+  }
+  // With src == dst no element may be stored before every thread's loads of
+  // it have completed: a full barrier (waits for this thread's loads too).
+  __syncthreads();
+  for (uint32_t p = t; p < n; p += BLOCK) {
+    elem_t x = sm.elems[p];
+    x.key = key_inv(x.key, ko);
+    dst[p] = x;
+  }
+}
+
+// Sorts every run inside one workgroup (runs handed out by a ticket, so
+// uneven runs balance across the grid).
+template <int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_msd_finish(MsdArgs a) {
+  static_assert(BLOCK * ITEMS >= (int)kMsdFinishCap, "a run fits one workgroup");
+  __shared__ TileSmem<8, BLOCK, ITEMS> sm;
+  __shared__ uint32_t s_whist[(BLOCK / kWave) * kNB];
+  const uint32_t nr = umin(a.run_ctr[0], a.cap_runs);
+  while (true) {
+    if (threadIdx.x == 0) sm.scalar[1] = atomicAdd(&a.run_ctr[1], 1u);
+    __syncthreads();
+    const uint32_t r = sm.scalar[1];
+    __syncthreads();  // every thread has read the ticket before the next one is taken
+    if (r >= nr) return;
+    const MsdRun R = a.runs[r];
+    if (R.digits == 0) {
+      const elem_t* src = reinterpret_cast<const elem_t*>(R.src);
+      elem_t* dst = reinterpret_cast<elem_t*>(R.dst);
+      for (uint32_t i = threadIdx.x; i < R.n; i += BLOCK) dst[i] = src[i];
+    } else {
+      finish_run<BLOCK, ITEMS>(sm, s_whist, reinterpret_cast<const elem_t*>(R.src),
+                               reinterpret_cast<elem_t*>(R.dst), R.n, a.key_bits, R.digits,
+                               a.ko);
+    }
+    __syncthreads();
+  }
+}
+
+// ----------------------------------------------------------------- launch
+namespace {
+
+template <class K>
+uint32_t resident_blocks(K kernel, int block) {
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    occ = 1;
+  }
+  return occ > 0 ? (uint32_t)occ : 1u;
+}
+
+constexpr int kScatterBlock = 1024, kScatterItems = 8;
+constexpr int kFinishBlock = 1024, kFinishItems = 8;
+
+}  // namespace
+
+hipError_t launch_msd_init(const MsdArgs& a, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_msd_init, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_msd_level(const MsdArgs& a, uint32_t cus, hipStream_t s) {
+  static const uint32_t scatter_occ =
+      resident_blocks(k_msd_scatter<kScatterBlock, kScatterItems>, kScatterBlock);
+  hipLaunchKernelGGL(k_msd_hist, dim3(2 * cus), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_msd_plan, dim3(4 * cus), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_msd_scatter<kScatterBlock, kScatterItems>), dim3(scatter_occ * cus),
+                     dim3(kScatterBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_msd_finish(const MsdArgs& a, uint32_t cus, hipStream_t s) {
+  static const uint32_t occ =
+      resident_blocks(k_msd_finish<kFinishBlock, kFinishItems>, kFinishBlock);
+  hipLaunchKernelGGL((k_msd_finish<kFinishBlock, kFinishItems>), dim3(occ * cus),
+                     dim3(kFinishBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace lsb

@@ -309,7 +309,10 @@ __device__ __forceinline__ void onesweep_tile_regs(TileSmem<D, BLOCK, ITEMS>& sm
   } else {
     for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
   }
-  for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) st_next[(size_t)tile * NB + b] = 0u;
+  // The next pass's row of this tile starts clear (null: the caller clears
+  // its status rows itself).
+  if (st_next)
+    for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) st_next[(size_t)tile * NB + b] = 0u;
 
   const uint32_t tile_base = tile * (uint32_t)T;
   const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
@@ -556,11 +559,14 @@ __device__ __forceinline__ void onesweep_tile(TileSmem<D, BLOCK, ITEMS>& sm,
 // Sorts n <= BLOCK*ITEMS elements of one array entirely inside the block:
 // every digit pass ranks the elements held in registers, writes them to
 // LDS in digit order and reloads them in the wave-contiguous layout.  Only
-// one read and one write of HBM per element, whatever the pass count.
-// Digits whose elements all share one bucket are skipped.
+// one read of src and one write of dst per element, whatever the pass count
+// (src may equal dst: every element is loaded before the first barrier and
+// stored after the last).  Digits p with bit p of `digits` set are sorted,
+// lowest first; digits whose elements all share one bucket are skipped.
 template <int D, int BLOCK, int ITEMS>
-__device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, elem_t* __restrict__ a,
-                                           uint32_t n, uint32_t key_bits, const KeyOrder& ko) {
+__device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, const elem_t* src,
+                                           elem_t* dst, uint32_t n, uint32_t key_bits,
+                                           uint32_t digits, const KeyOrder& ko) {
   using S = TileSmem<D, BLOCK, ITEMS>;
   constexpr int NB = S::NB, W = S::W;
   const uint32_t t = threadIdx.x;
@@ -574,7 +580,7 @@ __device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, elem_t
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     if ((uint32_t)i * kWave + lane < lim) {
-      e[i] = a[wrun + (uint32_t)i * kWave + lane];
+      e[i] = src[wrun + (uint32_t)i * kWave + lane];
     } else {
       e[i].key = 0;
       e[i].val = 0;
@@ -583,6 +589,7 @@ __device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, elem_t
   keys_fwd<ITEMS>(e, ko);
   const uint32_t passes = (key_bits + D - 1) / D;
   for (uint32_t p = 0; p < passes; ++p) {
+    if (!((digits >> p) & 1u)) continue;
     const uint32_t shift = p * D;
     const uint32_t w = key_bits - shift < (uint32_t)D ? key_bits - shift : (uint32_t)D;
     const uint32_t mask = (1u << w) - 1u;
@@ -623,9 +630,19 @@ __device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, elem_t
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) e[i].key = key_inv(e[i].key, ko);
   }
+  // With src == dst no element may be stored before every thread's loads of
+  // it have completed: a full barrier (waits for this thread's loads too).
+  __syncthreads();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
-    if ((uint32_t)i * kWave + lane < lim) a[wrun + (uint32_t)i * kWave + lane] = e[i];
+    if ((uint32_t)i * kWave + lane < lim) dst[wrun + (uint32_t)i * kWave + lane] = e[i];
+}
+
+// Whole sort of one array in place: every digit of the key.
+template <int D, int BLOCK, int ITEMS>
+__device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, elem_t* a, uint32_t n,
+                                           uint32_t key_bits, const KeyOrder& ko) {
+  small_sort<D, BLOCK, ITEMS>(sm, a, a, n, key_bits, ~0u, ko);
 }
 
 }  // namespace lsb

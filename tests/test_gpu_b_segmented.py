@@ -376,3 +376,78 @@ def test_randomized_batches():
             for s in range(nseg):
                 got = from_dev(As[s])[:sizes[s]]
                 assert np.array_equal(got, wants[s]), (case, "batch", s, sizes[s])
+
+
+def _msd_shapes(seed=0):
+    """Segments that take every branch of the MSD-first form: ranges whose
+    keys are all equal (in A, and as large buckets left in B), constant
+    digits above or between varying ones (pass-through levels), heavy
+    duplicates, buckets just above and below one workgroup's capacity,
+    sorted and reversed input, and 2^20 / 2^22-element segments (two
+    partition levels)."""
+    rng = np.random.default_rng(seed)
+    segs = []
+    n = 9000
+    a = O.gen_uniform(n, ranks=1, seed_base=seed + 1)
+    a["key"] = np.uint64(0xABCDEF0123456789)                       # all equal
+    segs.append(a)
+    a = O.gen_uniform(200_000, ranks=1, seed_base=seed + 2)
+    a["key"] = (a["key"] % np.uint64(3)) << np.uint64(56)          # top digit only: 3 big equal buckets
+    segs.append(a)
+    a = O.gen_uniform(150_000, ranks=1, seed_base=seed + 3)
+    a["key"] = (a["key"] & np.uint64(0xFFFF)) | np.uint64(0x7700000000000000)  # low digits vary
+    segs.append(a)
+    a = O.gen_uniform(120_000, ranks=1, seed_base=seed + 4)
+    a["key"] = ((a["key"] & np.uint64(0xFF)) << np.uint64(40)) | (a["key"] >> np.uint64(60))
+    segs.append(a)                                                 # digit 5 and bits 0-3 vary
+    a = O.gen_zipf(300_000, ranks=1, m=1 << 8, seed_base=seed + 5)  # heavy duplicates
+    segs.append(a)
+    a = O.gen_uniform(8193, ranks=1, seed_base=seed + 6)
+    segs.append(a)
+    a = O.gen_uniform(16384 * 256, ranks=4, seed_base=seed + 7)    # level-1 buckets of ~16384
+    segs.append(a)
+    a = O.gen_uniform(1 << 20, ranks=2, seed_base=seed + 8)
+    a["key"] = np.arange(1 << 20, dtype=np.uint64) * np.uint64(0x9E3779B1)  # sorted
+    segs.append(a)
+    a = O.gen_uniform(70_000, ranks=1, seed_base=seed + 9)
+    a["key"] = np.arange(70_000, 0, -1, dtype=np.uint64) << np.uint64(20)  # reversed
+    segs.append(a)
+    a = O.gen_uniform(1 << 22, ranks=8, seed_base=seed + 10)
+    segs.append(a)
+    rng.shuffle(segs)
+    return segs
+
+
+@pytest.mark.parametrize("key_order", ["unsigned", "signed"])
+def test_msd_first_branches_equal_lsd_and_oracle(key_order):
+    segs = _msd_shapes(3)
+    sizes = [len(s) for s in segs]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    a = np.concatenate(segs)
+    wants = [O.stable_sort_order(s, 64, key_order) for s in segs]
+    outs = {}
+    for lsd in (False, True):
+        for graph in (False, True):
+            A = to_dev(a)
+            B = torch.empty_like(A)
+            lsb_amd.sort_segmented(A, B, off, key_order=key_order, sync=True, lsd=lsd,
+                                   graph=graph)
+            outs[(lsd, graph)] = from_dev(A)
+    for s in range(len(segs)):
+        lo, hi = int(off[s]), int(off[s + 1])
+        for k, got in outs.items():
+            assert np.array_equal(got[lo:hi], wants[s]), (k, s, sizes[s])
+
+
+@pytest.mark.parametrize("key_bits", [3, 12, 33, 57])
+def test_msd_first_key_widths(key_bits):
+    """Key widths whose top digit is narrower than 8 bits; bits above the
+    key width are carried along but do not order."""
+    sizes = [20_000, 9_000, 300_000, 8192, 100_000]
+    a, off = batch(sizes, seed=key_bits)
+    want = expected(a, off, key_bits)
+    As = [to_dev(a[int(off[i]):int(off[i + 1])]) for i in range(len(sizes))]
+    Bs = [torch.empty_like(x) for x in As]
+    lsb_amd.sort_batch(As, Bs, key_bits=key_bits, sync=True)
+    for i, X in enumerate(As):
+        assert np.array_equal(from_dev(X), want[int(off[i]):int(off[i + 1])]), (key_bits, i)
