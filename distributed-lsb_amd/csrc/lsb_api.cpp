@@ -537,7 +537,7 @@ int make_msd_layout(size_t cap_n, SegLayout* L) {
   for (int i = 0; i < 2; ++i) {
     L->ment[i] = off;    off += align_up(ents * sizeof(lsb::MsdEntry));
     L->mcounts[i] = off; off += align_up(ents * 256 * 4);
-    L->morr[i] = off;    off += align_up(ents * 8);
+    L->morr[i] = off;    off += align_up(ents * 16);
     L->mtile[i] = off;   off += align_up(tiles * 4);
   }
   L->mstatus = off; off += align_up(tiles * 256 * 4);

@@ -206,7 +206,9 @@ struct MsdEntry {
   uint32_t flags;
 };
 // A run: consecutive children of one partitioned entry, sorted in LDS by
-// the digits in `digits` (bit p = digit p), read from src, written to dst.
+// the digits in the low byte of `digits` (bit p = digit p), read from src,
+// written to dst; bits 16+ give how many of the highest of them the LDS
+// passes sort before the tie fix-up.
 struct MsdRun {
   uint64_t src;
   uint64_t dst;
@@ -221,7 +223,7 @@ struct MsdArgs {
   const SegLarge* large;
   MsdEntry* ent[2];        // entries of even / odd levels
   uint32_t* counts[2];     // [entry][256] digit counts, then bucket starts
-  uint32_t* orr[2];        // [entry][2] OR over the entry of (key ^ first key)
+  uint32_t* orr[2];        // [entry][4] OR (lo, hi) and AND (lo, hi) of the entry's keys
   uint32_t* tilemap[2];    // tile -> entry
   uint32_t* status;        // [tile][256] look-back words of the current level
   MsdLevel* lvl;           // [kMsdMaxLevels]

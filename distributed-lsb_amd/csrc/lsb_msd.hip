@@ -12,9 +12,9 @@
 // in-LDS finish instead of eight passes.
 //
 // Per level k (launch_msd_level), over that level's list of open ranges:
-//   k_msd_hist     counts each range's partition digit and ORs together
-//                  (key ^ first key) over it, tile by tile; clears the
-//                  look-back words of the level's tiles.
+//   k_msd_hist     counts each range's partition digit and takes the OR and
+//                  the AND of its keys (their XOR: the bits that vary), tile
+//                  by tile; clears the look-back words of the level's tiles.
 //   k_msd_plan     one block per range: a range whose keys are all equal is
 //                  done (or copied to A when it sits in B); one whose digit
 //                  is constant moves on to the next level with its next
@@ -56,6 +56,18 @@ __device__ __forceinline__ int top_varying_digit(uint64_t x, int below) {
 
 __device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
+// LDS passes for a run of n elements over nb buckets: the partition digit
+// (it keeps the buckets in order) and enough digits below it that a bucket of
+// the average size m = n / nb has few keys left tied for the fix-up: with
+// uniform keys about m^2 / 2^(8d+1) tied pairs per bucket, so d digits with
+// 2^(8d) >= 4m leave under one tie per eight elements.
+__device__ __forceinline__ uint32_t finish_passes(uint32_t n, uint32_t nb) {
+  const uint32_t m = n / (nb ? nb : 1u);
+  uint32_t d = 1;
+  while (d < 3 && (1ull << (8 * d)) < 4ull * m) ++d;
+  return 1u + d;
+}
+
 // Appends one range to level k1's list (a single thread): its entry, its
 // zeroed counters and its tiles in the level's tile map.
 __device__ void msd_emit(const MsdArgs& a, uint32_t k1, uint64_t ea, uint64_t eb, uint32_t n,
@@ -81,8 +93,7 @@ __device__ void msd_emit(const MsdArgs& a, uint32_t k1, uint64_t ea, uint64_t eb
   a.ent[N][id] = e;
   uint4* c = reinterpret_cast<uint4*>(a.counts[N] + (size_t)id * kNB);
   for (int i = 0; i < kNB / 4; ++i) c[i] = make_uint4(0u, 0u, 0u, 0u);
-  a.orr[N][2 * (size_t)id] = 0u;
-  a.orr[N][2 * (size_t)id + 1] = 0u;
+  reinterpret_cast<uint4*>(a.orr[N])[id] = make_uint4(0u, 0u, ~0u, ~0u);
   for (uint32_t i = 0; i < nt; ++i) a.tilemap[N][tb + i] = id;
 }
 
@@ -114,7 +125,7 @@ __global__ __launch_bounds__(256) void k_msd_init(MsdArgs a) {
       a.ent[0][s] = e;
     }
     a.counts[0][(size_t)s * kNB + t] = 0u;
-    if (t < 2) a.orr[0][2 * (size_t)s + t] = 0u;
+    if (t < 4) a.orr[0][4 * (size_t)s + t] = t < 2 ? 0u : ~0u;
     for (uint32_t i = t; i < nt; i += 256)
       if (L.tile_begin + i < a.cap_tiles) a.tilemap[0][L.tile_begin + i] = s;
   }
@@ -122,9 +133,12 @@ __global__ __launch_bounds__(256) void k_msd_init(MsdArgs a) {
 
 // ----------------------------------------------------------- k_msd_hist
 // Each block owns a contiguous range of the level's tiles, counts each
-// range's partition digit in LDS and ORs (key ^ the range's first key),
-// flushing both when the range changes.  A warp whose 64 keys share one
-// digit adds them from one lane.
+// range's partition digit in LDS and accumulates the OR and the AND of its
+// keys (their XOR is the set of bits in which the keys differ), flushing
+// when the range changes.  A wave whose 64 keys share one digit adds them
+// from one lane.  The block's part of the tile map is staged in LDS and a
+// range's entry is read once for its consecutive tiles, so a tile's key
+// loads wait on no other load.
 __global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
   constexpr int U = 8;
   const uint32_t k = a.level, L = k & 1u;
@@ -136,77 +150,94 @@ __global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
   const uint32_t t1 = umin(t0 + per, tiles);
   if (t0 >= t1) return;
   __shared__ uint32_t h[kNB];
-  __shared__ uint32_t s_or[2];
+  __shared__ uint32_t s_or[4];  // OR lo, OR hi, AND lo, AND hi
+  __shared__ uint32_t s_tm[256];
   {  // the level's scatter starts from clear look-back words
     uint4* z = reinterpret_cast<uint4*>(a.status + (size_t)t0 * kNB);
     const size_t z4 = (size_t)(t1 - t0) * kNB / 4;
     for (size_t i = t; i < z4; i += 256) z[i] = make_uint4(0u, 0u, 0u, 0u);
   }
   h[t] = 0u;
-  if (t < 2) s_or[t] = 0u;
-  __syncthreads();
+  if (t < 4) s_or[t] = t < 2 ? 0u : ~0u;
   const uint64_t kbm = low_bits(a.key_bits);
-  uint32_t cur = a.tilemap[L][t0];
-  uint64_t acc = 0;
+  uint32_t cur = 0xFFFFFFFFu;  // range of the counts in h
+  uint64_t acc_or = 0, acc_and = ~0ull;
+  MsdEntry E = {0ull, 0ull, 0u, 0xFFFFFFFFu, 0u, kMsdCopy};
   auto flush = [&](uint32_t e) {
-    acc &= kbm;
-    if ((uint32_t)acc) atomicOr(&s_or[0], (uint32_t)acc);
-    if ((uint32_t)(acc >> 32)) atomicOr(&s_or[1], (uint32_t)(acc >> 32));
-    acc = 0;
+    acc_or &= kbm;
+    acc_and |= ~kbm;
+    if ((uint32_t)acc_or) atomicOr(&s_or[0], (uint32_t)acc_or);
+    if ((uint32_t)(acc_or >> 32)) atomicOr(&s_or[1], (uint32_t)(acc_or >> 32));
+    if (~(uint32_t)acc_and) atomicAnd(&s_or[2], (uint32_t)acc_and);
+    if (~(uint32_t)(acc_and >> 32)) atomicAnd(&s_or[3], (uint32_t)(acc_and >> 32));
+    acc_or = 0;
+    acc_and = ~0ull;
     __syncthreads();
     const uint32_t c = h[t];
     if (c && e < nent) atomicAdd(&a.counts[L][(size_t)e * kNB + t], c);
     h[t] = 0u;
-    if (t < 2) {
-      if (s_or[t] && e < nent) atomicOr(&a.orr[L][2 * (size_t)e + t], s_or[t]);
-      s_or[t] = 0u;
+    if (t < 4) {
+      const uint32_t v = s_or[t], idle = t < 2 ? 0u : ~0u;
+      if (v != idle && e < nent) {
+        if (t < 2) atomicOr(&a.orr[L][4 * (size_t)e + t], v);
+        else atomicAnd(&a.orr[L][4 * (size_t)e + t], v);
+      }
+      s_or[t] = idle;
     }
     __syncthreads();
   };
-  for (uint32_t tl = t0; tl < t1; ++tl) {
-    const uint32_t e = a.tilemap[L][tl];
-    if (e != cur) {
-      flush(cur);
-      cur = e;
-    }
-    if (e >= nent) continue;
-    const MsdEntry E = a.ent[L][e];
-    if (E.flags & kMsdCopy) continue;
-    if (tl < E.tile_begin || (uint64_t)(tl - E.tile_begin) * kMsdTile >= E.n) continue;
-    const elem_t* src = reinterpret_cast<const elem_t*>((E.flags & kMsdInB) ? E.b : E.a);
-    const uint64_t ref = key_fwd(src[0].key, a.ko);
-    const uint32_t shift = E.dig * 8u, mask = digit_mask(E.dig, a.key_bits);
-    const uint32_t first = (tl - E.tile_begin) * kMsdTile;
-    const uint32_t cnt = umin(kMsdTile, E.n - first);
-    const elem_t* p = src + first;
-    if (cnt == kMsdTile) {
-      for (uint32_t i0 = 0; i0 < cnt; i0 += 256 * U) {
-        uint64_t kk[U];
+  for (uint32_t c0 = t0; c0 < t1; c0 += 256) {
+    __syncthreads();  // the previous chunk of the map is consumed
+    if (c0 + t < t1) s_tm[t] = a.tilemap[L][c0 + t];
+    __syncthreads();
+    const uint32_t c1 = umin(c0 + 256, t1);
+    for (uint32_t tl = c0; tl < c1; ++tl) {
+      const uint32_t e = s_tm[tl - c0];
+      if (e != cur) {
+        if (cur != 0xFFFFFFFFu) flush(cur);
+        cur = e;
+        if (e < nent) E = a.ent[L][e];
+      }
+      if (e >= nent || (E.flags & kMsdCopy)) continue;
+      if (tl < E.tile_begin || (uint64_t)(tl - E.tile_begin) * kMsdTile >= E.n) continue;
+      const elem_t* src = reinterpret_cast<const elem_t*>((E.flags & kMsdInB) ? E.b : E.a);
+      const uint32_t shift = E.dig * 8u, mask = digit_mask(E.dig, a.key_bits);
+      const uint32_t first = (tl - E.tile_begin) * kMsdTile;
+      const uint32_t cnt = umin(kMsdTile, E.n - first);
+      const elem_t* p = src + first;
+      if (cnt == kMsdTile) {
+        for (uint32_t i0 = 0; i0 < cnt; i0 += 256 * U) {
+          uint64_t kk[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) kk[u] = key_fwd(p[i0 + u * 256 + t].key, a.ko);
+          for (int u = 0; u < U; ++u) kk[u] = key_fwd(p[i0 + u * 256 + t].key, a.ko);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          acc |= kk[u] ^ ref;
-          const uint32_t d = digit_of(kk[u], shift, mask);
-          const uint32_t f = __builtin_amdgcn_readfirstlane(d);
-          if (__ballot(d != f) == 0) {
-            if (lane_id() == 0) atomicAdd(&h[f], (uint32_t)kWave);
-          } else {
-            atomicAdd(&h[d], 1u);
+          for (int u = 0; u < U; ++u) {
+            acc_or |= kk[u];
+            acc_and &= kk[u];
+            const uint32_t d = digit_of(kk[u], shift, mask);
+            const uint32_t f = __builtin_amdgcn_readfirstlane(d);
+            if (__ballot(d != f) == 0) {
+              if (lane_id() == 0) atomicAdd(&h[f], (uint32_t)kWave);
+            } else {
+              atomicAdd(&h[d], 1u);
+            }
           }
         }
-      }
-    } else {
-      for (uint32_t i0 = 0; i0 < cnt; i0 += 256) {
-        const uint32_t i = i0 + t;
-        const bool v = i < cnt;
-        const uint64_t kk = v ? key_fwd(p[i].key, a.ko) : 0ull;
-        if (v) acc |= kk ^ ref;
-        hist_add(h, digit_of(kk, shift, mask), v);
+      } else {
+        for (uint32_t i0 = 0; i0 < cnt; i0 += 256) {
+          const uint32_t i = i0 + t;
+          const bool v = i < cnt;
+          const uint64_t kk = v ? key_fwd(p[i].key, a.ko) : 0ull;
+          if (v) {
+            acc_or |= kk;
+            acc_and &= kk;
+          }
+          hist_add(h, digit_of(kk, shift, mask), v);
+        }
       }
     }
   }
-  flush(cur);
+  if (cur != 0xFFFFFFFFu) flush(cur);
 }
 
 // ----------------------------------------------------------- k_msd_plan
@@ -224,14 +255,16 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
   __shared__ uint32_t s_next[kNB];    // end of a run that starts at b; for a large b, the next
                                       // small bucket | 0x80000000
   __shared__ uint32_t s_wsum[4];
-  __shared__ uint32_t s_rlo[kNB], s_rn[kNB];
+  __shared__ uint32_t s_rlo[kNB], s_rn[kNB], s_rb[kNB];
   __shared__ uint32_t s_misc[4];
   for (uint32_t e = blockIdx.x; e < nent; e += gridDim.x) {
     const MsdEntry E = a.ent[L][e];
     if ((E.flags & kMsdCopy) || E.n == 0) continue;
     const uint32_t rem = umin(8u * (E.dig + 1u), a.key_bits);
-    const uint64_t x = ((uint64_t)a.orr[L][2 * (size_t)e] |
-                        ((uint64_t)a.orr[L][2 * (size_t)e + 1] << 32)) & kbm & low_bits(rem);
+    // bits in which the range's keys differ: OR of the keys ^ AND of the keys
+    const uint4 oa = reinterpret_cast<const uint4*>(a.orr[L])[e];
+    const uint64_t x = ((((uint64_t)oa.y << 32) | oa.x) ^ (((uint64_t)oa.w << 32) | oa.z)) & kbm &
+                       low_bits(rem);
     const bool in_b = (E.flags & kMsdInB) != 0;
     if (x == 0) {
       // every key equal: done where it is, or copied to A by this level's scatter
@@ -299,8 +332,7 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
         a.ent[N][id] = ch;
         uint4* z = reinterpret_cast<uint4*>(a.counts[N] + (size_t)id * kNB);
         for (int i = 0; i < kNB / 4; ++i) z[i] = make_uint4(0u, 0u, 0u, 0u);
-        a.orr[N][2 * (size_t)id] = 0u;
-        a.orr[N][2 * (size_t)id + 1] = 0u;
+        reinterpret_cast<uint4*>(a.orr[N])[id] = make_uint4(0u, 0u, ~0u, ~0u);
         for (uint32_t i = 0; i < nt; ++i) a.tilemap[N][tb + i] = id;
       }
     }
@@ -355,6 +387,7 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
         if (rn) {
           s_rlo[R] = s_start[b];
           s_rn[R] = rn;
+          s_rb[R] = v - b;
           ++R;
         }
         b = v;
@@ -371,7 +404,8 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
     const uint32_t R = s_misc[2], rbase = s_misc[3];
     const uint64_t from = out_b ? E.b : E.a;
     for (uint32_t r = t; r < R; r += 256) {
-      MsdRun run = {from + 16ull * s_rlo[r], E.a + 16ull * s_rlo[r], s_rn[r], digits};
+      MsdRun run = {from + 16ull * s_rlo[r], E.a + 16ull * s_rlo[r], s_rn[r],
+                    digits | (finish_passes(s_rn[r], s_rb[r]) << 16)};
       a.runs[rbase + r] = run;
     }
     __syncthreads();
@@ -425,14 +459,15 @@ __global__ __launch_bounds__(BLOCK) void k_msd_scatter(MsdArgs a) {
 }
 
 // --------------------------------------------------------- k_msd_finish
-// A run is sorted in LDS by LSD passes over the (at most) four highest of its
-// digits.  Keys that still tie on everything from the lowest sorted digit up
-// form groups, already in input order; a group whose lower bits are out of
-// order is put in order by one thread (insertion sort, stable) when it holds
-// at most kFixGroup elements.  For uniform keys a run's groups are a few
-// pairs at most; long groups are mostly equal keys, already in order.  A
-// descent inside a longer group sends the run through LDS passes over every
-// one of its digits instead.
+// A run is sorted in LDS by LSD passes over the highest few of its digits
+// (as many as its buckets' size needs: finish_passes).  Keys that still tie
+// on everything from the lowest sorted digit up form groups, already in input
+// order; every element of a group of at most kFixGroup finds its place in the
+// group by comparing its lower bits with the other members (ties broken by
+// position, so stable) and is written there directly.  For uniform keys most
+// elements are in no group and the groups are pairs.  A longer group is
+// usually equal keys, already in order; if one is out of order the run goes
+// through LDS passes over every one of its digits instead.
 constexpr uint32_t kFixGroup = 32;
 
 // One stable LDS pass: ranks e[] by the digit at shift/mask and writes the
@@ -530,68 +565,68 @@ __device__ __forceinline__ void finish_run(TileSmem<8, BLOCK, ITEMS>& sm, uint32
   }
   keys_fwd<ITEMS>(e, ko);
   uint32_t top = 0, ntop = 0;
-  for (int p = 7; p >= 0 && ntop < 4; --p)
+  const uint32_t want = digits >> 16;
+  digits &= 0xFFu;
+  for (int p = 7; p >= 0 && ntop < want; --p)
     if ((digits >> p) & 1u) {
       top |= 1u << p;
       ++ntop;
     }
   lds_sort<BLOCK, ITEMS>(sm, whist, e, lim, n, key_bits, top);
+  // Final position of the element at LDS slot t + i * BLOCK.
+  uint32_t f[ITEMS];
+  int bad = 0;
   if (digits & ~top) {
-    // Groups: neighbours equal from the lowest sorted digit up.
     const uint32_t cut = 8u * (uint32_t)__builtin_ctz(top);
-    auto hi = [&](uint32_t i) { return (sm.elems[i].key & kbm) >> cut; };
-    int desc = 0;
-    for (uint32_t p = t + 1; p < n; p += BLOCK) {
-      const uint64_t ka = sm.elems[p - 1].key & kbm, kb = sm.elems[p].key & kbm;
-      if (((ka ^ kb) >> cut) == 0 && kb < ka) desc = 1;
-    }
-    if (__syncthreads_or(desc)) {
-      // a descent in a group longer than kFixGroup: sort every digit instead
-      int longd = 0;
-      for (uint32_t p = t + 1; p < n; p += BLOCK) {
-        const uint64_t ka = sm.elems[p - 1].key & kbm, kb = sm.elems[p].key & kbm;
-        if (((ka ^ kb) >> cut) != 0 || kb >= ka) continue;
-        const uint64_t g = kb >> cut;
-        uint32_t lo = p, hi2 = p + 1;
-        while (lo > 0 && p - lo <= kFixGroup && hi(lo - 1) == g) --lo;
-        while (hi2 < n && hi2 - lo <= kFixGroup && hi(hi2) == g) ++hi2;
-        if (hi2 - lo > kFixGroup) longd = 1;
-      }
-      if (__syncthreads_or(longd)) {
+    auto hi = [&](uint32_t j) { return (sm.elems[j].key & kbm) >> cut; };
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i)
-          if ((uint32_t)i * kWave + lane < lim) e[i] = sm.elems[wrun + (uint32_t)i * kWave + lane];
-        lds_sort<BLOCK, ITEMS>(sm, whist, e, lim, n, key_bits, digits);
-      } else {
-        for (uint32_t p = t; p < n; p += BLOCK) {
-          const uint64_t g = hi(p);
-          if (p > 0 && hi(p - 1) == g) continue;      // not a group start
-          if (p + 1 >= n || hi(p + 1) != g) continue;  // a group of one
-          uint32_t q = p + 1;
-          while (q < n && q - p <= kFixGroup && hi(q) == g) ++q;
-          if (q - p > kFixGroup) continue;  // long: no descent inside, already in order
-          for (uint32_t i = p + 1; i < q; ++i) {
-            const elem_t x = sm.elems[i];
-            const uint64_t kx = x.key & kbm;
-            uint32_t j = i;
-            while (j > p && (sm.elems[j - 1].key & kbm) > kx) {
-              sm.elems[j] = sm.elems[j - 1];
-              --j;
-            }
-            sm.elems[j] = x;
-          }
-        }
-        lds_barrier();
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t p = t + (uint32_t)i * BLOCK;
+      f[i] = p;
+      if (p >= n) continue;
+      const uint64_t k = sm.elems[p].key & kbm, g = k >> cut;
+      const bool tie_l = p > 0 && hi(p - 1) == g;
+      const bool tie_r = p + 1 < n && hi(p + 1) == g;
+      if (!tie_l && !tie_r) continue;
+      uint32_t lo = p, end = p + 1;
+      while (lo > 0 && p - lo < kFixGroup && hi(lo - 1) == g) --lo;
+      while (end < n && end - p <= kFixGroup && hi(end) == g) ++end;
+      const bool whole = (lo == 0 || hi(lo - 1) != g) && (end == n || hi(end) != g);
+      if (!whole || end - lo > kFixGroup) {
+        // a long group: fine only if already in order
+        if (tie_l && k < (sm.elems[p - 1].key & kbm)) bad = 1;
+        continue;
       }
+      uint32_t r = lo;
+      for (uint32_t j = lo; j < end; ++j) {
+        const uint64_t kj = sm.elems[j].key & kbm;
+        r += (kj < k || (kj == k && j < p)) ? 1u : 0u;
+      }
+      f[i] = r;
     }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) f[i] = t + (uint32_t)i * BLOCK;
+  }
+  if (__syncthreads_or(bad)) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i)
+      if ((uint32_t)i * kWave + lane < lim) e[i] = sm.elems[wrun + (uint32_t)i * kWave + lane];
+    lds_sort<BLOCK, ITEMS>(sm, whist, e, lim, n, key_bits, digits);
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) f[i] = t + (uint32_t)i * BLOCK;
   }
   // With src == dst no element may be stored before every thread's loads of
   // it have completed: a full barrier (waits for this thread's loads too).
   __syncthreads();
-  for (uint32_t p = t; p < n; p += BLOCK) {
-    elem_t x = sm.elems[p];
-    x.key = key_inv(x.key, ko);
-    dst[p] = x;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t p = t + (uint32_t)i * BLOCK;
+    if (p < n) {
+      elem_t x = sm.elems[p];
+      x.key = key_inv(x.key, ko);
+      dst[f[i]] = x;
+    }
   }
 }
 
@@ -610,7 +645,7 @@ __global__ __launch_bounds__(BLOCK) void k_msd_finish(MsdArgs a) {
     __syncthreads();  // every thread has read the ticket before the next one is taken
     if (r >= nr) return;
     const MsdRun R = a.runs[r];
-    if (R.digits == 0) {
+    if ((R.digits & 0xFFu) == 0) {
       const elem_t* src = reinterpret_cast<const elem_t*>(R.src);
       elem_t* dst = reinterpret_cast<elem_t*>(R.dst);
       for (uint32_t i = threadIdx.x; i < R.n; i += BLOCK) dst[i] = src[i];

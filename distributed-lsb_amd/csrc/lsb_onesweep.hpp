@@ -26,15 +26,14 @@ constexpr int kRegionsDev = 8;
 // operations of one wave complete in order, so row i+1 sees row i's adds),
 // and each lane then fetches its group leader's old count with
 // ds_bpermute.  No row waits on the previous row's LDS round trip.
+// (wave_rank_digits takes the rows' digits already computed in dg.)
 template <int D, int ITEMS>
-__device__ __forceinline__ void wave_rank(const elem_t (&e)[ITEMS], uint32_t lim, uint32_t shift,
-                                          uint32_t mask, uint32_t* wh, uint32_t (&dg)[ITEMS],
-                                          uint32_t (&rk)[ITEMS]) {
+__device__ __forceinline__ void wave_rank_digits(const uint32_t (&dg)[ITEMS], uint32_t lim,
+                                                 uint32_t* wh, uint32_t (&rk)[ITEMS]) {
   const uint32_t lane = lane_id();
   uint32_t old[ITEMS], lead[ITEMS], plo[ITEMS], phi[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    dg[i] = digit_of(e[i].key, shift, mask);
     const uint64_t vmask = __ballot((uint32_t)i * kWave + lane < lim);
     plo[i] = (uint32_t)vmask;
     phi[i] = (uint32_t)(vmask >> 32);
@@ -76,6 +75,15 @@ __device__ __forceinline__ void wave_rank(const elem_t (&e)[ITEMS], uint32_t lim
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
     rk[i] += (uint32_t)__builtin_amdgcn_ds_bpermute((int)(lead[i] << 2), (int)old[i]);
+}
+
+template <int D, int ITEMS>
+__device__ __forceinline__ void wave_rank(const elem_t (&e)[ITEMS], uint32_t lim, uint32_t shift,
+                                          uint32_t mask, uint32_t* wh, uint32_t (&dg)[ITEMS],
+                                          uint32_t (&rk)[ITEMS]) {
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) dg[i] = digit_of(e[i].key, shift, mask);
+  wave_rank_digits<D, ITEMS>(dg, lim, wh, rk);
 }
 
 // ------------------------------------------------------------- look-back
