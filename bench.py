@@ -369,6 +369,29 @@ def main():
 
     traffic = traffic_from_profile(args.log2n, d, tile)
 
+    # The pass's access pattern without its sorting (lsb_debug_scatter_floor):
+    # 8192-element tiles read in order, written as R runs into R regions
+    # (R = 256: a uniform pass's layout; R = 1: a plain copy).  Mean of 3
+    # after one warm-up, HIP events on the stream the probe runs on.
+    floor = None
+    if n % 8192 == 0:
+        floor = {}
+        for runs in (1, 16, 64, 256):
+            ms = []
+            for i in range(4):
+                e0.record(stream)
+                lsb_amd.debug_scatter_floor(pristine, A, runs=runs)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                if i:
+                    ms.append(e0.elapsed_time(e1))
+            m = sum(ms) / len(ms)
+            floor[f"runs_{runs}"] = {"ms": round(m, 4),
+                                     "gbs": round(32 * n / (m * 1e-3) / 1e9, 1)}
+        floor["pass_over_floor_256"] = round(avg_pass_ms / floor["runs_256"]["ms"], 4)
+        floor["what"] = ("scatter pass memory traffic alone: 8192-element tiles read in order, "
+                         "written as R runs of consecutive elements into R regions")
+
     value = n / (ms_wall * 1e-3) / 1e6
     out = {
         "metric": METRIC,
@@ -395,6 +418,7 @@ def main():
         "sort_roofline": {"model_bytes": model_bytes, "achieved": round(sort_gbs, 1),
                           "frac": round(sort_gbs / HBM_PEAK_GBS, 4),
                           "ms_event": round(ms_event, 4), "histogram_ms": round(hist_ms, 4)},
+        "scatter_floor": floor,
         "copy_ceiling": {"achieved": round(copy_gbs, 1), "unit": "GB/s", "ms": round(copy_avg, 4),
                          "frac": round(copy_gbs / HBM_PEAK_GBS, 4),
                          "what": f"device-to-device copy of the 2^{args.log2n}-element input "

@@ -1433,6 +1433,18 @@ int lsb_verify(const lsb_elem* out, const uint64_t* in_keys, size_t n, uint64_t 
   return LSB_OK;
 }
 
+int lsb_debug_scatter_floor(const lsb_elem* src, lsb_elem* dst, size_t n, uint32_t runs,
+                            void* stream) {
+  const uint32_t T = lsb::scatter_floor_tile();
+  if (n == 0) return LSB_OK;
+  if (!src || !dst || (((uintptr_t)src | (uintptr_t)dst) & 15u)) return LSB_EINVAL;
+  if (n % T || n / T > 0xFFFFFFFFull) return LSB_EINVAL;
+  if (runs == 0 || (runs & (runs - 1)) || runs > T) return LSB_EINVAL;
+  if (overlaps(src, dst, n * sizeof(lsb_elem))) return LSB_EINVAL;
+  LSB_HIP(lsb::launch_scatter_floor(src, dst, n, runs, device_cus(), (hipStream_t)stream));
+  return LSB_OK;
+}
+
 int lsb_extract_keys(const lsb_elem* A, size_t n, uint64_t* keys, void* stream) {
   if (n == 0) return LSB_OK;
   if (!A || !keys) return LSB_EINVAL;

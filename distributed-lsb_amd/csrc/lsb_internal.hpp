@@ -249,5 +249,10 @@ hipError_t launch_verify(const void* out, const uint64_t* in_keys, uint64_t n,
                          uint32_t* bitmap,
                          unsigned long long* result, hipStream_t s);
 hipError_t launch_extract_keys(const void* A, uint64_t n, uint64_t* keys, hipStream_t s);
+// One scatter pass's memory traffic alone (lsb_debug_scatter_floor): n a
+// multiple of scatter_floor_tile(), runs a power of two dividing it.
+uint32_t scatter_floor_tile();
+hipError_t launch_scatter_floor(const void* src, void* dst, uint64_t n, uint32_t runs,
+                                uint32_t grid, hipStream_t s);
 
 }  // namespace lsb

@@ -153,4 +153,44 @@ hipError_t launch_extract_keys(const void* A, uint64_t n, uint64_t* keys, hipStr
   return hipGetLastError();
 }
 
+// ------------------------------------------------------- k_scatter_floor
+// The memory traffic of one scatter pass without its sorting: tile t of
+// BLOCK * ITEMS elements is read in order (each wave its 64 * ITEMS
+// contiguous elements, as the pass loads them) and written as `runs` runs
+// of consecutive elements, run r of tile t at dst[r * region + t * run_len]
+// -- the layout one pass over uniform keys produces with runs = 256
+// buckets, consecutive lanes writing consecutive slots of a run.  One block
+// per CU walks the tiles in order, like the pass's persistent grid.
+template <int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_scatter_floor(const elem_t* __restrict__ src,
+                                                         elem_t* __restrict__ dst,
+                                                         uint32_t ntiles, uint32_t run_len,
+                                                         uint64_t region) {
+  constexpr uint32_t T = BLOCK * ITEMS;
+  const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    elem_t e[ITEMS];
+    const uint32_t w0 = wave * kWave * ITEMS;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) e[i] = src[(size_t)t * T + w0 + (uint32_t)i * kWave + lane];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t j = w0 + (uint32_t)i * kWave + lane;
+      const uint32_t r = j / run_len, q = j % run_len;
+      dst[(size_t)r * region + (size_t)t * run_len + q] = e[i];
+    }
+  }
+}
+
+uint32_t scatter_floor_tile() { return 1024 * 8; }
+
+hipError_t launch_scatter_floor(const void* src, void* dst, uint64_t n, uint32_t runs,
+                                uint32_t grid, hipStream_t s) {
+  const uint32_t T = scatter_floor_tile();
+  hipLaunchKernelGGL((k_scatter_floor<1024, 8>), dim3(grid), dim3(1024), 0, s,
+                     static_cast<const elem_t*>(src), static_cast<elem_t*>(dst),
+                     (uint32_t)(n / T), T / runs, n / runs);
+  return hipGetLastError();
+}
+
 }  // namespace lsb

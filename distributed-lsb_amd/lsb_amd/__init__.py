@@ -135,6 +135,7 @@ SIGNATURES = {
     "lsb_verify": (_I, [_P, _P, _SZ, _U64, _U32, _U32, ctypes.POINTER(_U64),
                         ctypes.POINTER(ctypes.c_int64), _P]),
     "lsb_extract_keys": (_I, [_P, _SZ, _P, _P]),
+    "lsb_debug_scatter_floor": (_I, [_P, _P, _SZ, _U32, _P]),
 }
 
 
@@ -395,6 +396,14 @@ def verify(out, in_keys, n=None, val_base=0, key_bits=64, stream=None, key_order
                             _stream_ptr(stream)),
            "lsb_verify")
     return bad.value, first.value
+
+
+def debug_scatter_floor(src, dst, runs=256, n=None, stream=None):
+    """One scatter pass's memory traffic without the sort: 8192-element tiles
+    read in order, each written as `runs` runs into `runs` regions of dst."""
+    n = _nelem(src) if n is None else n
+    _check(lib().lsb_debug_scatter_floor(_ptr(src), _ptr(dst), n, runs, _stream_ptr(stream)),
+           "lsb_debug_scatter_floor")
 
 
 def extract_keys(A, keys, n=None, stream=None):

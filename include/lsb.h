@@ -302,6 +302,16 @@ int lsb_profile_read(lsb_profile* out);
 int lsb_debug_histogram(const lsb_elem* A, size_t n, const lsb_config* cfg,
                         uint32_t* counts);
 
+/* Debug entry for the memory floor of one scatter pass: moves src[0..n) to
+ * dst with the pass's access pattern and none of its sorting -- tiles of
+ * 8192 elements read in order, each written as `runs` runs of consecutive
+ * elements, run r of tile t at dst[r * (n / runs) + t * (8192 / runs)]
+ * (runs = 256: the layout one pass over uniform keys writes; runs = 1: a
+ * plain copy).  n must be a multiple of 8192 and runs a power of two of at
+ * most 8192 (LSB_EINVAL otherwise).  No counterpart in the reference. */
+int lsb_debug_scatter_floor(const lsb_elem* src, lsb_elem* dst, size_t n, uint32_t runs,
+                            void* stream);
+
 /* ------------------------------------------------------------ workloads */
 
 /* Synthetic inputs, generated on the device.  UNIFORM reproduces the

@@ -613,3 +613,26 @@ def test_randomized_configurations():
         want = O.stable_sort_order(a, key_bits, order)
         got = gpu_sort(a, key_bits=key_bits, key_order=order, variant=v, no_skip=no_skip)
         assert np.array_equal(got, want), (case, n, order, key_bits, dist, v, no_skip)
+
+
+@pytest.mark.parametrize("runs", [1, 16, 256])
+def test_scatter_floor_probe_layout(runs):
+    """lsb_debug_scatter_floor writes run r of tile t (8192 / runs
+    consecutive elements) at dst[r * n / runs + t * 8192 / runs]; bad
+    sizes and run counts are refused."""
+    T, tiles = 8192, 24
+    n = T * tiles
+    src = torch.arange(2 * n, dtype=torch.int64, device=DEV).reshape(n, 2)
+    dst = torch.full_like(src, -1)
+    lsb_amd.debug_scatter_floor(src, dst, runs=runs)
+    torch.cuda.synchronize()
+    L = T // runs
+    j = torch.arange(n, device=DEV)
+    t, w = j // T, j % T
+    want_pos = (w // L) * (n // runs) + t * L + (w % L)
+    expect = torch.empty_like(src)
+    expect[want_pos] = src
+    assert torch.equal(dst, expect)
+    for bad in (dict(runs=3), dict(runs=0), dict(runs=16384), dict(n=n - 1)):
+        with pytest.raises(lsb_amd.LsbError):
+            lsb_amd.debug_scatter_floor(src, dst, **bad)
