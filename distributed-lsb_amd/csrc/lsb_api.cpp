@@ -505,8 +505,9 @@ struct SegLayout {
   uint32_t cap_segs, max_tiles;
   size_t hdr, meta, counts, plan, tickets, status, tileseg, perm, total;
   // MSD-first form
-  uint32_t cap_entries, cap_runs;
+  uint32_t cap_entries, cap_runs, cap_wide, cap_wide_tiles;
   size_t ment[2], mcounts[2], morr[2], mtile[2], mstatus, mlvl, mruns, mrunctr;
+  size_t went, wcounts, worr, wtile, wstatus, wlvl;
 };
 
 // The MSD-first form sorts the large segments unless the caller asks for the
@@ -526,13 +527,20 @@ bool wants_msd(const lsb_config& c) {
 // than kMsdFinishCap elements together, so runs <= 2 cap_n / kMsdFinishCap
 // + (number of partitioned ranges + large buckets) over all levels.
 int make_msd_layout(size_t cap_n, SegLayout* L) {
-  const uint64_t ents = cap_n / (lsb::kMsdFinishCap + 1) + 1;
-  const uint64_t tiles = cap_n / lsb::kMsdTile + ents + 1;
-  const uint64_t runs = 2 * (cap_n / lsb::kMsdFinishCap) + 2 * lsb::kMsdMaxLevels * ents + 64;
+  // Level 0 also holds one dead entry (and its tiles) per wide segment
+  // besides the wide segments' large buckets: twice the bound.
+  const uint64_t ents = 2 * (cap_n / (lsb::kMsdFinishCap + 1)) + 2;
+  const uint64_t tiles = 2 * (cap_n / lsb::kMsdTile) + ents + 1;
+  const uint64_t wide = cap_n / (lsb::kMsdWideMin + 1) + 1;
+  const uint64_t wtiles = cap_n / lsb::kMsdTile + wide;
+  const uint64_t runs = 2 * (cap_n / lsb::kMsdFinishCap) + 2 * lsb::kMsdMaxLevels * ents +
+                        ((1u << lsb::kMsdWideBits) + 2) * wide + 64;
   if (tiles > 0x7FFFFFFFull || runs > 0xFFFFFFFFull) return LSB_ERANGE;
   L->cap_entries = (uint32_t)ents;
   L->max_tiles = (uint32_t)tiles;
   L->cap_runs = (uint32_t)runs;
+  L->cap_wide = (uint32_t)wide;
+  L->cap_wide_tiles = (uint32_t)wtiles;
   size_t off = L->perm;  // after the header and the segment table
   for (int i = 0; i < 2; ++i) {
     L->ment[i] = off;    off += align_up(ents * sizeof(lsb::MsdEntry));
@@ -544,6 +552,13 @@ int make_msd_layout(size_t cap_n, SegLayout* L) {
   L->mlvl = off;    off += align_up(lsb::kMsdMaxLevels * sizeof(lsb::MsdLevel));
   L->mruns = off;   off += align_up(runs * sizeof(lsb::MsdRun));
   L->mrunctr = off; off += align_up(16);
+  const size_t wnb = (size_t)1 << lsb::kMsdWideBits;
+  L->went = off;    off += align_up(wide * sizeof(lsb::MsdEntry));
+  L->wcounts = off; off += align_up(wide * wnb * 4);
+  L->worr = off;    off += align_up(wide * 16);
+  L->wtile = off;   off += align_up(wtiles * 4);
+  L->wstatus = off; off += align_up(wtiles * wnb * 4);
+  L->wlvl = off;    off += align_up(sizeof(lsb::MsdLevel));
   L->total = off;
   return LSB_OK;
 }
@@ -681,10 +696,19 @@ int enqueue_segmented(const SegLayout& L, char* ws, uint32_t flags, hipStream_t 
     m.cap_entries = L.cap_entries;
     m.cap_tiles = L.max_tiles;
     m.cap_runs = L.cap_runs;
+    m.went = reinterpret_cast<lsb::MsdEntry*>(ws + L.went);
+    m.wcounts = reinterpret_cast<uint32_t*>(ws + L.wcounts);
+    m.worr = reinterpret_cast<uint32_t*>(ws + L.worr);
+    m.wtile = reinterpret_cast<uint32_t*>(ws + L.wtile);
+    m.wstatus = reinterpret_cast<uint32_t*>(ws + L.wstatus);
+    m.wlvl = reinterpret_cast<lsb::MsdLevel*>(ws + L.wlvl);
+    m.cap_wide = L.cap_wide;
+    m.cap_wide_tiles = L.cap_wide_tiles;
     m.key_bits = L.key_bits;
     m.ko = L.ko;
     m.level = 0;
     LSB_HIP(lsb::launch_msd_init(m, 4 * cus, s));
+    LSB_HIP(lsb::launch_msd_wide(m, cus, s));
     // levels 0 .. P: the last one only copies ranges left in B
     for (uint32_t k = 0; k <= L.passes && k < (uint32_t)lsb::kMsdMaxLevels; ++k) {
       m.level = k;
@@ -950,7 +974,7 @@ int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_c
     } else if (e.n <= lsb::seg_small_capacity()) {
       sml[is++] = lsb::SegSmall{a, (uint32_t)e.n, 0u};
     } else {
-      lg[il++] = lsb::SegLarge{a, b, (uint32_t)e.n, tiles, 0u, 0u};
+      lg[il++] = lsb::SegLarge{a, b, (uint32_t)e.n, tiles, 0u, lsb::kNotWide};
       tiles += (uint32_t)((e.n + L.tile - 1) / L.tile);
     }
   }
@@ -958,7 +982,22 @@ int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_c
   hdr->n_tiny = n_tiny;
   hdr->n_small = n_small;
   hdr->total_tiles = tiles;
+  hdr->n_wide = hdr->wide_tiles = 0;
   if (tiles > L.max_tiles) return LSB_ENOMEM;
+  if (L.msd) {
+    // Segments whose 8-bit buckets would need a second partition are first
+    // partitioned on an 11-bit digit (their buckets then finish in LDS);
+    // they get compact tile ranges of their own for that partition.
+    for (uint32_t i = 0; i < n_large; ++i) {
+      if (L.key_bits >= lsb::kMsdWideKeyBits && lg[i].n > lsb::kMsdWideMin &&
+          lg[i].n <= lsb::kMsdWideMax) {
+        lg[i].wide = hdr->n_wide++;
+        lg[i].perm_pos = hdr->wide_tiles;
+        hdr->wide_tiles += (lg[i].n + lsb::kMsdTile - 1) / lsb::kMsdTile;
+      }
+    }
+    if (hdr->n_wide > L.cap_wide || hdr->wide_tiles > L.cap_wide_tiles) return LSB_ENOMEM;
+  } else {
   // The scatter passes give each group of blocks whole segments: largest
   // first onto the least-loaded group; inside a group the segments keep
   // their order.  When that leaves one group with much more than its share
@@ -1001,6 +1040,7 @@ int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_c
       hdr->group_tiles[0] = tiles;
       for (uint32_t i = 0; i < n_large; ++i) lg[i].perm_pos = lg[i].tile_begin;
     }
+  }
   }
   LSB_HIP(hipMemcpyAsync(ws + L.hdr, h, meta_bytes, hipMemcpyHostToDevice, s));
   LSB_HIP(hipEventRecord(W.staged, s));

@@ -124,6 +124,9 @@ struct SegHeader {
   uint32_t groups;
   uint32_t group_off[kRegions];
   uint32_t group_tiles[kRegions];
+  // MSD-first form: large segments partitioned first on an 11-bit digit
+  uint32_t n_wide;
+  uint32_t wide_tiles;
 };
 // Segment tables hold device addresses, so one launch sequence serves
 // segments carved out of one array (lsb_sort_segmented) or scattered over
@@ -134,9 +137,11 @@ struct SegLarge {
   uint64_t b;           // its scratch of n elements
   uint32_t n;
   uint32_t tile_begin;  // global index of the segment's first tile
-  uint32_t perm_pos;    // where its tiles start in its group's tile order
-  uint32_t pad;
+  uint32_t perm_pos;    // LSD form: where its tiles start in its group's tile order;
+                        // MSD form: its first tile among the wide segments' tiles
+  uint32_t wide;        // MSD form: its index among the wide segments, or kNotWide
 };
+constexpr uint32_t kNotWide = 0xFFFFFFFFu;
 struct SegSmall {
   uint64_t a;
   uint32_t n;
@@ -191,6 +196,14 @@ hipError_t launch_seg_copy(const SegHeader* hdr, const SegLarge* large,
 constexpr uint32_t kMsdTile = 8192;       // elements per onesweep tile
 constexpr uint32_t kMsdFinishCap = 8192;  // largest run one workgroup sorts in LDS
 constexpr int kMsdMaxLevels = 9;          // 8 digits of 8 bits + a copy-only level
+// Segments of more than kMsdWideMin and at most kMsdWideMax elements, with
+// keys of at least kMsdWideKeyBits bits, are first partitioned on their top
+// kMsdWideBits bits: their buckets then average at most kMsdFinishCap
+// elements and finish in LDS, where 8-bit digits would take two partitions.
+constexpr uint32_t kMsdWideBits = 11;
+constexpr uint32_t kMsdWideMin = 256 * kMsdFinishCap;
+constexpr uint32_t kMsdWideMax = 2048 * kMsdFinishCap;
+constexpr uint32_t kMsdWideKeyBits = 16;
 
 // flags of an entry
 constexpr uint32_t kMsdInB = 1u;   // its elements are in the scratch buffer
@@ -230,12 +243,21 @@ struct MsdArgs {
   MsdRun* runs;
   uint32_t* run_ctr;       // [0] runs emitted, [1] finish ticket, [2] overflow
   uint32_t cap_entries, cap_tiles, cap_runs;
+  // the wide first partition (11-bit digit) of the segments the host marked
+  MsdEntry* went;          // [wide] (tile_begin: first tile among wide tiles)
+  uint32_t* wcounts;       // [wide][2048] digit counts, then bucket starts
+  uint32_t* worr;          // [wide][4] OR and AND of its keys
+  uint32_t* wtile;         // wide tile -> wide segment
+  uint32_t* wstatus;       // [wide tile][2048] look-back words
+  MsdLevel* wlvl;          // entries, tiles, ticket of the wide partition
+  uint32_t cap_wide, cap_wide_tiles;
   uint32_t key_bits;
   KeyOrder ko;
   uint32_t level;
 };
 // One kernel each; every launch reads its work counts from the device.
 hipError_t launch_msd_init(const MsdArgs& a, uint32_t grid, hipStream_t s);
+hipError_t launch_msd_wide(const MsdArgs& a, uint32_t cus, hipStream_t s);
 hipError_t launch_msd_level(const MsdArgs& a, uint32_t cus, hipStream_t s);
 hipError_t launch_msd_finish(const MsdArgs& a, uint32_t cus, hipStream_t s);
 

@@ -56,7 +56,7 @@ __device__ __forceinline__ int top_varying_digit(uint64_t x, int below) {
 
 __device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
-// LDS passes for a run of n elements over nb buckets: the partition digit
+// LDS passes for a run of n elements over nb non-empty buckets: the partition digit
 // (it keeps the buckets in order) and enough digits below it that a bucket of
 // the average size m = n / nb has few keys left tied for the fix-up: with
 // uniform keys about m^2 / 2^(8d+1) tied pairs per bucket, so d digits with
@@ -117,17 +117,36 @@ __global__ __launch_bounds__(256) void k_msd_init(MsdArgs a) {
     }
     if (t < 4) a.run_ctr[t] = 0u;
   }
+  if (blockIdx.x == 0 && t == 0) {
+    MsdLevel w = {umin(a.hdr->n_wide, a.cap_wide), umin(a.hdr->wide_tiles, a.cap_wide_tiles),
+                  0u, 0u};
+    *a.wlvl = w;
+  }
+  // A segment marked wide goes to the wide partition; its level-0 entry is
+  // left empty and its level-0 tiles map to no entry.
   for (uint32_t s = blockIdx.x; s < nl; s += gridDim.x) {
     const SegLarge L = a.large[s];
     const uint32_t nt = (L.n + kMsdTile - 1) / kMsdTile;
+    const bool wide = L.wide < a.cap_wide;
     if (t == 0) {
-      MsdEntry e = {L.a, L.b, L.n, L.tile_begin, P - 1u, 0u};
+      MsdEntry e = {L.a, L.b, wide ? 0u : L.n, wide ? 0xFFFFFFFFu : L.tile_begin, P - 1u, 0u};
       a.ent[0][s] = e;
+      if (wide) {
+        MsdEntry w = {L.a, L.b, L.n, L.perm_pos, 0u, 0u};
+        a.went[L.wide] = w;
+      }
     }
     a.counts[0][(size_t)s * kNB + t] = 0u;
     if (t < 4) a.orr[0][4 * (size_t)s + t] = t < 2 ? 0u : ~0u;
     for (uint32_t i = t; i < nt; i += 256)
-      if (L.tile_begin + i < a.cap_tiles) a.tilemap[0][L.tile_begin + i] = s;
+      if (L.tile_begin + i < a.cap_tiles) a.tilemap[0][L.tile_begin + i] = wide ? 0xFFFFFFFFu : s;
+    if (wide) {
+      constexpr uint32_t WNB = 1u << kMsdWideBits;
+      for (uint32_t i = t; i < WNB; i += 256) a.wcounts[(size_t)L.wide * WNB + i] = 0u;
+      if (t < 4) a.worr[4 * (size_t)L.wide + t] = t < 2 ? 0u : ~0u;
+      for (uint32_t i = t; i < nt; i += 256)
+        if (L.perm_pos + i < a.cap_wide_tiles) a.wtile[L.perm_pos + i] = L.wide;
+    }
   }
 }
 
@@ -139,27 +158,33 @@ __global__ __launch_bounds__(256) void k_msd_init(MsdArgs a) {
 // from one lane.  The block's part of the tile map is staged in LDS and a
 // range's entry is read once for its consecutive tiles, so a tile's key
 // loads wait on no other load.
-__global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
+// NB = 256: each range's own 8-bit digit E.dig; NB = 2048 (the wide
+// partition): the top kMsdWideBits bits of the key.
+template <int NB>
+__device__ __forceinline__ void msd_hist(const MsdEntry* __restrict__ ent,
+                                         uint32_t* __restrict__ counts,
+                                         uint32_t* __restrict__ orr,
+                                         const uint32_t* __restrict__ tilemap,
+                                         uint32_t* __restrict__ status, uint32_t tiles,
+                                         uint32_t nent, uint32_t key_bits, const KeyOrder& ko) {
   constexpr int U = 8;
-  const uint32_t k = a.level, L = k & 1u;
+  constexpr int PER = NB / 256;  // counters per thread
   const uint32_t t = threadIdx.x;
-  const uint32_t tiles = umin(a.lvl[k].tiles, a.cap_tiles);
-  const uint32_t nent = umin(a.lvl[k].entries, a.cap_entries);
   const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
   const uint32_t t0 = blockIdx.x * per;
   const uint32_t t1 = umin(t0 + per, tiles);
   if (t0 >= t1) return;
-  __shared__ uint32_t h[kNB];
+  __shared__ uint32_t h[NB];
   __shared__ uint32_t s_or[4];  // OR lo, OR hi, AND lo, AND hi
   __shared__ uint32_t s_tm[256];
   {  // the level's scatter starts from clear look-back words
-    uint4* z = reinterpret_cast<uint4*>(a.status + (size_t)t0 * kNB);
-    const size_t z4 = (size_t)(t1 - t0) * kNB / 4;
+    uint4* z = reinterpret_cast<uint4*>(status + (size_t)t0 * NB);
+    const size_t z4 = (size_t)(t1 - t0) * NB / 4;
     for (size_t i = t; i < z4; i += 256) z[i] = make_uint4(0u, 0u, 0u, 0u);
   }
-  h[t] = 0u;
+  for (int i = 0; i < PER; ++i) h[t + 256 * i] = 0u;
   if (t < 4) s_or[t] = t < 2 ? 0u : ~0u;
-  const uint64_t kbm = low_bits(a.key_bits);
+  const uint64_t kbm = low_bits(key_bits);
   uint32_t cur = 0xFFFFFFFFu;  // range of the counts in h
   uint64_t acc_or = 0, acc_and = ~0ull;
   MsdEntry E = {0ull, 0ull, 0u, 0xFFFFFFFFu, 0u, kMsdCopy};
@@ -173,14 +198,16 @@ __global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
     acc_or = 0;
     acc_and = ~0ull;
     __syncthreads();
-    const uint32_t c = h[t];
-    if (c && e < nent) atomicAdd(&a.counts[L][(size_t)e * kNB + t], c);
-    h[t] = 0u;
+    for (int i = 0; i < PER; ++i) {
+      const uint32_t c = h[t + 256 * i];
+      if (c && e < nent) atomicAdd(&counts[(size_t)e * NB + t + 256 * i], c);
+      h[t + 256 * i] = 0u;
+    }
     if (t < 4) {
       const uint32_t v = s_or[t], idle = t < 2 ? 0u : ~0u;
       if (v != idle && e < nent) {
-        if (t < 2) atomicOr(&a.orr[L][4 * (size_t)e + t], v);
-        else atomicAnd(&a.orr[L][4 * (size_t)e + t], v);
+        if (t < 2) atomicOr(&orr[4 * (size_t)e + t], v);
+        else atomicAnd(&orr[4 * (size_t)e + t], v);
       }
       s_or[t] = idle;
     }
@@ -188,7 +215,7 @@ __global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
   };
   for (uint32_t c0 = t0; c0 < t1; c0 += 256) {
     __syncthreads();  // the previous chunk of the map is consumed
-    if (c0 + t < t1) s_tm[t] = a.tilemap[L][c0 + t];
+    if (c0 + t < t1) s_tm[t] = tilemap[c0 + t];
     __syncthreads();
     const uint32_t c1 = umin(c0 + 256, t1);
     for (uint32_t tl = c0; tl < c1; ++tl) {
@@ -196,12 +223,13 @@ __global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
       if (e != cur) {
         if (cur != 0xFFFFFFFFu) flush(cur);
         cur = e;
-        if (e < nent) E = a.ent[L][e];
+        if (e < nent) E = ent[e];
       }
       if (e >= nent || (E.flags & kMsdCopy)) continue;
       if (tl < E.tile_begin || (uint64_t)(tl - E.tile_begin) * kMsdTile >= E.n) continue;
       const elem_t* src = reinterpret_cast<const elem_t*>((E.flags & kMsdInB) ? E.b : E.a);
-      const uint32_t shift = E.dig * 8u, mask = digit_mask(E.dig, a.key_bits);
+      const uint32_t shift = NB == kNB ? E.dig * 8u : key_bits - kMsdWideBits;
+      const uint32_t mask = NB == kNB ? digit_mask(E.dig, key_bits) : (uint32_t)NB - 1u;
       const uint32_t first = (tl - E.tile_begin) * kMsdTile;
       const uint32_t cnt = umin(kMsdTile, E.n - first);
       const elem_t* p = src + first;
@@ -209,7 +237,7 @@ __global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
         for (uint32_t i0 = 0; i0 < cnt; i0 += 256 * U) {
           uint64_t kk[U];
 #pragma unroll
-          for (int u = 0; u < U; ++u) kk[u] = key_fwd(p[i0 + u * 256 + t].key, a.ko);
+          for (int u = 0; u < U; ++u) kk[u] = key_fwd(p[i0 + u * 256 + t].key, ko);
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             acc_or |= kk[u];
@@ -227,7 +255,7 @@ __global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
         for (uint32_t i0 = 0; i0 < cnt; i0 += 256) {
           const uint32_t i = i0 + t;
           const bool v = i < cnt;
-          const uint64_t kk = v ? key_fwd(p[i].key, a.ko) : 0ull;
+          const uint64_t kk = v ? key_fwd(p[i].key, ko) : 0ull;
           if (v) {
             acc_or |= kk;
             acc_and &= kk;
@@ -240,6 +268,20 @@ __global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
   if (cur != 0xFFFFFFFFu) flush(cur);
 }
 
+__global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
+  const uint32_t k = a.level, L = k & 1u;
+  msd_hist<kNB>(a.ent[L], a.counts[L], a.orr[L], a.tilemap[L], a.status,
+                umin(a.lvl[k].tiles, a.cap_tiles), umin(a.lvl[k].entries, a.cap_entries),
+                a.key_bits, a.ko);
+}
+
+// The wide partition's histogram: the marked segments' top 11-bit digit.
+__global__ __launch_bounds__(256) void k_msd_whist(MsdArgs a) {
+  msd_hist<(1 << kMsdWideBits)>(a.went, a.wcounts, a.worr, a.wtile, a.wstatus,
+                                umin(a.wlvl->tiles, a.cap_wide_tiles),
+                                umin(a.wlvl->entries, a.cap_wide), a.key_bits, a.ko);
+}
+
 // ----------------------------------------------------------- k_msd_plan
 // One block per range; thread b owns bucket b of a partitioned range.
 __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
@@ -250,7 +292,8 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
   const uint64_t kbm = low_bits(a.key_bits);
   __shared__ uint32_t s_start[kNB];   // element offset of each bucket
   __shared__ uint32_t s_q[kNB + 1];   // exclusive prefix of min(count, C + 1)
-  __shared__ uint32_t s_nl[kNB + 1];  // exclusive prefix of buckets of <= C elements
+  __shared__ uint32_t s_nl[kNB + 1];  // exclusive prefix of buckets of <= C elements (low 16
+                                      // bits) and of non-empty buckets (high 16 bits)
   __shared__ uint32_t s_emit[kNB];    // exclusive prefix of (tiles << 9 | 1) of next-level buckets
   __shared__ uint32_t s_next[kNB];    // end of a run that starts at b; for a large b, the next
                                       // small bucket | 0x80000000
@@ -286,7 +329,8 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
     const bool large = c > C;
     const bool emit = large && (cd >= 0 || out_b);
     const uint32_t nt = (c + kMsdTile - 1) / kMsdTile;
-    const uint32_t my_q = large ? C + 1 : c, my_nl = large ? 0u : 1u;
+    const uint32_t my_q = large ? C + 1 : c;
+    const uint32_t my_nl = (large ? 0u : 1u) | (c ? 0x10000u : 0u);
     const uint32_t my_emit = emit ? ((nt << 9) | 1u) : 0u;
     s_start[t] = c;
     s_q[t] = my_q;
@@ -338,9 +382,9 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
     }
     // Runs: consecutive buckets of at most C elements packed greedily up to
     // C elements; a larger bucket ends a run.  Each run is sorted by E.dig
-    // (so its buckets keep their order) and the digits below it that vary
-    // in the range; with none varying its buckets are already sorted, and
-    // it is a copy when they sit in B.
+    // (so its buckets keep their order; not needed in a run of one bucket)
+    // and the digits below it that vary in the range; with none varying its
+    // buckets are already sorted, and it is a copy when they sit in B.
     if (!any_small || (cd < 0 && !out_b)) {
       __syncthreads();  // the shared arrays are rewritten for the next range
       continue;
@@ -356,11 +400,11 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
     // bucket counts C + 1); for a large t, the next small bucket.
     if (large) {
       // smallest m in (t + 1, kNB] with s_nl[m] > s_nl[t + 1]: bucket m - 1
-      const uint32_t want = s_nl[t + 1] + 1u;
+      const uint32_t want = (s_nl[t + 1] & 0xFFFFu) + 1u;
       uint32_t lo = t + 2, hi = kNB + 1;
       while (lo < hi) {
         const uint32_t mid = (lo + hi) / 2;
-        if (s_nl[mid] >= want) hi = mid;
+        if ((s_nl[mid] & 0xFFFFu) >= want) hi = mid;
         else lo = mid + 1;
       }
       s_next[t] = 0x80000000u | (lo <= (uint32_t)kNB ? lo - 1 : (uint32_t)kNB);
@@ -387,7 +431,7 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
         if (rn) {
           s_rlo[R] = s_start[b];
           s_rn[R] = rn;
-          s_rb[R] = v - b;
+          s_rb[R] = (s_nl[v] >> 16) - (s_nl[b] >> 16);  // its non-empty buckets
           ++R;
         }
         b = v;
@@ -404,8 +448,12 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
     const uint32_t R = s_misc[2], rbase = s_misc[3];
     const uint64_t from = out_b ? E.b : E.a;
     for (uint32_t r = t; r < R; r += 256) {
+      // A run of one bucket has a constant E.dig: no pass over it.
+      const bool one = s_rb[r] == 1u;
+      const uint32_t dr = one ? digits & ~(1u << E.dig) : digits;
+      const uint32_t passes = finish_passes(s_rn[r], s_rb[r]) - (one ? 1u : 0u);
       MsdRun run = {from + 16ull * s_rlo[r], E.a + 16ull * s_rlo[r], s_rn[r],
-                    digits | (finish_passes(s_rn[r], s_rb[r]) << 16)};
+                    dr | (passes << 16)};
       a.runs[rbase + r] = run;
     }
     __syncthreads();
@@ -658,6 +706,212 @@ __global__ __launch_bounds__(BLOCK) void k_msd_finish(MsdArgs a) {
   }
 }
 
+// ------------------------------------------------------- wide partition
+// The segments the host marked (kMsdWideMin < n <= kMsdWideMax) are first
+// partitioned on the top kMsdWideBits bits of the key, so their buckets
+// average at most kMsdFinishCap elements and most finish in LDS at once.
+// k_msd_wplan is k_msd_plan for one such segment with 2048 buckets (eight
+// per thread); its large buckets join level 0, its small ones form runs.
+__global__ __launch_bounds__(256) void k_msd_wplan(MsdArgs a) {
+  constexpr uint32_t C = kMsdFinishCap;
+  constexpr int WNB = 1 << kMsdWideBits, PER = WNB / 256;
+  const uint32_t t = threadIdx.x;
+  const uint32_t nw = umin(a.wlvl->entries, a.cap_wide);
+  const uint32_t kb = a.key_bits;
+  const uint64_t kbm = low_bits(kb);
+  const uint32_t fshift = kb - kMsdWideBits;  // the wide digit's low bit
+  __shared__ uint32_t s_start[WNB];
+  __shared__ uint32_t s_q[WNB + 1];   // exclusive prefix of min(count, C + 1)
+  __shared__ uint32_t s_nl[WNB + 1];  // small buckets (low 16 bits), non-empty ones (high 16)
+  __shared__ uint32_t s_emit[WNB];    // (tiles << 12 | 1) of buckets that join level 0
+  __shared__ uint32_t s_next[WNB];
+  __shared__ uint32_t s_rlo[WNB], s_rn[WNB], s_rb[WNB];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_misc[4];
+  for (uint32_t w = blockIdx.x; w < nw; w += gridDim.x) {
+    const MsdEntry E = a.went[w];
+    const uint4 oa = reinterpret_cast<const uint4*>(a.worr)[w];
+    const uint64_t x = ((((uint64_t)oa.y << 32) | oa.x) ^ (((uint64_t)oa.w << 32) | oa.z)) & kbm;
+    if (x == 0) continue;  // every key equal: already sorted, in A
+    if (((x >> fshift) & (uint64_t)(WNB - 1)) == 0) {
+      // the wide digit is constant: level 0 partitions on the next varying one
+      if (t == 0) msd_emit(a, 0, E.a, E.b, E.n, (uint32_t)top_varying_digit(x, 8), 0u);
+      continue;
+    }
+    const int cd = top_varying_digit(x & low_bits(fshift), 8);
+    uint32_t c[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const uint32_t b = t * PER + i;
+      c[i] = a.wcounts[(size_t)w * WNB + b];
+      const bool large = c[i] > C;
+      const uint32_t nt = (c[i] + kMsdTile - 1) / kMsdTile;
+      s_start[b] = c[i];
+      s_q[b] = large ? C + 1 : c[i];
+      s_nl[b] = (large ? 0u : 1u) | (c[i] ? 0x10000u : 0u);
+      s_emit[b] = large ? ((nt << 12) | 1u) : 0u;  // every bucket lands in B
+    }
+    block_exclusive_scan<WNB, 256>(s_start, s_wsum);
+    block_exclusive_scan<WNB, 256>(s_q, s_wsum);
+    block_exclusive_scan<WNB, 256>(s_nl, s_wsum);
+    block_exclusive_scan<WNB, 256>(s_emit, s_wsum);
+    int any_small = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const uint32_t b = t * PER + i;
+      a.wcounts[(size_t)w * WNB + b] = s_start[b];
+      any_small |= (c[i] && c[i] <= C) ? 1 : 0;
+    }
+    if (t == 255) {
+      const uint32_t lb = WNB - 1, lc = c[PER - 1];
+      s_q[WNB] = s_q[lb] + (lc > C ? C + 1 : lc);
+      s_nl[WNB] = s_nl[lb] + ((lc > C ? 0u : 1u) | (lc ? 0x10000u : 0u));
+      const uint32_t tot = s_emit[lb] + (lc > C ? (((lc + kMsdTile - 1) / kMsdTile) << 12) | 1u : 0u);
+      s_misc[0] = tot ? atomicAdd(&a.lvl[0].entries, tot & 4095u) : 0u;
+      s_misc[1] = tot ? atomicAdd(&a.lvl[0].tiles, tot >> 12) : 0u;
+      a.went[w].flags = E.flags | kMsdMove;
+    }
+    any_small = __syncthreads_or(any_small);
+    // buckets of more than C elements: ranges of level 0
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const uint32_t b = t * PER + i;
+      if (c[i] <= C) continue;
+      const uint32_t nt = (c[i] + kMsdTile - 1) / kMsdTile;
+      const uint32_t id = s_misc[0] + (s_emit[b] & 4095u);
+      const uint32_t tb = s_misc[1] + (s_emit[b] >> 12);
+      const uint64_t ca = E.a + 16ull * s_start[b], cb = E.b + 16ull * s_start[b];
+      if (id >= a.cap_entries || (uint64_t)tb + nt > a.cap_tiles) {
+        atomicAdd(&a.run_ctr[2], 1u);  // cannot happen within the workspace's capacities
+        if (id < a.cap_entries) {
+          MsdEntry dead = {ca, cb, 0u, 0xFFFFFFFFu, 0u, 0u};
+          a.ent[0][id] = dead;
+        }
+        continue;
+      }
+      MsdEntry ch = {ca, cb, c[i], tb, cd >= 0 ? (uint32_t)cd : 0u,
+                     kMsdInB | (cd >= 0 ? 0u : kMsdCopy)};
+      a.ent[0][id] = ch;
+      uint4* z = reinterpret_cast<uint4*>(a.counts[0] + (size_t)id * kNB);
+      for (int j = 0; j < kNB / 4; ++j) z[j] = make_uint4(0u, 0u, 0u, 0u);
+      reinterpret_cast<uint4*>(a.orr[0])[id] = make_uint4(0u, 0u, ~0u, ~0u);
+      for (uint32_t j = 0; j < nt; ++j) a.tilemap[0][tb + j] = id;
+    }
+    if (!any_small) {
+      __syncthreads();
+      continue;
+    }
+    // Runs of consecutive small buckets, as in k_msd_plan, that also stay
+    // inside one aligned group of G buckets: the 8-bit digit p0 holding the
+    // wide digit's lowest bits then tells a run's buckets apart, so one LDS
+    // pass on it keeps them in order.  With no varying bit below the wide
+    // digit a run's buckets are already in order and it is a copy to A.
+    const uint32_t p0 = fshift / 8u;
+    const uint32_t G = 1u << (8u * p0 + 8u - fshift);
+    uint32_t lower = 0;
+    for (uint32_t p = 0; p < 8; ++p)
+      if (((x & low_bits(fshift)) >> (8 * p)) & 0xFFull) lower |= 1u << p;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const uint32_t b = t * PER + i;
+      if (c[i] > C) {
+        // smallest m in (b + 1, WNB] with more small buckets before it: bucket m - 1
+        const uint32_t want = (s_nl[b + 1] & 0xFFFFu) + 1u;
+        uint32_t lo = b + 2, hi = WNB + 1;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) / 2;
+          if ((s_nl[mid] & 0xFFFFu) >= want) hi = mid;
+          else lo = mid + 1;
+        }
+        s_next[b] = 0x80000000u | (lo <= (uint32_t)WNB ? lo - 1 : (uint32_t)WNB);
+      } else {
+        const uint32_t base = s_q[b];
+        uint32_t lo = b + 1, hi = umin((b / G + 1u) * G, (uint32_t)WNB);
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1) / 2;
+          if (s_q[mid] - base <= C) lo = mid;
+          else hi = mid - 1;
+        }
+        s_next[b] = lo;
+      }
+    }
+    __syncthreads();
+    if (t == 0) {
+      uint32_t R = 0;
+      for (uint32_t b = 0; b < (uint32_t)WNB;) {
+        const uint32_t v = s_next[b];
+        if (v & 0x80000000u) {
+          b = v & 0xFFFFu;
+          continue;
+        }
+        const uint32_t rn = s_q[v] - s_q[b];
+        if (rn) {
+          s_rlo[R] = s_start[b];
+          s_rn[R] = rn;
+          s_rb[R] = (s_nl[v] >> 16) - (s_nl[b] >> 16);
+          ++R;
+        }
+        b = v;
+      }
+      uint32_t base = R ? atomicAdd(&a.run_ctr[0], R) : 0u;
+      if (R && (uint64_t)base + R > a.cap_runs) {
+        atomicAdd(&a.run_ctr[2], 1u);  // cannot happen within the workspace's capacities
+        R = 0;
+      }
+      s_misc[2] = R;
+      s_misc[3] = base;
+    }
+    __syncthreads();
+    const uint32_t R = s_misc[2], rbase = s_misc[3];
+    for (uint32_t r = t; r < R; r += 256) {
+      const bool one = s_rb[r] == 1u;
+      uint32_t digits = 0, passes = 0;
+      if (cd >= 0) {
+        digits = one ? lower : (lower | (1u << p0));
+        passes = finish_passes(s_rn[r], s_rb[r]) - (one ? 1u : 0u);
+      }
+      MsdRun run = {E.b + 16ull * s_rlo[r], E.a + 16ull * s_rlo[r], s_rn[r],
+                    digits | (passes << 16)};
+      a.runs[rbase + r] = run;
+    }
+    __syncthreads();
+  }
+}
+
+template <int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_msd_wscatter(MsdArgs a) {
+  static_assert(BLOCK * ITEMS == (int)kMsdTile, "one onesweep tile per level tile");
+  constexpr int WNB = 1 << kMsdWideBits;
+  // 2048 per-wave counter rows do not fit beside the tile: they share the
+  // staging buffer
+  __shared__ TileSmem<kMsdWideBits, BLOCK, ITEMS> sm;
+  const uint32_t tiles = umin(a.wlvl->tiles, a.cap_wide_tiles);
+  const uint32_t nw = umin(a.wlvl->entries, a.cap_wide);
+  uint32_t* ticket = &a.wlvl->ticket;
+  const uint32_t fshift = a.key_bits - kMsdWideBits;
+  if (threadIdx.x == 0) sm.scalar[3] = atomicAdd(ticket, 1u);
+  __syncthreads();
+  while (true) {
+    const uint32_t tl = sm.scalar[3];
+    if (tl >= tiles) return;
+    const uint32_t e = a.wtile[tl];
+    bool took = false;
+    if (e < nw) {
+      const MsdEntry E = a.went[e];
+      const uint32_t local = tl - E.tile_begin;
+      if ((E.flags & kMsdMove) && tl >= E.tile_begin && (uint64_t)local * kMsdTile < E.n) {
+        onesweep_tile<kMsdWideBits, BLOCK, ITEMS>(
+            sm, reinterpret_cast<const elem_t*>(E.a), reinterpret_cast<elem_t*>(E.b), E.n, local,
+            a.wstatus + (size_t)E.tile_begin * WNB, nullptr, a.wcounts + (size_t)e * WNB, fshift,
+            (uint32_t)WNB - 1u, a.ko, ticket, nullptr);
+        took = true;
+      }
+    }
+    if (!took && threadIdx.x == 0) sm.scalar[3] = atomicAdd(ticket, 1u);
+    __syncthreads();
+  }
+}
+
 // ----------------------------------------------------------------- launch
 namespace {
 
@@ -678,6 +932,14 @@ constexpr int kFinishBlock = 1024, kFinishItems = 8;
 
 hipError_t launch_msd_init(const MsdArgs& a, uint32_t grid, hipStream_t s) {
   hipLaunchKernelGGL(k_msd_init, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_msd_wide(const MsdArgs& a, uint32_t cus, hipStream_t s) {
+  hipLaunchKernelGGL(k_msd_whist, dim3(2 * cus), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_msd_wplan, dim3(cus), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_msd_wscatter<kScatterBlock, kScatterItems>), dim3(cus),
+                     dim3(kScatterBlock), 0, s, a);
   return hipGetLastError();
 }
 

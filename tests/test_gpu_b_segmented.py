@@ -451,3 +451,73 @@ def test_msd_first_key_widths(key_bits):
     lsb_amd.sort_batch(As, Bs, key_bits=key_bits, sync=True)
     for i, X in enumerate(As):
         assert np.array_equal(from_dev(X), want[int(off[i]):int(off[i + 1])]), (key_bits, i)
+
+
+def _wide_shapes(seed=0):
+    """Segments of more than 2^21 and at most 2^24 elements, which are first
+    partitioned on their top 11 key bits: uniform keys (buckets of about 1500
+    elements, and of about 8192 at 2^24, where half of them are partitioned
+    again), all keys equal, a constant top 11 bits (partitioned from the next
+    8-bit digit instead), keys that differ only in their top 11 bits (every
+    bucket is one key: copies), heavy duplicates, sorted and reversed input
+    and the sizes at both ends of the range."""
+    segs = []
+    a = O.gen_uniform(3_000_000, ranks=4, seed_base=seed + 1)
+    segs.append(a)
+    a = O.gen_uniform(1 << 24, ranks=16, seed_base=seed + 2)
+    segs.append(a)
+    a = O.gen_uniform(2_500_000, ranks=1, seed_base=seed + 3)
+    a["key"] = np.uint64(0x0123456789ABCDEF)
+    segs.append(a)
+    a = O.gen_uniform(3_100_000, ranks=2, seed_base=seed + 4)
+    a["key"] = (a["key"] & np.uint64((1 << 53) - 1)) | np.uint64(0x5AB << 53)
+    segs.append(a)
+    a = O.gen_uniform(2_200_000, ranks=2, seed_base=seed + 5)
+    a["key"] = (a["key"] >> np.uint64(53)) << np.uint64(53)
+    segs.append(a)
+    a = O.gen_zipf(2_400_000, ranks=2, m=1 << 10, seed_base=seed + 6)
+    a["key"] = (a["key"] << np.uint64(53)) | (a["key"] * np.uint64(40503) & np.uint64(0xFF))
+    segs.append(a)
+    a = O.gen_uniform((1 << 21) + 1, ranks=1, seed_base=seed + 7)
+    a["key"] = np.arange((1 << 21) + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15 >> 22)
+    segs.append(a)
+    a = O.gen_uniform(2_300_000, ranks=1, seed_base=seed + 8)
+    a["key"] = np.arange(2_300_000, 0, -1, dtype=np.uint64) << np.uint64(40)
+    segs.append(a)
+    return segs
+
+
+def test_msd_wide_partition_shapes():
+    segs = _wide_shapes(11)
+    sizes = [len(s) for s in segs]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    a = np.concatenate(segs)
+    wants = [O.stable_sort_order(s, 64, "unsigned") for s in segs]
+    for graph in (False, True):
+        A = to_dev(a)
+        B = torch.empty_like(A)
+        lsb_amd.sort_segmented(A, B, off, sync=True, graph=graph)
+        got = from_dev(A)
+        for s in range(len(segs)):
+            lo, hi = int(off[s]), int(off[s + 1])
+            assert np.array_equal(got[lo:hi], wants[s]), (graph, s, sizes[s])
+    del A, B
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("key_bits", [16, 17, 33, 58, 59])
+def test_msd_wide_partition_key_widths(key_bits):
+    """The 11-bit digit's lowest bits fall at every offset inside an 8-bit
+    digit (runs stay within aligned groups of 2 to 256 buckets); bits above
+    the key width are carried along but do not order."""
+    sizes = [2_200_000, 50_000, 3_000_000]
+    a, off = batch(sizes, seed=100 + key_bits)
+    a[: sizes[0]]["key"] = a[: sizes[0]]["key"] % np.uint64(1 << 20) * np.uint64(0x10001)
+    want = expected(a, off, key_bits)
+    A = to_dev(a)
+    B = torch.empty_like(A)
+    lsb_amd.sort_segmented(A, B, off, key_bits=key_bits, sync=True)
+    got = from_dev(A)
+    for s in range(len(sizes)):
+        lo, hi = int(off[s]), int(off[s + 1])
+        assert np.array_equal(got[lo:hi], want[lo:hi]), (key_bits, s)
