@@ -986,7 +986,7 @@ int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_c
   if (tiles > L.max_tiles) return LSB_ENOMEM;
   if (L.msd) {
     // Segments whose 8-bit buckets would need a second partition are first
-    // partitioned on an 11-bit digit (their buckets then finish in LDS);
+    // partitioned on a kMsdWideBits-bit digit (their buckets then finish in LDS);
     // they get compact tile ranges of their own for that partition.
     for (uint32_t i = 0; i < n_large; ++i) {
       if (L.key_bits >= lsb::kMsdWideKeyBits && lg[i].n > lsb::kMsdWideMin &&

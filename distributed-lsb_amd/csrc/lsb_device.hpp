@@ -41,6 +41,33 @@ __device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Element loads and stores through the global address space.  Pointers the
+// kernels read from memory (segment tables, range lists, runs) are generic
+// to the compiler, which then emits flat instructions; those count against
+// lgkmcnt as well as vmcnt, so every later wait for an LDS access would also
+// wait for them to complete (a store's acknowledgement included).
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) u32x4_t g_u32x4_t;
+typedef __attribute__((address_space(1))) uint64_t g_u64_t;
+
+__device__ __forceinline__ elem_t gload(const elem_t* p) {
+  const u32x4_t v = *(const g_u32x4_t*)p;
+  elem_t x;
+  x.key = ((uint64_t)v.y << 32) | v.x;
+  x.val = ((uint64_t)v.w << 32) | v.z;
+  return x;
+}
+
+__device__ __forceinline__ uint64_t gload_key(const elem_t* p) {
+  return *(const g_u64_t*)&p->key;
+}
+
+__device__ __forceinline__ void gstore(elem_t* p, const elem_t& x) {
+  const u32x4_t v = {(uint32_t)x.key, (uint32_t)(x.key >> 32), (uint32_t)x.val,
+                     (uint32_t)(x.val >> 32)};
+  *(g_u32x4_t*)p = v;
+}
+
 // Workgroup barrier that orders LDS only.  __syncthreads() also orders
 // global memory, so the compiler drains every outstanding global load and
 // store of the wave at it; the sort kernels communicate through LDS inside

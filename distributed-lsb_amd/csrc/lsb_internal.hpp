@@ -200,9 +200,9 @@ constexpr int kMsdMaxLevels = 9;          // 8 digits of 8 bits + a copy-only le
 // keys of at least kMsdWideKeyBits bits, are first partitioned on their top
 // kMsdWideBits bits: their buckets then average at most kMsdFinishCap
 // elements and finish in LDS, where 8-bit digits would take two partitions.
-constexpr uint32_t kMsdWideBits = 11;
+constexpr uint32_t kMsdWideBits = 10;
 constexpr uint32_t kMsdWideMin = 256 * kMsdFinishCap;
-constexpr uint32_t kMsdWideMax = 2048 * kMsdFinishCap;
+constexpr uint32_t kMsdWideMax = 1024 * kMsdFinishCap;
 constexpr uint32_t kMsdWideKeyBits = 16;
 
 // flags of an entry

@@ -158,8 +158,8 @@ __global__ __launch_bounds__(256) void k_msd_init(MsdArgs a) {
 // from one lane.  The block's part of the tile map is staged in LDS and a
 // range's entry is read once for its consecutive tiles, so a tile's key
 // loads wait on no other load.
-// NB = 256: each range's own 8-bit digit E.dig; NB = 2048 (the wide
-// partition): the top kMsdWideBits bits of the key.
+// NB = 256: each range's own 8-bit digit E.dig; NB = 2^kMsdWideBits (the
+// wide partition): the top kMsdWideBits bits of the key.
 template <int NB>
 __device__ __forceinline__ void msd_hist(const MsdEntry* __restrict__ ent,
                                          uint32_t* __restrict__ counts,
@@ -237,7 +237,7 @@ __device__ __forceinline__ void msd_hist(const MsdEntry* __restrict__ ent,
         for (uint32_t i0 = 0; i0 < cnt; i0 += 256 * U) {
           uint64_t kk[U];
 #pragma unroll
-          for (int u = 0; u < U; ++u) kk[u] = key_fwd(p[i0 + u * 256 + t].key, ko);
+          for (int u = 0; u < U; ++u) kk[u] = key_fwd(gload_key(p + i0 + u * 256 + t), ko);
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             acc_or |= kk[u];
@@ -255,7 +255,7 @@ __device__ __forceinline__ void msd_hist(const MsdEntry* __restrict__ ent,
         for (uint32_t i0 = 0; i0 < cnt; i0 += 256) {
           const uint32_t i = i0 + t;
           const bool v = i < cnt;
-          const uint64_t kk = v ? key_fwd(p[i].key, ko) : 0ull;
+          const uint64_t kk = v ? key_fwd(gload_key(p + i), ko) : 0ull;
           if (v) {
             acc_or |= kk;
             acc_and &= kk;
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
                 a.key_bits, a.ko);
 }
 
-// The wide partition's histogram: the marked segments' top 11-bit digit.
+// The wide partition's histogram: the marked segments' top kMsdWideBits bits.
 __global__ __launch_bounds__(256) void k_msd_whist(MsdArgs a) {
   msd_hist<(1 << kMsdWideBits)>(a.went, a.wcounts, a.worr, a.wtile, a.wstatus,
                                 umin(a.wlvl->tiles, a.cap_wide_tiles),
@@ -497,7 +497,7 @@ __global__ __launch_bounds__(BLOCK) void k_msd_scatter(MsdArgs a) {
           const uint32_t cnt = umin(kMsdTile, E.n - first);
           const elem_t* src = reinterpret_cast<const elem_t*>(E.b) + first;
           elem_t* dst = reinterpret_cast<elem_t*>(E.a) + first;
-          for (uint32_t i = threadIdx.x; i < cnt; i += BLOCK) dst[i] = src[i];
+          for (uint32_t i = threadIdx.x; i < cnt; i += BLOCK) gstore(dst + i, gload(src + i));
         }
       }
     }
@@ -591,6 +591,8 @@ __device__ __forceinline__ void lds_sort(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t
   lds_barrier();
 }
 
+constexpr int kRawBufferFlags = 0x00020000;  // dword 3 of a raw gfx9 buffer resource
+
 template <int BLOCK, int ITEMS>
 __device__ __forceinline__ void finish_run(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t* whist,
                                            const elem_t* src, elem_t* dst, uint32_t n,
@@ -601,14 +603,26 @@ __device__ __forceinline__ void finish_run(TileSmem<8, BLOCK, ITEMS>& sm, uint32
   const uint32_t wrun = (t / kWave) * (kWave * ITEMS);
   const uint32_t lim = n > wrun ? n - wrun : 0u;
   const uint64_t kbm = low_bits(key_bits);
+  // Raw buffer loads over exactly the run: slots past its end read zero with
+  // no branch, so the loads go out back to back.
   elem_t e[ITEMS];
+  {
+    // the run is the same in every lane: the resource goes in scalar registers
+    // (readfirstlane returns int; each word goes through uint32_t so that no
+    // sign extension reaches the address's high word)
+    const uint64_t sa = reinterpret_cast<uint64_t>(src);
+    const uint64_t su = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(sa >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)sa);
+    const uint32_t nu = (uint32_t)__builtin_amdgcn_readfirstlane(n);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        reinterpret_cast<void*>(su), (short)0, (int)(nu * (uint32_t)sizeof(elem_t)), kRawBufferFlags);
+    const uint32_t off = (wrun + lane) * (uint32_t)sizeof(elem_t);
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    if ((uint32_t)i * kWave + lane < lim) {
-      e[i] = src[wrun + (uint32_t)i * kWave + lane];
-    } else {
-      e[i].key = 0;
-      e[i].val = 0;
+    for (int i = 0; i < ITEMS; ++i) {
+      const auto x = __builtin_amdgcn_raw_buffer_load_b128(
+          rs, (int)(off + (uint32_t)(i * kWave * (int)sizeof(elem_t))), 0, 0);
+      e[i].key = ((uint64_t)x[1] << 32) | x[0];
+      e[i].val = ((uint64_t)x[3] << 32) | x[2];
     }
   }
   keys_fwd<ITEMS>(e, ko);
@@ -673,7 +687,7 @@ __device__ __forceinline__ void finish_run(TileSmem<8, BLOCK, ITEMS>& sm, uint32
     if (p < n) {
       elem_t x = sm.elems[p];
       x.key = key_inv(x.key, ko);
-      dst[f[i]] = x;
+      gstore(dst + f[i], x);
     }
   }
 }
@@ -696,7 +710,7 @@ __global__ __launch_bounds__(BLOCK) void k_msd_finish(MsdArgs a) {
     if ((R.digits & 0xFFu) == 0) {
       const elem_t* src = reinterpret_cast<const elem_t*>(R.src);
       elem_t* dst = reinterpret_cast<elem_t*>(R.dst);
-      for (uint32_t i = threadIdx.x; i < R.n; i += BLOCK) dst[i] = src[i];
+      for (uint32_t i = threadIdx.x; i < R.n; i += BLOCK) gstore(dst + i, gload(src + i));
     } else {
       finish_run<BLOCK, ITEMS>(sm, s_whist, reinterpret_cast<const elem_t*>(R.src),
                                reinterpret_cast<elem_t*>(R.dst), R.n, a.key_bits, R.digits,
@@ -710,8 +724,9 @@ __global__ __launch_bounds__(BLOCK) void k_msd_finish(MsdArgs a) {
 // The segments the host marked (kMsdWideMin < n <= kMsdWideMax) are first
 // partitioned on the top kMsdWideBits bits of the key, so their buckets
 // average at most kMsdFinishCap elements and most finish in LDS at once.
-// k_msd_wplan is k_msd_plan for one such segment with 2048 buckets (eight
-// per thread); its large buckets join level 0, its small ones form runs.
+// k_msd_wplan is k_msd_plan for one such segment with 2^kMsdWideBits buckets
+// (several per thread); its large buckets join level 0, its small ones form
+// runs.
 __global__ __launch_bounds__(256) void k_msd_wplan(MsdArgs a) {
   constexpr uint32_t C = kMsdFinishCap;
   constexpr int WNB = 1 << kMsdWideBits, PER = WNB / 256;
@@ -882,8 +897,8 @@ template <int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_msd_wscatter(MsdArgs a) {
   static_assert(BLOCK * ITEMS == (int)kMsdTile, "one onesweep tile per level tile");
   constexpr int WNB = 1 << kMsdWideBits;
-  // 2048 per-wave counter rows do not fit beside the tile: they share the
-  // staging buffer
+  // per-wave counter rows of 2^kMsdWideBits buckets do not fit beside the
+  // tile: they share the staging buffer
   __shared__ TileSmem<kMsdWideBits, BLOCK, ITEMS> sm;
   const uint32_t tiles = umin(a.wlvl->tiles, a.cap_wide_tiles);
   const uint32_t nw = umin(a.wlvl->entries, a.cap_wide);

@@ -182,7 +182,7 @@ __device__ __forceinline__ void tile_load(elem_t (&e)[ITEMS], const elem_t* __re
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     if ((uint32_t)i * kWave + lane < lim) {
-      e[i] = src[(size_t)tile_base + wrun + (uint32_t)i * kWave + lane];
+      e[i] = gload(src + (size_t)tile_base + wrun + (uint32_t)i * kWave + lane);
     } else {
       e[i].key = 0;
       e[i].val = 0;
@@ -226,7 +226,7 @@ __device__ __forceinline__ void store_run(const elem_t* elems, const uint2* grs,
       for (int k = 0; k < ITEMS; ++k) x[k].key = key_inv(x[k].key, ko);
     }
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) dst[(size_t)(g[k].x + (uint32_t)(k * BLOCK) + t)] = x[k];
+    for (int k = 0; k < ITEMS; ++k) gstore(dst + (size_t)(g[k].x + (uint32_t)(k * BLOCK) + t), x[k]);
     return;
   }
 #pragma unroll
@@ -241,7 +241,7 @@ __device__ __forceinline__ void store_run(const elem_t* elems, const uint2* grs,
         atomicAdd(&hnext[r * NB + digit_of(x.key, nshift, nmask)], 1u);
       }
       x.key = key_inv(x.key, ko);
-      dst[(size_t)(g.x + j)] = x;
+      gstore(dst + (size_t)(g.x + j), x);
     }
   }
 }
@@ -588,7 +588,7 @@ __device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, const 
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     if ((uint32_t)i * kWave + lane < lim) {
-      e[i] = src[wrun + (uint32_t)i * kWave + lane];
+      e[i] = gload(src + wrun + (uint32_t)i * kWave + lane);
     } else {
       e[i].key = 0;
       e[i].val = 0;
@@ -643,7 +643,7 @@ __device__ __forceinline__ void small_sort(TileSmem<D, BLOCK, ITEMS>& sm, const 
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
-    if ((uint32_t)i * kWave + lane < lim) dst[wrun + (uint32_t)i * kWave + lane] = e[i];
+    if ((uint32_t)i * kWave + lane < lim) gstore(dst + wrun + (uint32_t)i * kWave + lane, e[i]);
 }
 
 // Whole sort of one array in place: every digit of the key.

@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_seg_hist(const SegHeader* __restrict__ 
       for (uint32_t i0 = 0; i0 < cnt; i0 += 256 * U) {
         uint64_t k[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) k[u] = a[i0 + u * 256 + t].key;
+        for (int u = 0; u < U; ++u) k[u] = gload_key(a + i0 + u * 256 + t);
         if (ko.sign) {
 #pragma unroll
           for (int u = 0; u < U; ++u) k[u] = key_fwd(k[u], ko);
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_seg_hist(const SegHeader* __restrict__ 
       for (uint32_t i0 = 0; i0 < cnt; i0 += 256) {
         const uint32_t i = i0 + t;
         const bool v = i < cnt;
-        const uint64_t k = v ? key_fwd(a[i].key, ko) : 0ull;
+        const uint64_t k = v ? key_fwd(gload_key(a + i), ko) : 0ull;
 #pragma unroll
         for (int p = 0; p < P; ++p) hist_add(h + p * NB, digit_of(k, p * D, mask[p]), v);
       }
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) void k_seg_copy(const SegHeader* __restrict__ 
     const uint32_t first = (tl - L.tile_begin) * tile;
     const uint32_t cnt = L.n - first < tile ? L.n - first : tile;
     for (uint32_t i = threadIdx.x; i < cnt; i += 256)
-      reinterpret_cast<elem_t*>(L.a)[first + i] = reinterpret_cast<const elem_t*>(L.b)[first + i];
+      gstore(reinterpret_cast<elem_t*>(L.a) + first + i, gload(reinterpret_cast<const elem_t*>(L.b) + first + i));
   }
 }
 

@@ -454,13 +454,13 @@ def test_msd_first_key_widths(key_bits):
 
 
 def _wide_shapes(seed=0):
-    """Segments of more than 2^21 and at most 2^24 elements, which are first
-    partitioned on their top 11 key bits: uniform keys (buckets of about 1500
-    elements, and of about 8192 at 2^24, where half of them are partitioned
-    again), all keys equal, a constant top 11 bits (partitioned from the next
-    8-bit digit instead), keys that differ only in their top 11 bits (every
-    bucket is one key: copies), heavy duplicates, sorted and reversed input
-    and the sizes at both ends of the range."""
+    """Segments of more than 2^21 and at most 2^23 elements, which are first
+    partitioned on their top 10 key bits: uniform keys (buckets of about 2900
+    elements), all keys equal, a constant top 10 bits (partitioned from the
+    next 8-bit digit instead), keys that differ only in their top 11 bits
+    (every bucket holds two keys), heavy duplicates, sorted and reversed
+    input, the smallest size of the range, and a 2^24-element segment beside
+    them that takes two 8-bit partitions."""
     segs = []
     a = O.gen_uniform(3_000_000, ranks=4, seed_base=seed + 1)
     segs.append(a)
@@ -507,9 +507,9 @@ def test_msd_wide_partition_shapes():
 
 @pytest.mark.parametrize("key_bits", [16, 17, 33, 58, 59])
 def test_msd_wide_partition_key_widths(key_bits):
-    """The 11-bit digit's lowest bits fall at every offset inside an 8-bit
-    digit (runs stay within aligned groups of 2 to 256 buckets); bits above
-    the key width are carried along but do not order."""
+    """The 10-bit digit's lowest bit falls at several offsets inside an
+    8-bit digit (runs stay within aligned groups of 2 to 256 buckets); bits
+    above the key width are carried along but do not order."""
     sizes = [2_200_000, 50_000, 3_000_000]
     a, off = batch(sizes, seed=100 + key_bits)
     a[: sizes[0]]["key"] = a[: sizes[0]]["key"] % np.uint64(1 << 20) * np.uint64(0x10001)
