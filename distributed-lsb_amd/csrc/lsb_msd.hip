@@ -509,14 +509,16 @@ __global__ __launch_bounds__(BLOCK) void k_msd_scatter(MsdArgs a) {
 // --------------------------------------------------------- k_msd_finish
 // A run is sorted in LDS by LSD passes over the highest few of its digits
 // (as many as its buckets' size needs: finish_passes).  Keys that still tie
-// on everything from the lowest sorted digit up form groups, already in input
-// order; every element of a group of at most kFixGroup finds its place in the
-// group by comparing its lower bits with the other members (ties broken by
-// position, so stable) and is written there directly.  For uniform keys most
-// elements are in no group and the groups are pairs.  A longer group is
-// usually equal keys, already in order; if one is out of order the run goes
-// through LDS passes over every one of its digits instead.
-constexpr uint32_t kFixGroup = 32;
+// on everything from the lowest sorted digit up then sit next to each other
+// in input order; rounds of odd-even transposition over the LDS copy sort
+// each such group by its remaining bits (neighbours swap only when both tie
+// there and the first key is strictly larger, so equal keys keep their
+// order) until a round changes nothing after a round of the other parity.
+// For uniform keys the groups are mostly pairs and a few rounds do.  Groups
+// of many distinct keys (still swapping after kFixRounds rounds) send the run
+// through LDS passes over every one of its digits instead; long groups of
+// equal keys never swap.
+constexpr int kFixRounds = 8;
 
 // One stable LDS pass: ranks e[] by the digit at shift/mask and writes the
 // elements to sm.elems in digit order.  Returns false (nothing written) when
@@ -524,7 +526,8 @@ constexpr uint32_t kFixGroup = 32;
 template <int BLOCK, int ITEMS>
 __device__ __forceinline__ bool lds_pass(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t* whist,
                                          const elem_t (&e)[ITEMS], uint32_t lim, uint32_t n,
-                                         uint32_t shift, uint32_t mask) {
+                                         uint32_t shift, uint32_t mask, uint32_t* one_flag,
+                                         uint32_t pass_id) {
   constexpr int W = BLOCK / kWave;
   static_assert(BLOCK >= kNB, "one bucket per thread");
   const uint32_t t = threadIdx.x;
@@ -534,9 +537,8 @@ __device__ __forceinline__ bool lds_pass(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t
   // Only this wave has touched its row since the last pass's scan.
   for (uint32_t i = lane; i < (uint32_t)kNB; i += kWave) wh[i] = 0u;
   uint32_t dg[ITEMS], rk[ITEMS];
-  wave_rank<8, ITEMS>(e, lim, shift, mask, wh, dg, rk);
+  wave_rank<8, ITEMS, true>(e, lim, shift, mask, wh, dg, rk);
   lds_barrier();
-  int one = 0;
   if (t < (uint32_t)kNB) {
     uint32_t s = 0;
 #pragma unroll
@@ -546,12 +548,14 @@ __device__ __forceinline__ bool lds_pass(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t
       s += c;
     }
     sm.start[t] = s;
-    one = s == n;
+    // the bucket holding all n elements marks the pass (no word to clear:
+    // each pass of a block has its own id)
+    if (s == n) *one_flag = pass_id;
   }
-  one = __syncthreads_or(one);
   // adds each bucket's start into every wave's row: slot = wh[digit] + rank
+  // (its barriers also publish the mark)
   block_exclusive_scan<kNB, BLOCK, W, true>(sm.start, sm.wsum, whist);
-  if (one) return false;
+  if (*one_flag == pass_id) return false;
   uint32_t pos[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) pos[i] = wh[dg[i]] + rk[i];
@@ -566,7 +570,8 @@ __device__ __forceinline__ bool lds_pass(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t
 template <int BLOCK, int ITEMS>
 __device__ __forceinline__ void lds_sort(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t* whist,
                                          elem_t (&e)[ITEMS], uint32_t lim, uint32_t n,
-                                         uint32_t key_bits, uint32_t digits) {
+                                         uint32_t key_bits, uint32_t digits, uint32_t* one_flag,
+                                         uint32_t& pass_id) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t wrun = (threadIdx.x / kWave) * (kWave * ITEMS);
   bool in_lds = false;  // sm.elems holds the current order (else e[] does)
@@ -579,7 +584,8 @@ __device__ __forceinline__ void lds_sort(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t
         if ((uint32_t)i * kWave + lane < lim) e[i] = sm.elems[wrun + (uint32_t)i * kWave + lane];
       in_lds = false;
     }
-    if (lds_pass<BLOCK, ITEMS>(sm, whist, e, lim, n, p * 8u, digit_mask(p, key_bits)))
+    if (lds_pass<BLOCK, ITEMS>(sm, whist, e, lim, n, p * 8u, digit_mask(p, key_bits), one_flag,
+                               ++pass_id))
       in_lds = true;
   }
   if (!in_lds) {
@@ -593,38 +599,94 @@ __device__ __forceinline__ void lds_sort(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t
 
 constexpr int kRawBufferFlags = 0x00020000;  // dword 3 of a raw gfx9 buffer resource
 
+// Issues the loads of the run at src (n elements, the same in every lane)
+// into e[] in the wave-contiguous layout: raw buffer loads over exactly the
+// run, so slots past its end read zero with no branch and the loads go out
+// back to back.  The resource lives in scalar registers (readfirstlane
+// returns int: each word goes through uint32_t so that no sign extension
+// reaches the address's high word).
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ void run_load(elem_t (&e)[ITEMS], const elem_t* src, uint32_t n) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t wrun = (threadIdx.x / kWave) * (kWave * ITEMS);
+  const uint64_t sa = reinterpret_cast<uint64_t>(src);
+  const uint64_t su =
+      ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(sa >> 32)) << 32) |
+      (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)sa);
+  const uint32_t nu = (uint32_t)__builtin_amdgcn_readfirstlane(n);
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      reinterpret_cast<void*>(su), (short)0, (int)(nu * (uint32_t)sizeof(elem_t)),
+      kRawBufferFlags);
+  const uint32_t off = (wrun + lane) * (uint32_t)sizeof(elem_t);
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const auto x = __builtin_amdgcn_raw_buffer_load_b128(
+        rs, (int)(off + (uint32_t)(i * kWave * (int)sizeof(elem_t))), 0, 0);
+    e[i].key = ((uint64_t)x[1] << 32) | x[0];
+    e[i].val = ((uint64_t)x[3] << 32) | x[2];
+  }
+}
+
+// Odd-even transposition rounds over sm.elems[0, n) within groups of keys
+// equal from bit `cut` up.  Returns false if the groups are still unsorted
+// after kFixRounds rounds.
+// flags: 2 * BLOCK / 64 words of LDS, one per wave and round parity, so a
+// round's OR needs one barrier.
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ bool fix_ties(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t* flags,
+                                         uint32_t n, uint64_t kbm, uint32_t cut) {
+  constexpr int PAIRS = (BLOCK * ITEMS / 2 + BLOCK - 1) / BLOCK;  // pairs per thread
+  constexpr int W = BLOCK / kWave;
+  for (int r = 0; r < kFixRounds; ++r) {
+    int sw = 0;
+    // recomputed here rather than kept live across the kernel's loop (the
+    // compiler would otherwise hoist the pair indices and spill them)
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+#pragma unroll
+    for (int q = 0; q < PAIRS; ++q) {
+      const uint32_t j = 2u * (tid + (uint32_t)(q * BLOCK)) + (uint32_t)(r & 1);
+      if (j + 1 < n) {
+        const uint64_t ka = sm.elems[j].key & kbm, kb = sm.elems[j + 1].key & kbm;
+        if ((ka >> cut) == (kb >> cut) && ka > kb) {
+          const elem_t x = sm.elems[j];
+          sm.elems[j] = sm.elems[j + 1];
+          sm.elems[j + 1] = x;
+          sw = 1;
+        }
+      }
+    }
+    // a round without a swap after one of the other parity: every adjacent
+    // pair is in order
+    uint32_t* fl = flags + (r & 1) * W;
+    const bool any_w = __ballot(sw != 0) != 0;
+    if ((threadIdx.x & (kWave - 1)) == 0) fl[threadIdx.x / kWave] = any_w ? 1u : 0u;
+    lds_barrier();  // also orders this round's swaps before the next round's reads
+    uint32_t any = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) any |= fl[w];
+    if (!any && r > 0) return true;
+  }
+  return false;
+}
+
+// Sorts one run (n <= BLOCK * ITEMS) from src into dst.  Thread 0 takes the
+// next run's ticket (into sm.scalar[1]) before the stores go out, so its
+// round trip does not wait behind them.
 template <int BLOCK, int ITEMS>
 __device__ __forceinline__ void finish_run(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t* whist,
                                            const elem_t* src, elem_t* dst, uint32_t n,
                                            uint32_t key_bits, uint32_t digits,
-                                           const KeyOrder& ko) {
+                                           const KeyOrder& ko, uint32_t* ticket,
+                                           uint32_t* flags, uint32_t* one_flag,
+                                           uint32_t& pass_id) {
   const uint32_t t = threadIdx.x;
   const uint32_t lane = t & (kWave - 1);
   const uint32_t wrun = (t / kWave) * (kWave * ITEMS);
   const uint32_t lim = n > wrun ? n - wrun : 0u;
   const uint64_t kbm = low_bits(key_bits);
-  // Raw buffer loads over exactly the run: slots past its end read zero with
-  // no branch, so the loads go out back to back.
   elem_t e[ITEMS];
-  {
-    // the run is the same in every lane: the resource goes in scalar registers
-    // (readfirstlane returns int; each word goes through uint32_t so that no
-    // sign extension reaches the address's high word)
-    const uint64_t sa = reinterpret_cast<uint64_t>(src);
-    const uint64_t su = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(sa >> 32)) << 32) |
-                        (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)sa);
-    const uint32_t nu = (uint32_t)__builtin_amdgcn_readfirstlane(n);
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        reinterpret_cast<void*>(su), (short)0, (int)(nu * (uint32_t)sizeof(elem_t)), kRawBufferFlags);
-    const uint32_t off = (wrun + lane) * (uint32_t)sizeof(elem_t);
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const auto x = __builtin_amdgcn_raw_buffer_load_b128(
-          rs, (int)(off + (uint32_t)(i * kWave * (int)sizeof(elem_t))), 0, 0);
-      e[i].key = ((uint64_t)x[1] << 32) | x[0];
-      e[i].val = ((uint64_t)x[3] << 32) | x[2];
-    }
-  }
+  run_load<BLOCK, ITEMS>(e, src, n);
   keys_fwd<ITEMS>(e, ko);
   uint32_t top = 0, ntop = 0;
   const uint32_t want = digits >> 16;
@@ -634,89 +696,71 @@ __device__ __forceinline__ void finish_run(TileSmem<8, BLOCK, ITEMS>& sm, uint32
       top |= 1u << p;
       ++ntop;
     }
-  lds_sort<BLOCK, ITEMS>(sm, whist, e, lim, n, key_bits, top);
-  // Final position of the element at LDS slot t + i * BLOCK.
-  uint32_t f[ITEMS];
-  int bad = 0;
-  if (digits & ~top) {
-    const uint32_t cut = 8u * (uint32_t)__builtin_ctz(top);
-    auto hi = [&](uint32_t j) { return (sm.elems[j].key & kbm) >> cut; };
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t p = t + (uint32_t)i * BLOCK;
-      f[i] = p;
-      if (p >= n) continue;
-      const uint64_t k = sm.elems[p].key & kbm, g = k >> cut;
-      const bool tie_l = p > 0 && hi(p - 1) == g;
-      const bool tie_r = p + 1 < n && hi(p + 1) == g;
-      if (!tie_l && !tie_r) continue;
-      uint32_t lo = p, end = p + 1;
-      while (lo > 0 && p - lo < kFixGroup && hi(lo - 1) == g) --lo;
-      while (end < n && end - p <= kFixGroup && hi(end) == g) ++end;
-      const bool whole = (lo == 0 || hi(lo - 1) != g) && (end == n || hi(end) != g);
-      if (!whole || end - lo > kFixGroup) {
-        // a long group: fine only if already in order
-        if (tie_l && k < (sm.elems[p - 1].key & kbm)) bad = 1;
-        continue;
-      }
-      uint32_t r = lo;
-      for (uint32_t j = lo; j < end; ++j) {
-        const uint64_t kj = sm.elems[j].key & kbm;
-        r += (kj < k || (kj == k && j < p)) ? 1u : 0u;
-      }
-      f[i] = r;
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) f[i] = t + (uint32_t)i * BLOCK;
-  }
-  if (__syncthreads_or(bad)) {
+  lds_sort<BLOCK, ITEMS>(sm, whist, e, lim, n, key_bits, top, one_flag, pass_id);
+  if ((digits & ~top) &&
+      !fix_ties<BLOCK, ITEMS>(sm, flags, n, kbm, 8u * (uint32_t)__builtin_ctz(top))) {
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i)
       if ((uint32_t)i * kWave + lane < lim) e[i] = sm.elems[wrun + (uint32_t)i * kWave + lane];
-    lds_sort<BLOCK, ITEMS>(sm, whist, e, lim, n, key_bits, digits);
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) f[i] = t + (uint32_t)i * BLOCK;
+    lds_sort<BLOCK, ITEMS>(sm, whist, e, lim, n, key_bits, digits, one_flag, pass_id);
   }
-  // With src == dst no element may be stored before every thread's loads of
-  // it have completed: a full barrier (waits for this thread's loads too).
-  __syncthreads();
+  uint32_t nt = 0;
+  if (t == 0) nt = atomicAdd(ticket, 1u);
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const uint32_t p = t + (uint32_t)i * BLOCK;
     if (p < n) {
       elem_t x = sm.elems[p];
       x.key = key_inv(x.key, ko);
-      gstore(dst + f[i], x);
+      gstore(dst + p, x);
     }
   }
+  if (t == 0) sm.scalar[1] = nt;
 }
 
 // Sorts every run inside one workgroup (runs handed out by a ticket, so
-// uneven runs balance across the grid).
+// uneven runs balance across the grid).  A run's loads were consumed by its
+// first LDS pass before anything is stored, so a run sorted in place
+// (src == dst) needs no further wait; runs are disjoint.
 template <int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_msd_finish(MsdArgs a) {
   static_assert(BLOCK * ITEMS >= (int)kMsdFinishCap, "a run fits one workgroup");
   __shared__ TileSmem<8, BLOCK, ITEMS> sm;
   __shared__ uint32_t s_whist[(BLOCK / kWave) * kNB];
+  __shared__ uint32_t s_flags[2 * (BLOCK / kWave)];
+  __shared__ uint32_t s_one;
   const uint32_t nr = umin(a.run_ctr[0], a.cap_runs);
+  uint32_t* ticket = &a.run_ctr[1];
+  uint32_t pass_id = 0;  // LDS passes of this block so far
+  if (threadIdx.x == 0) {
+    s_one = 0u;
+    sm.scalar[1] = atomicAdd(ticket, 1u);
+  }
+  lds_barrier();
   while (true) {
-    if (threadIdx.x == 0) sm.scalar[1] = atomicAdd(&a.run_ctr[1], 1u);
-    __syncthreads();
-    const uint32_t r = sm.scalar[1];
-    __syncthreads();  // every thread has read the ticket before the next one is taken
+    const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane(sm.scalar[1]);
+    lds_barrier();  // every thread has the ticket before thread 0 replaces it
     if (r >= nr) return;
     const MsdRun R = a.runs[r];
+    const elem_t* src = reinterpret_cast<const elem_t*>(R.src);
+    elem_t* dst = reinterpret_cast<elem_t*>(R.dst);
     if ((R.digits & 0xFFu) == 0) {
-      const elem_t* src = reinterpret_cast<const elem_t*>(R.src);
-      elem_t* dst = reinterpret_cast<elem_t*>(R.dst);
-      for (uint32_t i = threadIdx.x; i < R.n; i += BLOCK) gstore(dst + i, gload(src + i));
+      // a copy
+      elem_t e[ITEMS];
+      run_load<BLOCK, ITEMS>(e, src, R.n);
+      uint32_t nt = 0;
+      if (threadIdx.x == 0) nt = atomicAdd(ticket, 1u);
+      const uint32_t lane = threadIdx.x & (kWave - 1);
+      const uint32_t wrun = (threadIdx.x / kWave) * (kWave * ITEMS);
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i)
+        if (wrun + (uint32_t)i * kWave + lane < R.n) gstore(dst + wrun + (uint32_t)i * kWave + lane, e[i]);
+      if (threadIdx.x == 0) sm.scalar[1] = nt;
     } else {
-      finish_run<BLOCK, ITEMS>(sm, s_whist, reinterpret_cast<const elem_t*>(R.src),
-                               reinterpret_cast<elem_t*>(R.dst), R.n, a.key_bits, R.digits,
-                               a.ko);
+      finish_run<BLOCK, ITEMS>(sm, s_whist, src, dst, R.n, a.key_bits, R.digits, a.ko, ticket,
+                               s_flags, &s_one, pass_id);
     }
-    __syncthreads();
+    lds_barrier();  // the next ticket is in sm.scalar[1]; LDS is free again
   }
 }
 

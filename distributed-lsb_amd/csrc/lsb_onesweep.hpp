@@ -26,11 +26,15 @@ constexpr int kRegionsDev = 8;
 // operations of one wave complete in order, so row i+1 sees row i's adds),
 // and each lane then fetches its group leader's old count with
 // ds_bpermute.  No row waits on the previous row's LDS round trip.
-// (wave_rank_digits takes the rows' digits already computed in dg.)
-template <int D, int ITEMS>
+// (wave_rank_digits takes the rows' digits already computed in dg.)  With
+// SKIP, the ballots of row groups holding no valid element are skipped (a
+// branch on the wave-uniform lim): worth it where runs often fill only part
+// of the block.
+template <int D, int ITEMS, bool SKIP = false>
 __device__ __forceinline__ void wave_rank_digits(const uint32_t (&dg)[ITEMS], uint32_t lim,
                                                  uint32_t* wh, uint32_t (&rk)[ITEMS]) {
   const uint32_t lane = lane_id();
+  const uint32_t ulim = SKIP ? (uint32_t)__builtin_amdgcn_readfirstlane(lim) : lim;
   uint32_t old[ITEMS], lead[ITEMS], plo[ITEMS], phi[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
@@ -48,6 +52,16 @@ __device__ __forceinline__ void wave_rank_digits(const uint32_t (&dg)[ITEMS], ui
   static_assert(ITEMS % G == 0, "rows come in whole groups");
 #pragma unroll
   for (int i0 = 0; i0 < ITEMS; i0 += G) {
+    if (SKIP && (uint32_t)i0 * kWave >= ulim) {
+      // no valid element from here on: peers stay empty, no counter changes
+#pragma unroll
+      for (int i = i0; i < ITEMS; ++i) {
+        old[i] = 0;
+        lead[i] = lane;
+        rk[i] = 0;
+      }
+      break;
+    }
 #pragma unroll
     for (int b = 0; b < D; ++b) {
 #pragma unroll
@@ -77,13 +91,13 @@ __device__ __forceinline__ void wave_rank_digits(const uint32_t (&dg)[ITEMS], ui
     rk[i] += (uint32_t)__builtin_amdgcn_ds_bpermute((int)(lead[i] << 2), (int)old[i]);
 }
 
-template <int D, int ITEMS>
+template <int D, int ITEMS, bool SKIP = false>
 __device__ __forceinline__ void wave_rank(const elem_t (&e)[ITEMS], uint32_t lim, uint32_t shift,
                                           uint32_t mask, uint32_t* wh, uint32_t (&dg)[ITEMS],
                                           uint32_t (&rk)[ITEMS]) {
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) dg[i] = digit_of(e[i].key, shift, mask);
-  wave_rank_digits<D, ITEMS>(dg, lim, wh, rk);
+  wave_rank_digits<D, ITEMS, SKIP>(dg, lim, wh, rk);
 }
 
 // ------------------------------------------------------------- look-back
