@@ -251,6 +251,74 @@ def traffic_summary(fetch_dir, write_dir, log2n, d, tile, cmd_fetch, cmd_write):
     print("\n".join(lines))
 
 
+def variant_table(log2ns=(24, 26, 28, 30), reps=3):
+    """Every compiled plain-sort variant timed (median of `reps` after one
+    warm-up, HIP events around the whole lsb_sort on its stream) on uniform
+    keys over 64 virtual ranks, as a markdown table with the time per
+    executed pass."""
+    import torch
+    import lsb_amd
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    vs = lsb_amd.variants()
+    forms = {0: "one tile per block", 1: "persistent, one chain", 3: "persistent, 8 regions"}
+    print("| log2 n | key bits | digit | tile | form | ms | ms per pass | Melem/s |")
+    print("|---|---|---|---|---|---|---|---|")
+    for log2n in log2ns:
+        n = 1 << log2n
+        pristine = torch.empty((n, 2), dtype=torch.int64, device=dev)
+        lsb_amd.generate(pristine, kind=lsb_amd.GEN_UNIFORM, ranks=64)
+        A = torch.empty_like(pristine)
+        B = torch.empty_like(pristine)
+        stream = torch.cuda.current_stream()
+        for kb in (32, 64):
+            for v, d, tile, form in vs:
+                ts = []
+                for r in range(reps + 1):
+                    A.copy_(pristine)
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    lsb_amd.sort(A, B, key_bits=kb, variant=v, stream=stream)
+                    e1.record(stream)
+                    e1.synchronize()
+                    if r:
+                        ts.append(e0.elapsed_time(e1))
+                ms = sorted(ts)[len(ts) // 2]
+                passes = (kb + d - 1) // d
+                print(f"| {log2n} | {kb} | {d} | {tile} | {forms.get(form, form)} | {ms:.3f} | "
+                      f"{ms / passes:.3f} | {n / ms / 1e3:.0f} |", flush=True)
+        del pristine, A, B
+        torch.cuda.empty_cache()
+    # The write pattern a pass with 2^d digits lays down, without the
+    # sorting: 8192-element tiles read in order, each written as R runs of
+    # 8192 / R consecutive elements (R = 2^d for uniform keys).
+    n = 1 << log2ns[-1]
+    src = torch.empty((n, 2), dtype=torch.int64, device=dev)
+    lsb_amd.generate(src, kind=lsb_amd.GEN_UNIFORM, ranks=64)
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream()
+    print()
+    print(f"| write runs per 8192-element tile (2^{log2ns[-1]} elements) | elements per run | "
+          "ms | GB/s (32 B per element) |")
+    print("|---|---|---|---|")
+    for runs in (1, 256, 512, 1024, 2048, 4096):
+        ts = []
+        for r in range(reps + 1):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            lsb_amd.debug_scatter_floor(src, dst, runs=runs)
+            e1.record(stream)
+            e1.synchronize()
+            if r:
+                ts.append(e0.elapsed_time(e1))
+        ms = sorted(ts)[len(ts) // 2]
+        print(f"| {runs} | {8192 // runs} | {ms:.3f} | {32 * n / (ms * 1e-3) / 1e9:.0f} |",
+              flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,6 +342,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=500)
     ap.add_argument("--serving-wide-jobs", type=int, default=96)
+    ap.add_argument("--variant-table", action="store_true",
+                    help="only time every compiled plain-sort variant at 2^24..2^30 for 32- "
+                         "and 64-bit keys and print a markdown table")
     args = ap.parse_args()
 
     if args.traffic_summary:
@@ -292,6 +363,10 @@ def main():
         print("bench.py measures one GPU only: the sort is one rank's share on one MI355X "
               "(the multi-GPU exchange is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.variant_table:
+        variant_table()
+        return
 
     base = None
     if not args.no_cpu_baseline:

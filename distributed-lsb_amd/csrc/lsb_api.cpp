@@ -41,7 +41,7 @@ constexpr size_t kAlign = 256;
 inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 struct Plan {
-  uint32_t key_bits, d, tile, passes, nb, ntiles, region_tiles;
+  uint32_t key_bits, d, tile, passes, nb, ntiles, region_tiles, regions;
   lsb::KeyOrder ko;
   const lsb::SortVariant* v;
   // workspace offsets
@@ -79,44 +79,44 @@ std::vector<lsb_tune_entry> g_tune;
 // made by lsb_autotune or lsb_tune_load takes precedence; lsb_tune_clear
 // returns to this one.
 const lsb_tune_entry kBuiltinTune[] = {
-    {12, 64, 8, 2048, 3, 0, 0.0708f, 0.0708f},
-    {13, 64, 8, 2048, 0, 0, 0.0696f, 0.0696f},
-    {14, 64, 8, 2048, 3, 0, 0.0739f, 0.0758f},
-    {15, 64, 8, 2048, 0, 0, 0.0834f, 0.0966f},
-    {16, 64, 8, 2048, 3, 0, 0.0869f, 0.0945f},
-    {17, 64, 8, 2048, 3, 0, 0.0982f, 0.1098f},
-    {18, 64, 8, 2048, 3, 0, 0.1120f, 0.1230f},
-    {19, 64, 8, 4096, 3, 0, 0.1468f, 0.1485f},
-    {20, 64, 8, 4096, 3, 0, 0.1778f, 0.1809f},
-    {21, 64, 8, 8192, 3, 0, 0.1963f, 0.2224f},
-    {22, 64, 8, 8192, 3, 0, 0.3023f, 0.3309f},
-    {23, 64, 8, 8192, 3, 0, 0.5193f, 0.6012f},
-    {24, 64, 8, 8192, 3, 0, 1.0241f, 1.2055f},
-    {25, 64, 8, 8192, 3, 0, 1.9244f, 2.2476f},
-    {26, 64, 8, 8192, 3, 0, 3.6850f, 4.3229f},
-    {27, 64, 8, 8192, 3, 0, 7.2261f, 8.4240f},
-    {28, 64, 8, 8192, 3, 0, 14.4495f, 17.4637f},
-    {29, 64, 8, 8192, 3, 0, 29.0067f, 35.1728f},
-    {30, 64, 8, 8192, 3, 0, 58.6729f, 71.0898f},
-    {12, 32, 8, 2048, 0, 0, 0.0405f, 0.0415f},
-    {13, 32, 8, 2048, 3, 0, 0.0418f, 0.0423f},
-    {14, 32, 8, 2048, 3, 0, 0.0437f, 0.0440f},
-    {15, 32, 8, 2048, 0, 0, 0.0474f, 0.0546f},
-    {16, 32, 8, 2048, 3, 0, 0.0510f, 0.0536f},
-    {17, 32, 8, 2048, 3, 0, 0.0584f, 0.0648f},
-    {18, 32, 8, 2048, 3, 0, 0.0687f, 0.0722f},
-    {19, 32, 8, 4096, 3, 0, 0.0845f, 0.0850f},
-    {20, 32, 8, 8192, 3, 0, 0.0982f, 0.0987f},
-    {21, 32, 8, 8192, 3, 0, 0.1095f, 0.1192f},
-    {22, 32, 8, 8192, 3, 0, 0.1647f, 0.1737f},
-    {23, 32, 8, 8192, 3, 0, 0.2695f, 0.3015f},
-    {24, 32, 8, 8192, 3, 0, 0.5139f, 0.6119f},
-    {25, 32, 8, 8192, 3, 0, 0.9806f, 1.1642f},
-    {26, 32, 8, 8192, 3, 0, 1.8870f, 2.2409f},
-    {27, 32, 8, 8192, 3, 0, 3.6775f, 4.3568f},
-    {28, 32, 8, 8192, 3, 0, 7.3839f, 8.8683f},
-    {29, 32, 8, 8192, 3, 0, 14.8838f, 17.7564f},
-    {30, 32, 8, 8192, 3, 0, 29.8698f, 35.4811f},
+    {12, 64, 8, 2048, 0, 0, 0.0671f, 0.0678f},
+    {13, 64, 8, 2048, 3, 0, 0.0689f, 0.0726f},
+    {14, 64, 8, 2048, 3, 0, 0.0727f, 0.0768f},
+    {15, 64, 8, 2048, 3, 0, 0.0814f, 0.0829f},
+    {16, 64, 8, 2048, 3, 0, 0.0857f, 0.0948f},
+    {17, 64, 8, 2048, 3, 0, 0.0961f, 0.1086f},
+    {18, 64, 8, 2048, 3, 0, 0.1114f, 0.1208f},
+    {19, 64, 8, 4096, 3, 0, 0.1455f, 0.1495f},
+    {20, 64, 8, 8192, 3, 0, 0.1777f, 0.1799f},
+    {21, 64, 8, 8192, 3, 0, 0.1960f, 0.2246f},
+    {22, 64, 8, 8192, 3, 0, 0.3044f, 0.3286f},
+    {23, 64, 8, 8192, 3, 0, 0.5250f, 0.6053f},
+    {24, 64, 8, 8192, 3, 0, 1.0219f, 1.1940f},
+    {25, 64, 8, 8192, 3, 0, 1.9267f, 2.2351f},
+    {26, 64, 8, 8192, 3, 0, 3.7347f, 4.3239f},
+    {27, 64, 8, 8192, 3, 0, 7.3250f, 8.5104f},
+    {28, 64, 8, 8192, 3, 0, 14.4841f, 17.4149f},
+    {29, 64, 8, 8192, 3, 0, 29.1784f, 35.0985f},
+    {30, 64, 8, 8192, 3, 0, 58.8061f, 70.4506f},
+    {12, 32, 8, 2048, 3, 0, 0.0401f, 0.0412f},
+    {13, 32, 8, 2048, 0, 0, 0.0417f, 0.0417f},
+    {14, 32, 8, 2048, 3, 0, 0.0426f, 0.0434f},
+    {15, 32, 8, 2048, 3, 0, 0.0473f, 0.0478f},
+    {16, 32, 8, 2048, 3, 0, 0.0498f, 0.0542f},
+    {17, 32, 8, 2048, 3, 0, 0.0571f, 0.0637f},
+    {18, 32, 8, 2048, 3, 0, 0.0652f, 0.0703f},
+    {19, 32, 8, 2048, 3, 0, 0.0815f, 0.0821f},
+    {20, 32, 8, 8192, 3, 0, 0.0956f, 0.0981f},
+    {21, 32, 8, 8192, 3, 0, 0.1049f, 0.1191f},
+    {22, 32, 8, 8192, 3, 0, 0.1601f, 0.1724f},
+    {23, 32, 8, 8192, 3, 0, 0.2634f, 0.2971f},
+    {24, 32, 8, 8192, 3, 0, 0.5076f, 0.6040f},
+    {25, 32, 8, 8192, 3, 0, 0.9753f, 1.1542f},
+    {26, 32, 8, 8192, 3, 0, 1.8945f, 2.2264f},
+    {27, 32, 8, 8192, 3, 0, 3.6853f, 4.3765f},
+    {28, 32, 8, 8192, 3, 0, 7.3920f, 8.8008f},
+    {29, 32, 8, 8192, 3, 0, 14.8625f, 17.6192f},
+    {30, 32, 8, 8192, 3, 0, 29.6654f, 35.0282f},
 };
 
 uint32_t variant_id(uint32_t d, uint32_t tile, uint32_t persistent) {
@@ -193,7 +193,8 @@ int make_plan(size_t n, const lsb_config& c, Plan* p) {
   p->passes = (p->key_bits + d - 1) / d;
   p->nb = 1u << d;
   p->ntiles = (uint32_t)((n + p->tile - 1) / p->tile);
-  p->region_tiles = (p->ntiles + lsb::kRegions - 1) / lsb::kRegions;
+  p->regions = p->v->persistent == 1 ? 1u : (uint32_t)lsb::kRegions;
+  p->region_tiles = (p->ntiles + p->regions - 1) / p->regions;
   size_t off = 0;
   // counts: [P][R][2^d] words, then the lean histogram's two OR words
   p->counts = off;  off += align_up((size_t)p->passes * lsb::kRegions * p->nb * 4 + 8);
@@ -401,7 +402,7 @@ int enqueue_sort(lsb_elem* A, lsb_elem* B, uint32_t n, const Plan& p, char* ws,
   if ((rc = prof_mark(prof, 0, s))) return rc;
   lsb::launch_zero(counts, count_words + 2, s);
   LSB_HIP(lsb::launch_histogram(p.d, A, n, p.key_bits, p.ko, p.passes, counted, counts,
-                                p.region_tiles * p.tile, status, slice, orr, s));
+                                p.region_tiles * p.tile, p.regions, status, slice, orr, s));
   if ((rc = prof_mark(prof, 1, s))) return rc;
   if (lean)
     LSB_HIP(lsb::launch_plan_first(p.d, A, n, p.key_bits, p.ko, counts, p.region_tiles * p.tile,
@@ -1385,7 +1386,7 @@ int lsb_debug_histogram(const lsb_elem* A, size_t n, const lsb_config* cfg, uint
   LSB_HIP(hipMallocAsync(&tmp, words * 4, s));
   lsb::launch_zero(tmp, words, s);
   if (n) LSB_HIP(lsb::launch_histogram(p.d, A, (uint32_t)n, p.key_bits, p.ko, p.passes, p.passes,
-                                       tmp, (uint32_t)n, nullptr, 0, nullptr, s));
+                                       tmp, (uint32_t)n, 1u, nullptr, 0, nullptr, s));
   for (uint32_t q = 0; q < p.passes; ++q)
     LSB_HIP(hipMemcpyAsync(counts + (size_t)q * p.nb, tmp + (size_t)q * lsb::kRegions * p.nb,
                            p.nb * 4, hipMemcpyDeviceToDevice, s));

@@ -54,7 +54,8 @@ struct SortVariant {
   uint32_t digit_bits;
   uint32_t tile;  // elements per tile (block * items)
   uint32_t block;
-  uint32_t persistent;  // 0 = one tile per block, 3 = region form (persistent)
+  uint32_t persistent;  // 0 = one tile per block, 3 = region form (persistent),
+                        // 1 = persistent with one look-back chain
   ScatterLaunch scatter;
 };
 
@@ -83,10 +84,11 @@ constexpr int kLeanDigits = 2;
 // counted == passes counts every digit; counted == kLeanDigits (d = 8 only)
 // is the lean form, which also ORs every key's XOR with key 0 into orr[0..1]
 // (zeroed beforehand; n >= 1).
+// `regions` (1 .. kRegions) regions of region_elems elements are counted.
 hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
                             const KeyOrder& ko, uint32_t passes, uint32_t counted,
-                            uint32_t* regcount, uint32_t region_elems, uint32_t* zero,
-                            size_t zero_words, uint32_t* orr, hipStream_t s);
+                            uint32_t* regcount, uint32_t region_elems, uint32_t regions,
+                            uint32_t* zero, size_t zero_words, uint32_t* orr, hipStream_t s);
 // Lean form, after the histogram: skip detection from orr, the plan, the
 // cleared tickets[P][R] and regnext[P][R][2^d], and the per-region counts of
 // the first executed pass when it is not one of the `counted` digits (d = 8).

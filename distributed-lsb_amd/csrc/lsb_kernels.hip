@@ -140,8 +140,9 @@ __global__ __launch_bounds__(256) void k_histogram(const elem_t* __restrict__ A,
 template <int D, int P, int L>
 static void launch_hist(const elem_t* a, uint32_t n, uint32_t key_bits, const KeyOrder& ko,
                         uint32_t* regcount, uint32_t region_elems, uint32_t per_region,
-                        uint32_t* zero, size_t zero_words, uint32_t* orr, hipStream_t s) {
-  hipLaunchKernelGGL((k_histogram<D, P, L>), dim3(per_region * kRegions), dim3(256), 0, s, a, n,
+                        uint32_t regions, uint32_t* zero, size_t zero_words, uint32_t* orr,
+                        hipStream_t s) {
+  hipLaunchKernelGGL((k_histogram<D, P, L>), dim3(per_region * regions), dim3(256), 0, s, a, n,
                      key_bits, ko, regcount, region_elems, per_region, zero, zero_words, orr);
 }
 
@@ -149,8 +150,8 @@ static void launch_hist(const elem_t* a, uint32_t n, uint32_t key_bits, const Ke
 // keys per block, and at most two blocks per CU over the whole grid: the
 // 2^30-key histogram on 256 CUs takes 2.82 ms with two long-running blocks
 // per CU, 2.90 ms with four and 3.01 ms with eight (one per CU: 2.80 ms).
-static uint32_t hist_blocks_per_region(uint32_t region_elems) {
-  static const uint32_t cap = [] {
+static uint32_t hist_blocks_per_region(uint32_t region_elems, uint32_t regions = kRegions) {
+  static const uint32_t blocks = [] {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -158,9 +159,9 @@ static uint32_t hist_blocks_per_region(uint32_t region_elems) {
       (void)hipGetLastError();
       cus = 256;
     }
-    const uint32_t c = (uint32_t)(2 * cus / kRegions);
-    return c < 1 ? 1u : c;
+    return (uint32_t)(2 * cus);
   }();
+  const uint32_t cap = blocks / regions < 1 ? 1u : blocks / regions;
   size_t per_region = ((size_t)region_elems + 256 * 8 * 8 - 1) / (256 * 8 * 8);
   if (per_region < 1) per_region = 1;
   if (per_region > cap) per_region = cap;
@@ -169,9 +170,10 @@ static uint32_t hist_blocks_per_region(uint32_t region_elems) {
 
 hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_bits,
                             const KeyOrder& ko, uint32_t passes, uint32_t counted,
-                            uint32_t* regcount, uint32_t region_elems, uint32_t* zero,
-                            size_t zero_words, uint32_t* orr, hipStream_t s) {
-  const uint32_t pr = hist_blocks_per_region(region_elems);
+                            uint32_t* regcount, uint32_t region_elems, uint32_t regions,
+                            uint32_t* zero, size_t zero_words, uint32_t* orr, hipStream_t s) {
+  if (regions < 1 || regions > (uint32_t)kRegions) return hipErrorInvalidValue;
+  const uint32_t pr = hist_blocks_per_region(region_elems, regions);
   const elem_t* a = static_cast<const elem_t*>(A);
   // counted == passes: every digit (orr unused); counted == kLeanDigits with
   // orr: the lean form for the region passes.
@@ -181,9 +183,9 @@ hipError_t launch_histogram(uint32_t d, const void* A, uint32_t n, uint32_t key_
   case PP:                                                                                     \
     if (lean)                                                                                  \
       launch_hist<DD, PP, (PP > kLeanDigits ? kLeanDigits : PP)>(                              \
-          a, n, key_bits, ko, regcount, region_elems, pr, zero, zero_words, orr, s);           \
+          a, n, key_bits, ko, regcount, region_elems, pr, regions, zero, zero_words, orr, s);  \
     else                                                                                       \
-      launch_hist<DD, PP, PP>(a, n, key_bits, ko, regcount, region_elems, pr, zero,            \
+      launch_hist<DD, PP, PP>(a, n, key_bits, ko, regcount, region_elems, pr, regions, zero,   \
                               zero_words, orr, s);                                             \
     break;
   if (d == 8) {
@@ -405,20 +407,27 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(PassArgs args) {
 // tickets for the tiles of region r only and runs that region's look-back
 // chain, so kRegions chains advance side by side.  Each region's bucket
 // bases come from the per-region counts of the previous executed pass (or of
-// the histogram kernel for the first one); the pass itself counts, per
-// region of its output, the digits of the next executed pass.
-template <int D, int BLOCK, int ITEMS>
+// the histogram kernel for the first one); with NEXT the pass itself counts,
+// per region of its output, the digits of the next executed pass.  Without
+// NEXT every pass's per-region counts come from the histogram kernel, which
+// then counts every digit up front: the form for wide digits, whose
+// per-region next-pass counters (kRegions x 2^d words) would not fit in LDS
+// beside the tile.  That is exact only while the per-region counts do not
+// depend on the order a pass leaves behind: with one region (R = 1, one
+// look-back chain over the whole array).
+template <int D, int BLOCK, int ITEMS, bool NEXT = true, int R = kRegions>
 __global__ __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(PassArgs args) {
   constexpr int NB = 1 << D;
   static_assert(kRegions == kRegionsDev, "region count");
+  static_assert(NEXT || R == 1, "counts from the histogram hold for one region only");
   __shared__ TileSmem<D, BLOCK, ITEMS> sm;
   __shared__ uint32_t s_rbase[NB];
-  __shared__ uint32_t s_hnext[kRegions * NB];
+  __shared__ uint32_t s_hnext[NEXT ? kRegions * NB : 1];
   // Per-wave counters get their own LDS when a block fills the CU alone
-  // (512+ threads); smaller blocks keep them in the staging buffer so two
-  // blocks still fit.
-  constexpr bool kOwnCounters = BLOCK >= 512;
+  // (512+ threads) and 8-bit digits leave room; otherwise they live in the
+  // staging buffer (so two small blocks still fit).
+  constexpr bool kOwnCounters = BLOCK >= 512 && D <= 8;
   __shared__ uint32_t s_whist[kOwnCounters ? (BLOCK / kWave) * NB : 1];
   const uint32_t* plan = args.plan;
   const uint32_t pass = args.pass;
@@ -430,7 +439,7 @@ __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(Pass
   const size_t slice = (size_t)args.ntiles * NB;
   uint32_t* st_cur = args.status + (size_t)(ord & 1u) * slice;
   uint32_t* st_next = args.status + (size_t)((ord + 1u) & 1u) * slice;
-  const uint32_t region = blockIdx.x % kRegions;
+  const uint32_t region = blockIdx.x % (uint32_t)R;
   const uint32_t t_begin = region * args.region_tiles;
   if (t_begin >= args.ntiles) return;
   const uint32_t t_count =
@@ -439,11 +448,12 @@ __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(Pass
 
   // the next executed pass, whose per-region counts this pass produces
   int q = -1;
-  for (uint32_t p2 = pass + 1; p2 < args.passes; ++p2)
-    if (plan[4 * p2]) {
-      q = (int)p2;
-      break;
-    }
+  if (NEXT)
+    for (uint32_t p2 = pass + 1; p2 < args.passes; ++p2)
+      if (plan[4 * p2]) {
+        q = (int)p2;
+        break;
+      }
   uint32_t nshift = 0, nmask = 0;
   if (q >= 0) {
     nshift = (uint32_t)q * D;
@@ -452,7 +462,8 @@ __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(Pass
   }
   // Bucket b of region r starts at the global start of b (exclusive scan of
   // the counts summed over regions) plus b's count in regions before r.
-  const uint32_t* C = (ord == 0 ? args.regcount : args.regnext) + (size_t)pass * kRegions * NB;
+  const uint32_t* C =
+      (!NEXT || ord == 0 ? args.regcount : args.regnext) + (size_t)pass * kRegions * NB;
   for (uint32_t b = threadIdx.x; b < (uint32_t)NB; b += BLOCK) {
     uint32_t before = 0, all = 0;
 #pragma unroll
@@ -466,7 +477,8 @@ __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(Pass
   }
   block_exclusive_scan<NB, BLOCK>(sm.start, sm.wsum);
   for (uint32_t b = threadIdx.x; b < (uint32_t)NB; b += BLOCK) s_rbase[b] += sm.start[b];
-  for (uint32_t i = threadIdx.x; i < (uint32_t)(kRegions * NB); i += BLOCK) s_hnext[i] = 0u;
+  if (NEXT)
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(kRegions * NB); i += BLOCK) s_hnext[i] = 0u;
   if (kOwnCounters)
     for (uint32_t i = threadIdx.x; i < (uint32_t)((BLOCK / kWave) * NB); i += BLOCK) s_whist[i] = 0u;
   uint32_t* ticket = &args.tickets[pass * kRegions + region];
@@ -502,7 +514,7 @@ __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(Pass
                                 nmask);
     lds_barrier();
   }
-  if (q >= 0) {
+  if (NEXT && q >= 0) {
     uint32_t* out = args.regnext + (size_t)q * kRegions * NB;
     for (uint32_t i = threadIdx.x; i < (uint32_t)(kRegions * NB); i += BLOCK) {
       const uint32_t c = s_hnext[i];
@@ -511,24 +523,26 @@ __attribute__((amdgpu_waves_per_eu(ITEMS <= 8 ? 3 : 2))) void k_onesweep_rg(Pass
   }
 }
 
-template <int D, int BLOCK, int ITEMS>
+template <int D, int BLOCK, int ITEMS, bool NEXT = true, int R = kRegions>
 static void launch_onesweep_rg(const PassArgs& a, hipStream_t s) {
   // resident blocks per region, found once (thread-safe static init)
   static const uint32_t per_region = [] {
     int dev = 0, cus = 0, occ = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_onesweep_rg<D, BLOCK, ITEMS>, BLOCK,
-                                                     0) != hipSuccess) {
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ,
+                                                     k_onesweep_rg<D, BLOCK, ITEMS, NEXT, R>,
+                                                     BLOCK, 0) != hipSuccess) {
       (void)hipGetLastError();
       cus = 256;
       occ = 1;
     }
-    const uint32_t pr = (uint32_t)((cus * (occ > 0 ? occ : 1)) / kRegions);
+    const uint32_t pr = (uint32_t)((cus * (occ > 0 ? occ : 1)) / R);
     return pr < 1 ? 1u : pr;
   }();
   const uint32_t pr = a.region_tiles < per_region ? a.region_tiles : per_region;
-  hipLaunchKernelGGL((k_onesweep_rg<D, BLOCK, ITEMS>), dim3(pr * kRegions), dim3(BLOCK), 0, s, a);
+  hipLaunchKernelGGL((k_onesweep_rg<D, BLOCK, ITEMS, NEXT, R>), dim3(pr * R), dim3(BLOCK), 0, s,
+                     a);
 }
 
 template <int D, int BLOCK, int ITEMS>
@@ -537,7 +551,8 @@ static void launch_onesweep(const PassArgs& a, hipStream_t s) {
 }
 
 // Variants are (digit width, tile, block, form, launcher); form 3 is the
-// region form, 0 the one-tile-per-block form.  The first one per digit
+// region form, 1 the persistent form with one look-back chain (every pass's
+// counts from the histogram kernel), 0 the one-tile-per-block form.  The first one per digit
 // width is the fallback for that width when no tuned choice applies.
 static const SortVariant kVariants[] = {
     {8, 1024 * 8, 1024, 3, &launch_onesweep_rg<8, 1024, 8>},
@@ -546,6 +561,7 @@ static const SortVariant kVariants[] = {
     {8, 256 * 16, 256, 0, &launch_onesweep<8, 256, 16>},
     {8, 512 * 16, 512, 0, &launch_onesweep<8, 512, 16>},
     {8, 256 * 8, 256, 0, &launch_onesweep<8, 256, 8>},
+    {11, 512 * 12, 512, 1, &launch_onesweep_rg<11, 512, 12, false, 1>},
     {11, 512 * 12, 512, 0, &launch_onesweep<11, 512, 12>},
 };
 
