@@ -529,8 +529,9 @@ def _timed(fn):
 def bench_batched(nseg, seed=2024, reps=3):
     """Batched segmented sort: nseg independent arrays, sizes log-uniform over
     [2^10, 2^22], uniform 64-bit keys, val = index within the segment; one
-    segmented launch sequence (replayed from a hipGraph) against one plain
-    lsb_sort launch per segment, both on HBM-resident data."""
+    segmented launch sequence (replayed from a hipGraph; most-significant-
+    digit-first) against the LSD segmented form and one plain lsb_sort
+    launch per segment, all on HBM-resident data."""
     import torch
     import lsb_amd
     from lsb_amd import workloads as W
@@ -553,6 +554,15 @@ def bench_batched(nseg, seed=2024, reps=3):
         if r:
             ts.append(t)
     t_batched = sorted(ts)[len(ts) // 2]
+    # the LSD segmented form (LSB_SEG_LSD: one onesweep pass per digit over
+    # every large segment) on the same input, for comparison
+    tl = []
+    for r in range(2):
+        A.copy_(pristine)
+        tl.append(_timed(lambda: lsb_amd.sort_segmented(A, B, off, graph=True, lsd=True)))
+    t_lsd = min(tl)
+    A.copy_(pristine)
+    lsb_amd.sort_segmented(A, B, off, graph=True)
     # spot-check segments (the test suite checks all of them)
     ok = True
     keys = torch.empty(max(sizes), dtype=torch.int64, device=dev)
@@ -577,7 +587,9 @@ def bench_batched(nseg, seed=2024, reps=3):
             "melem_s": round(total / t_batched / 1e6, 1),
             "plain_per_segment_ms": round(t_plain * 1e3, 3),
             "plain_per_segment_melem_s": round(total / t_plain / 1e6, 1),
-            "speedup_vs_plain": round(t_plain / t_batched, 3), "verified_sample": ok}
+            "speedup_vs_plain": round(t_plain / t_batched, 3),
+            "lsd_form_ms": round(t_lsd * 1e3, 3), "msd_vs_lsd_form": round(t_lsd / t_batched, 3),
+            "verified_sample": ok}
 
 
 def bench_serving(jobs, seed=77, lo_log2=12, span_log2=16):

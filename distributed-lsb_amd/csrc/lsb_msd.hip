@@ -60,12 +60,32 @@ __device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b 
 // (it keeps the buckets in order) and enough digits below it that a bucket of
 // the average size m = n / nb has few keys left tied for the fix-up: with
 // uniform keys about m^2 / 2^(8d+1) tied pairs per bucket, so d digits with
-// 2^(8d) >= 4m leave under one tie per eight elements.
+// 2^(8d) >= 2m leave about one tied pair per four elements, which the fix-up's
+// transposition rounds sort.
 __device__ __forceinline__ uint32_t finish_passes(uint32_t n, uint32_t nb) {
   const uint32_t m = n / (nb ? nb : 1u);
   uint32_t d = 1;
-  while (d < 3 && (1ull << (8 * d)) < 4ull * m) ++d;
+  while (d < 3 && (1ull << (8 * d)) < 2ull * m) ++d;
   return 1u + d;
+}
+
+// LDS passes for a run of the wide partition, over its digits `digits` from
+// the top: digit p0 (holding the wide digit's lowest bits) sorts only its
+// bits below the wide digit (fshift - 8 p0 of them) inside a bucket.  As
+// many digits as bring the bits sorted inside a bucket of the average size
+// m to 2^bits >= 2m (about one tied pair per four elements for uniform
+// keys; the fix-up's transposition rounds take them).
+__device__ __forceinline__ uint32_t wide_finish_passes(uint32_t digits, uint32_t fshift,
+                                                       uint32_t m) {
+  uint32_t want = 0, bits = 0;
+  for (int p = 7; p >= 0; --p) {
+    if (!((digits >> p) & 1u)) continue;
+    ++want;
+    const uint32_t lo = 8u * (uint32_t)p, hi = umin(lo + 8u, fshift);
+    bits += hi > lo ? hi - lo : 0u;
+    if (bits >= 40u || (1ull << bits) >= 2ull * m) break;
+  }
+  return want;
 }
 
 // Appends one range to level k1's list (a single thread): its entry, its
@@ -927,7 +947,7 @@ __global__ __launch_bounds__(256) void k_msd_wplan(MsdArgs a) {
       uint32_t digits = 0, passes = 0;
       if (cd >= 0) {
         digits = one ? lower : (lower | (1u << p0));
-        passes = finish_passes(s_rn[r], s_rb[r]) - (one ? 1u : 0u);
+        passes = wide_finish_passes(digits, fshift, s_rn[r] / s_rb[r]);
       }
       MsdRun run = {E.b + 16ull * s_rlo[r], E.a + 16ull * s_rlo[r], s_rn[r],
                     digits | (passes << 16)};
