@@ -513,10 +513,16 @@ struct SegLayout {
 
 // The MSD-first form sorts the large segments unless the caller asks for the
 // LSD passes (LSB_SEG_LSD), names a digit width other than 8 or a tile, or
-// sets LSB_NO_SKIP (run every digit pass).
+// sets LSB_NO_SKIP (run every digit pass).  Unless asked for (LSB_SEG_MSD)
+// it is used for keys wider than 32 bits only: at 32 bits and below the LSD
+// form's 4 passes move as many bytes as MSD's partitions and finish, run as
+// fast on uniform keys and skip the constant digits of skewed ones (Zipf
+// keys: 34 against 25 Gelem/s over a serving stream's jobs).
 bool wants_msd(const lsb_config& c) {
-  return !(c.flags & (LSB_SEG_LSD | LSB_NO_SKIP)) && (c.digit_bits == 0 || c.digit_bits == 8) &&
-         c.tile == 0;
+  if ((c.flags & (LSB_SEG_LSD | LSB_NO_SKIP)) || !(c.digit_bits == 0 || c.digit_bits == 8) ||
+      c.tile != 0)
+    return false;
+  return (c.flags & LSB_SEG_MSD) || c.key_bits == 0 || c.key_bits > 32;
 }
 
 // Capacities of the MSD-first form for batches of at most cap_n elements.

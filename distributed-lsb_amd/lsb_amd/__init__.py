@@ -23,6 +23,7 @@ LSB_OK, LSB_EINVAL, LSB_ENOMEM, LSB_EHIP, LSB_ERANGE, LSB_EBUSY, LSB_ENOTFOUND =
 LSB_SYNC, LSB_NO_SKIP, LSB_PROFILE, LSB_GRAPH = 1, 2, 4, 8
 LSB_KEYS_SIGNED, LSB_KEYS_FLOAT = 16, 32
 LSB_SEG_LSD = 64
+LSB_SEG_MSD = 128
 # key_order names accepted by the sort, verify and server calls
 KEY_ORDERS = {"unsigned": 0, "signed": LSB_KEYS_SIGNED, "float": LSB_KEYS_FLOAT}
 LSB_MAX_N = 0x7FFFFFFE
@@ -283,19 +284,26 @@ def noalloc_scratch_bytes():
     return lib().lsb_noalloc_scratch_bytes()
 
 
+def _form_flag(lsd):
+    """Batched form: None (the library's choice), True (LSD), False (MSD first)."""
+    if lsd is None:
+        return 0
+    return LSB_SEG_LSD if lsd else LSB_SEG_MSD
+
+
 def sort_segmented(A, B, offsets, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,
                    no_skip=False, graph=False, workspace=None, workspace_bytes=0,
-                   key_order="unsigned", lsd=False):
+                   key_order="unsigned", lsd=None):
     """Sorts every segment [offsets[s], offsets[s+1]) of A independently and
     stably by key; B is scratch; results in A.  offsets: host sequence of
     nseg + 1 non-decreasing element offsets.  lsd=True sorts the large
-    segments with LSD passes instead of most significant digit first."""
+    segments with LSD passes, lsd=False most significant digit first, None
+    the library's choice (MSD first for keys wider than 32 bits)."""
     import numpy as _np
     off = _np.ascontiguousarray(_np.asarray(offsets, dtype=_np.uint64))
     nseg = len(off) - 1
     flags = ((LSB_SYNC if sync else 0) | (LSB_NO_SKIP if no_skip else 0)
-             | (LSB_GRAPH if graph else 0) | (LSB_SEG_LSD if lsd else 0)
-             | _key_flags(key_order))
+             | (LSB_GRAPH if graph else 0) | _form_flag(lsd) | _key_flags(key_order))
     c = make_config(key_bits, digit_bits, tile, stream, flags, workspace, workspace_bytes)
     _check(lib().lsb_sort_segmented(_ptr(A), _ptr(B),
                                     off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
@@ -310,10 +318,11 @@ def release(stream=None):
 
 
 def sort_batch(As, Bs, ns=None, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,
-               no_skip=False, graph=False, key_order="unsigned", lsd=False):
+               no_skip=False, graph=False, key_order="unsigned", lsd=None):
     """Sorts each buffer As[i] (its first ns[i] elements) on its own, stably
     by key, with Bs[i] as its scratch (None allowed for at most 8192
-    elements), in one batched launch sequence without copying anything."""
+    elements), in one batched launch sequence without copying anything.
+    lsd: as for sort_segmented."""
     nseg = len(As)
     if ns is None:
         ns = [_nelem(a) for a in As]
@@ -321,8 +330,7 @@ def sort_batch(As, Bs, ns=None, key_bits=64, digit_bits=0, tile=0, stream=None, 
     Bp = (ctypes.c_void_p * max(nseg, 1))(*[(_ptr(b) if b is not None else None) for b in Bs])
     Np = (ctypes.c_uint64 * max(nseg, 1))(*ns)
     flags = ((LSB_SYNC if sync else 0) | (LSB_NO_SKIP if no_skip else 0)
-             | (LSB_GRAPH if graph else 0) | (LSB_SEG_LSD if lsd else 0)
-             | _key_flags(key_order))
+             | (LSB_GRAPH if graph else 0) | _form_flag(lsd) | _key_flags(key_order))
     c = make_config(key_bits, digit_bits, tile, stream, flags)
     _check(lib().lsb_sort_batch(Ap, Bp, Np, nseg, ctypes.byref(c)), "lsb_sort_batch")
 

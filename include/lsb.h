@@ -63,11 +63,15 @@ typedef struct lsb_elem {
  * (chpl/arkouda-radix-sort.chpl:37-39). */
 #define LSB_KEYS_SIGNED 16u
 #define LSB_KEYS_FLOAT 32u
-/* Batched sorts (lsb_sort_segmented, lsb_sort_batch): sort the segments
- * larger than one workgroup's capacity with LSD passes over every digit,
- * like lsb_sort, instead of the default most-significant-digit-first form
- * (same results; see lsb_sort_segmented). */
+/* Batched sorts (lsb_sort_segmented, lsb_sort_batch): the form for the
+ * segments larger than one workgroup's capacity (same results either way;
+ * see lsb_sort_segmented).  LSB_SEG_LSD: LSD passes over every digit, like
+ * lsb_sort.  LSB_SEG_MSD: most significant digit first.  Neither: MSD first
+ * for keys wider than 32 bits, LSD for the others (on MI355X the two run
+ * level for uniform 32-bit keys and LSD is faster for skewed ones).
+ * LSB_SEG_LSD takes precedence. */
 #define LSB_SEG_LSD 64u
+#define LSB_SEG_MSD 128u
 
 typedef struct lsb_config {
   uint32_t key_bits;          /* sort by the low key_bits bits of key (1..64); 0 = 64.
@@ -168,11 +172,12 @@ int lsb_last_hip_error(void);
  * non-decreasing element offsets; each segment holds at most LSB_MAX_N
  * elements.  One launch sequence covers the whole batch: segments of up to
  * one workgroup's capacity (8192 elements) are sorted entirely in LDS.
- * Larger ones are by default partitioned most significant digit first with
- * persistent onesweep passes whose look-back stays inside each range, until
- * every bucket fits one workgroup and is finished in LDS; with LSB_SEG_LSD,
- * an explicit tile, a digit width other than 8 or LSB_NO_SKIP they run LSD
- * passes over every digit instead.  Both give each segment its stable sort
+ * Larger ones are partitioned most significant digit first (LSB_SEG_MSD;
+ * the default for keys wider than 32 bits) with persistent onesweep passes
+ * whose look-back stays inside each range, until every bucket fits one
+ * workgroup and is finished in LDS; with LSB_SEG_LSD (the default for keys
+ * of at most 32 bits), an explicit tile, a digit width other than 8 or
+ * LSB_NO_SKIP they run LSD passes over every digit instead.  Both give each segment its stable sort
  * by key.  Each segment is one rank's mySort (mpi/mpi_lsbsort.cpp:591-597);
  * the reference has no batched form.  With LSB_GRAPH the sequence is a
  * cached hipGraph that is replayed for every batch fitting the library

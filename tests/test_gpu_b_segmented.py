@@ -448,7 +448,7 @@ def test_msd_first_key_widths(key_bits):
     want = expected(a, off, key_bits)
     As = [to_dev(a[int(off[i]):int(off[i + 1])]) for i in range(len(sizes))]
     Bs = [torch.empty_like(x) for x in As]
-    lsb_amd.sort_batch(As, Bs, key_bits=key_bits, sync=True)
+    lsb_amd.sort_batch(As, Bs, key_bits=key_bits, sync=True, lsd=False)
     for i, X in enumerate(As):
         assert np.array_equal(from_dev(X), want[int(off[i]):int(off[i + 1])]), (key_bits, i)
 
@@ -516,8 +516,25 @@ def test_msd_wide_partition_key_widths(key_bits):
     want = expected(a, off, key_bits)
     A = to_dev(a)
     B = torch.empty_like(A)
-    lsb_amd.sort_segmented(A, B, off, key_bits=key_bits, sync=True)
+    lsb_amd.sort_segmented(A, B, off, key_bits=key_bits, sync=True, lsd=False)
     got = from_dev(A)
     for s in range(len(sizes)):
         lo, hi = int(off[s]), int(off[s + 1])
         assert np.array_equal(got[lo:hi], want[lo:hi]), (key_bits, s)
+
+
+@pytest.mark.parametrize("key_bits", [20, 32])
+def test_automatic_form_for_narrow_keys(key_bits):
+    """Keys of at most 32 bits take the LSD form unless MSD is asked for;
+    all three calls give every segment its stable sort."""
+    sizes = [9_000, 300_000, 2_500_000, 70_000]
+    a, off = batch(sizes, seed=300 + key_bits, kind="zipf")
+    want = expected(a, off, key_bits)
+    for lsd in (None, True, False):
+        A = to_dev(a)
+        B = torch.empty_like(A)
+        lsb_amd.sort_segmented(A, B, off, key_bits=key_bits, sync=True, lsd=lsd)
+        got = from_dev(A)
+        for s in range(len(sizes)):
+            lo, hi = int(off[s]), int(off[s + 1])
+            assert np.array_equal(got[lo:hi], want[lo:hi]), (key_bits, lsd, s)
