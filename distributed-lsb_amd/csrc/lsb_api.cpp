@@ -565,7 +565,7 @@ int make_msd_layout(size_t cap_n, SegLayout* L) {
   L->worr = off;    off += align_up(wide * 16);
   L->wtile = off;   off += align_up(wtiles * 4);
   L->wstatus = off; off += align_up(wtiles * wnb * 4);
-  L->wlvl = off;    off += align_up(sizeof(lsb::MsdLevel));
+  L->wlvl = off;    off += align_up(2 * sizeof(lsb::MsdLevel));
   L->total = off;
   return LSB_OK;
 }
@@ -715,11 +715,17 @@ int enqueue_segmented(const SegLayout& L, char* ws, uint32_t flags, hipStream_t 
     m.ko = L.ko;
     m.level = 0;
     LSB_HIP(lsb::launch_msd_init(m, 4 * cus, s));
+    m.wstep = 0;
     LSB_HIP(lsb::launch_msd_wide(m, cus, s));
-    // levels 0 .. P: the last one only copies ranges left in B
+    // levels 0 .. P: the last one only copies ranges left in B; after level
+    // 0 the wide step for its children of the wide size range
     for (uint32_t k = 0; k <= L.passes && k < (uint32_t)lsb::kMsdMaxLevels; ++k) {
       m.level = k;
       LSB_HIP(lsb::launch_msd_level(m, cus, s));
+      if (k == 0) {
+        m.wstep = 1;
+        LSB_HIP(lsb::launch_msd_wide(m, cus, s));
+      }
     }
     LSB_HIP(lsb::launch_msd_finish(m, cus, s));
     return LSB_OK;

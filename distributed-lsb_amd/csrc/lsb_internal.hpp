@@ -245,13 +245,18 @@ struct MsdArgs {
   MsdRun* runs;
   uint32_t* run_ctr;       // [0] runs emitted, [1] finish ticket, [2] overflow
   uint32_t cap_entries, cap_tiles, cap_runs;
-  // the wide first partition (11-bit digit) of the segments the host marked
+  // Wide partitions (kMsdWideBits-bit digit) of ranges of kMsdWideMin ..
+  // kMsdWideMax elements, in two steps: step 0 takes the segments the host
+  // marked and sends its large buckets to level 0; step 1 takes level 0's
+  // children of that size and sends its large buckets to level 1.  An entry's
+  // `dig` is the wide digit's lowest bit; kMsdInB says where its elements are.
   MsdEntry* went;          // [wide] (tile_begin: first tile among wide tiles)
-  uint32_t* wcounts;       // [wide][2048] digit counts, then bucket starts
+  uint32_t* wcounts;       // [wide][2^kMsdWideBits] digit counts, then bucket starts
   uint32_t* worr;          // [wide][4] OR and AND of its keys
-  uint32_t* wtile;         // wide tile -> wide segment
-  uint32_t* wstatus;       // [wide tile][2048] look-back words
-  MsdLevel* wlvl;          // entries, tiles, ticket of the wide partition
+  uint32_t* wtile;         // wide tile -> wide range
+  uint32_t* wstatus;       // [wide tile][2^kMsdWideBits] look-back words
+  MsdLevel* wlvl;          // [2] entries, tiles, ticket of each step
+  uint32_t wstep;          // the step a wide launch runs (0 or 1)
   uint32_t cap_wide, cap_wide_tiles;
   uint32_t key_bits;
   KeyOrder ko;

@@ -140,7 +140,9 @@ __global__ __launch_bounds__(256) void k_msd_init(MsdArgs a) {
   if (blockIdx.x == 0 && t == 0) {
     MsdLevel w = {umin(a.hdr->n_wide, a.cap_wide), umin(a.hdr->wide_tiles, a.cap_wide_tiles),
                   0u, 0u};
-    *a.wlvl = w;
+    a.wlvl[0] = w;
+    MsdLevel z = {0u, 0u, 0u, 0u};
+    a.wlvl[1] = z;
   }
   // A segment marked wide goes to the wide partition; its level-0 entry is
   // left empty and its level-0 tiles map to no entry.
@@ -152,7 +154,8 @@ __global__ __launch_bounds__(256) void k_msd_init(MsdArgs a) {
       MsdEntry e = {L.a, L.b, wide ? 0u : L.n, wide ? 0xFFFFFFFFu : L.tile_begin, P - 1u, 0u};
       a.ent[0][s] = e;
       if (wide) {
-        MsdEntry w = {L.a, L.b, L.n, L.perm_pos, 0u, 0u};
+        // the key's top kMsdWideBits bits, elements in A
+        MsdEntry w = {L.a, L.b, L.n, L.perm_pos, a.key_bits - kMsdWideBits, 0u};
         a.went[L.wide] = w;
       }
     }
@@ -178,8 +181,8 @@ __global__ __launch_bounds__(256) void k_msd_init(MsdArgs a) {
 // from one lane.  The block's part of the tile map is staged in LDS and a
 // range's entry is read once for its consecutive tiles, so a tile's key
 // loads wait on no other load.
-// NB = 256: each range's own 8-bit digit E.dig; NB = 2^kMsdWideBits (the
-// wide partition): the top kMsdWideBits bits of the key.
+// NB = 256: each range's own 8-bit digit E.dig; NB = 2^kMsdWideBits (a
+// wide partition): kMsdWideBits bits from bit E.dig up.
 template <int NB>
 __device__ __forceinline__ void msd_hist(const MsdEntry* __restrict__ ent,
                                          uint32_t* __restrict__ counts,
@@ -248,7 +251,7 @@ __device__ __forceinline__ void msd_hist(const MsdEntry* __restrict__ ent,
       if (e >= nent || (E.flags & kMsdCopy)) continue;
       if (tl < E.tile_begin || (uint64_t)(tl - E.tile_begin) * kMsdTile >= E.n) continue;
       const elem_t* src = reinterpret_cast<const elem_t*>((E.flags & kMsdInB) ? E.b : E.a);
-      const uint32_t shift = NB == kNB ? E.dig * 8u : key_bits - kMsdWideBits;
+      const uint32_t shift = NB == kNB ? E.dig * 8u : E.dig;
       const uint32_t mask = NB == kNB ? digit_mask(E.dig, key_bits) : (uint32_t)NB - 1u;
       const uint32_t first = (tl - E.tile_begin) * kMsdTile;
       const uint32_t cnt = umin(kMsdTile, E.n - first);
@@ -295,11 +298,12 @@ __global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
                 a.key_bits, a.ko);
 }
 
-// The wide partition's histogram: the marked segments' top kMsdWideBits bits.
+// A wide step's histogram: each range's kMsdWideBits-bit digit.
 __global__ __launch_bounds__(256) void k_msd_whist(MsdArgs a) {
+  const MsdLevel* wl = &a.wlvl[a.wstep];
   msd_hist<(1 << kMsdWideBits)>(a.went, a.wcounts, a.worr, a.wtile, a.wstatus,
-                                umin(a.wlvl->tiles, a.cap_wide_tiles),
-                                umin(a.wlvl->entries, a.cap_wide), a.key_bits, a.ko);
+                                umin(wl->tiles, a.cap_wide_tiles),
+                                umin(wl->entries, a.cap_wide), a.key_bits, a.ko);
 }
 
 // ----------------------------------------------------------- k_msd_plan
@@ -347,7 +351,12 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
     const bool out_b = !in_b;
     const uint32_t c = a.counts[L][(size_t)e * kNB + t];
     const bool large = c > C;
-    const bool emit = large && (cd >= 0 || out_b);
+    // At level 0 a bucket of the wide size range whose next varying digit
+    // leaves room for a wide digit goes to wide step 1 instead of level 1:
+    // its buckets then finish after one more partition.
+    const bool to_wide = k == 0 && large && c > kMsdWideMin && c <= kMsdWideMax && cd >= 1 &&
+                         a.key_bits >= kMsdWideKeyBits;
+    const bool emit = large && !to_wide && (cd >= 0 || out_b);
     const uint32_t nt = (c + kMsdTile - 1) / kMsdTile;
     const uint32_t my_q = large ? C + 1 : c;
     const uint32_t my_nl = (large ? 0u : 1u) | (c ? 0x10000u : 0u);
@@ -379,6 +388,28 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
       a.ent[L][e].flags = E.flags | kMsdMove;
     }
     any_small = __syncthreads_or(any_small);
+    if (to_wide) {
+      constexpr uint32_t WNB = 1u << kMsdWideBits;
+      const uint32_t nt = (c + kMsdTile - 1) / kMsdTile;
+      const uint64_t ca = E.a + 16ull * start, cb = E.b + 16ull * start;
+      const uint32_t id = atomicAdd(&a.wlvl[1].entries, 1u);
+      const uint32_t tb = atomicAdd(&a.wlvl[1].tiles, nt);
+      if (id >= a.cap_wide || (uint64_t)tb + nt > a.cap_wide_tiles) {
+        atomicAdd(&a.run_ctr[2], 1u);  // cannot happen within the workspace's capacities
+        if (id < a.cap_wide) {
+          MsdEntry dead = {ca, cb, 0u, 0xFFFFFFFFu, 0u, 0u};
+          a.went[id] = dead;
+        }
+      } else {
+        MsdEntry w = {ca, cb, c, tb, 8u * (uint32_t)cd + 8u - kMsdWideBits,
+                      out_b ? kMsdInB : 0u};
+        a.went[id] = w;
+        uint4* z = reinterpret_cast<uint4*>(a.wcounts + (size_t)id * WNB);
+        for (uint32_t i = 0; i < WNB / 4; ++i) z[i] = make_uint4(0u, 0u, 0u, 0u);
+        reinterpret_cast<uint4*>(a.worr)[id] = make_uint4(0u, 0u, ~0u, ~0u);
+        for (uint32_t i = 0; i < nt; ++i) a.wtile[tb + i] = id;
+      }
+    }
     if (emit && k + 1 < (uint32_t)kMsdMaxLevels) {
       const uint32_t N = (k + 1) & 1u;
       const uint32_t id = s_misc[0] + (s_emit[t] & 511u);
@@ -795,10 +826,10 @@ __global__ __launch_bounds__(256) void k_msd_wplan(MsdArgs a) {
   constexpr uint32_t C = kMsdFinishCap;
   constexpr int WNB = 1 << kMsdWideBits, PER = WNB / 256;
   const uint32_t t = threadIdx.x;
-  const uint32_t nw = umin(a.wlvl->entries, a.cap_wide);
+  const uint32_t nw = umin(a.wlvl[a.wstep].entries, a.cap_wide);
+  const uint32_t kt = a.wstep;  // the level its large buckets join
   const uint32_t kb = a.key_bits;
   const uint64_t kbm = low_bits(kb);
-  const uint32_t fshift = kb - kMsdWideBits;  // the wide digit's low bit
   __shared__ uint32_t s_start[WNB];
   __shared__ uint32_t s_q[WNB + 1];   // exclusive prefix of min(count, C + 1)
   __shared__ uint32_t s_nl[WNB + 1];  // small buckets (low 16 bits), non-empty ones (high 16)
@@ -809,15 +840,27 @@ __global__ __launch_bounds__(256) void k_msd_wplan(MsdArgs a) {
   __shared__ uint32_t s_misc[4];
   for (uint32_t w = blockIdx.x; w < nw; w += gridDim.x) {
     const MsdEntry E = a.went[w];
+    const uint32_t fshift = E.dig;  // the wide digit's low bit
+    const bool in_b = (E.flags & kMsdInB) != 0;
+    const bool out_b = !in_b;       // where the partition puts the elements
     const uint4 oa = reinterpret_cast<const uint4*>(a.worr)[w];
-    const uint64_t x = ((((uint64_t)oa.y << 32) | oa.x) ^ (((uint64_t)oa.w << 32) | oa.z)) & kbm;
-    if (x == 0) continue;  // every key equal: already sorted, in A
+    const uint64_t x = ((((uint64_t)oa.y << 32) | oa.x) ^ (((uint64_t)oa.w << 32) | oa.z)) & kbm &
+                       low_bits(fshift + kMsdWideBits);
+    if (x == 0) {
+      // every key equal: done in A, or copied there by level kt's scatter
+      if (in_b && t == 0) msd_emit(a, kt, E.a, E.b, E.n, 0u, kMsdInB | kMsdCopy);
+      continue;
+    }
     if (((x >> fshift) & (uint64_t)(WNB - 1)) == 0) {
-      // the wide digit is constant: level 0 partitions on the next varying one
-      if (t == 0) msd_emit(a, 0, E.a, E.b, E.n, (uint32_t)top_varying_digit(x, 8), 0u);
+      // the wide digit is constant: level kt partitions on the next varying one
+      if (t == 0)
+        msd_emit(a, kt, E.a, E.b, E.n, (uint32_t)top_varying_digit(x, 8), E.flags & kMsdInB);
       continue;
     }
     const int cd = top_varying_digit(x & low_bits(fshift), 8);
+    // Buckets of more than C elements join level kt (with the next varying
+    // digit, or, when no digit below varies and they land in B, as copies).
+    const bool emit_large = cd >= 0 || out_b;
     uint32_t c[PER];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -828,7 +871,7 @@ __global__ __launch_bounds__(256) void k_msd_wplan(MsdArgs a) {
       s_start[b] = c[i];
       s_q[b] = large ? C + 1 : c[i];
       s_nl[b] = (large ? 0u : 1u) | (c[i] ? 0x10000u : 0u);
-      s_emit[b] = large ? ((nt << 12) | 1u) : 0u;  // every bucket lands in B
+      s_emit[b] = large && emit_large ? ((nt << 12) | 1u) : 0u;
     }
     block_exclusive_scan<WNB, 256>(s_start, s_wsum);
     block_exclusive_scan<WNB, 256>(s_q, s_wsum);
@@ -845,17 +888,19 @@ __global__ __launch_bounds__(256) void k_msd_wplan(MsdArgs a) {
       const uint32_t lb = WNB - 1, lc = c[PER - 1];
       s_q[WNB] = s_q[lb] + (lc > C ? C + 1 : lc);
       s_nl[WNB] = s_nl[lb] + ((lc > C ? 0u : 1u) | (lc ? 0x10000u : 0u));
-      const uint32_t tot = s_emit[lb] + (lc > C ? (((lc + kMsdTile - 1) / kMsdTile) << 12) | 1u : 0u);
-      s_misc[0] = tot ? atomicAdd(&a.lvl[0].entries, tot & 4095u) : 0u;
-      s_misc[1] = tot ? atomicAdd(&a.lvl[0].tiles, tot >> 12) : 0u;
+      const uint32_t tot =
+          s_emit[lb] +
+          (lc > C && emit_large ? (((lc + kMsdTile - 1) / kMsdTile) << 12) | 1u : 0u);
+      s_misc[0] = tot ? atomicAdd(&a.lvl[kt].entries, tot & 4095u) : 0u;
+      s_misc[1] = tot ? atomicAdd(&a.lvl[kt].tiles, tot >> 12) : 0u;
       a.went[w].flags = E.flags | kMsdMove;
     }
     any_small = __syncthreads_or(any_small);
-    // buckets of more than C elements: ranges of level 0
+    const uint32_t NL = kt & 1u;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const uint32_t b = t * PER + i;
-      if (c[i] <= C) continue;
+      if (c[i] <= C || !emit_large) continue;
       const uint32_t nt = (c[i] + kMsdTile - 1) / kMsdTile;
       const uint32_t id = s_misc[0] + (s_emit[b] & 4095u);
       const uint32_t tb = s_misc[1] + (s_emit[b] >> 12);
@@ -864,20 +909,20 @@ __global__ __launch_bounds__(256) void k_msd_wplan(MsdArgs a) {
         atomicAdd(&a.run_ctr[2], 1u);  // cannot happen within the workspace's capacities
         if (id < a.cap_entries) {
           MsdEntry dead = {ca, cb, 0u, 0xFFFFFFFFu, 0u, 0u};
-          a.ent[0][id] = dead;
+          a.ent[NL][id] = dead;
         }
         continue;
       }
       MsdEntry ch = {ca, cb, c[i], tb, cd >= 0 ? (uint32_t)cd : 0u,
-                     kMsdInB | (cd >= 0 ? 0u : kMsdCopy)};
-      a.ent[0][id] = ch;
-      uint4* z = reinterpret_cast<uint4*>(a.counts[0] + (size_t)id * kNB);
+                     (out_b ? kMsdInB : 0u) | (cd >= 0 ? 0u : kMsdCopy)};
+      a.ent[NL][id] = ch;
+      uint4* z = reinterpret_cast<uint4*>(a.counts[NL] + (size_t)id * kNB);
       for (int j = 0; j < kNB / 4; ++j) z[j] = make_uint4(0u, 0u, 0u, 0u);
-      reinterpret_cast<uint4*>(a.orr[0])[id] = make_uint4(0u, 0u, ~0u, ~0u);
-      for (uint32_t j = 0; j < nt; ++j) a.tilemap[0][tb + j] = id;
+      reinterpret_cast<uint4*>(a.orr[NL])[id] = make_uint4(0u, 0u, ~0u, ~0u);
+      for (uint32_t j = 0; j < nt; ++j) a.tilemap[NL][tb + j] = id;
     }
-    if (!any_small) {
-      __syncthreads();
+    if (!any_small || (cd < 0 && !out_b)) {
+      __syncthreads();  // the shared arrays are rewritten for the next range
       continue;
     }
     // Runs of consecutive small buckets, as in k_msd_plan, that also stay
@@ -949,7 +994,7 @@ __global__ __launch_bounds__(256) void k_msd_wplan(MsdArgs a) {
         digits = one ? lower : (lower | (1u << p0));
         passes = wide_finish_passes(digits, fshift, s_rn[r] / s_rb[r]);
       }
-      MsdRun run = {E.b + 16ull * s_rlo[r], E.a + 16ull * s_rlo[r], s_rn[r],
+      MsdRun run = {(out_b ? E.b : E.a) + 16ull * s_rlo[r], E.a + 16ull * s_rlo[r], s_rn[r],
                     digits | (passes << 16)};
       a.runs[rbase + r] = run;
     }
@@ -964,10 +1009,10 @@ __global__ __launch_bounds__(BLOCK) void k_msd_wscatter(MsdArgs a) {
   // per-wave counter rows of 2^kMsdWideBits buckets do not fit beside the
   // tile: they share the staging buffer
   __shared__ TileSmem<kMsdWideBits, BLOCK, ITEMS> sm;
-  const uint32_t tiles = umin(a.wlvl->tiles, a.cap_wide_tiles);
-  const uint32_t nw = umin(a.wlvl->entries, a.cap_wide);
-  uint32_t* ticket = &a.wlvl->ticket;
-  const uint32_t fshift = a.key_bits - kMsdWideBits;
+  MsdLevel* wl = &a.wlvl[a.wstep];
+  const uint32_t tiles = umin(wl->tiles, a.cap_wide_tiles);
+  const uint32_t nw = umin(wl->entries, a.cap_wide);
+  uint32_t* ticket = &wl->ticket;
   if (threadIdx.x == 0) sm.scalar[3] = atomicAdd(ticket, 1u);
   __syncthreads();
   while (true) {
@@ -979,9 +1024,11 @@ __global__ __launch_bounds__(BLOCK) void k_msd_wscatter(MsdArgs a) {
       const MsdEntry E = a.went[e];
       const uint32_t local = tl - E.tile_begin;
       if ((E.flags & kMsdMove) && tl >= E.tile_begin && (uint64_t)local * kMsdTile < E.n) {
+        const bool in_b = (E.flags & kMsdInB) != 0;
         onesweep_tile<kMsdWideBits, BLOCK, ITEMS>(
-            sm, reinterpret_cast<const elem_t*>(E.a), reinterpret_cast<elem_t*>(E.b), E.n, local,
-            a.wstatus + (size_t)E.tile_begin * WNB, nullptr, a.wcounts + (size_t)e * WNB, fshift,
+            sm, reinterpret_cast<const elem_t*>(in_b ? E.b : E.a),
+            reinterpret_cast<elem_t*>(in_b ? E.a : E.b), E.n, local,
+            a.wstatus + (size_t)E.tile_begin * WNB, nullptr, a.wcounts + (size_t)e * WNB, E.dig,
             (uint32_t)WNB - 1u, a.ko, ticket, nullptr);
         took = true;
       }

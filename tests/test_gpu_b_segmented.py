@@ -538,3 +538,32 @@ def test_automatic_form_for_narrow_keys(key_bits):
         for s in range(len(sizes)):
             lo, hi = int(off[s]), int(off[s + 1])
             assert np.array_equal(got[lo:hi], want[lo:hi]), (key_bits, lsd, s)
+
+
+def test_msd_wide_step_after_level_zero():
+    """A segment whose top byte takes three values: level 0 leaves three
+    buckets of about 3 M elements, which go through the second wide step
+    (10-bit digit just below the top byte) before they finish; beside it a
+    segment (9.5 M) with one such bucket of 7.6 M among small ones, in both
+    key orders."""
+    rng = np.random.default_rng(5)
+    a = O.gen_uniform(9_000_000, ranks=4, seed_base=41)
+    top = np.array([0x10, 0x20, 0x30], dtype=np.uint64)[rng.integers(0, 3, len(a))]
+    a["key"] = (a["key"] & np.uint64((1 << 56) - 1)) | (top << np.uint64(56))
+    b = O.gen_uniform(9_500_000, ranks=2, seed_base=42)
+    big = rng.random(len(b)) < 0.8
+    b["key"][big] = (b["key"][big] & np.uint64((1 << 56) - 1)) | np.uint64(0x77 << 56)
+    segs = [a, b]
+    sizes = [len(x) for x in segs]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    for order in ("unsigned", "signed"):
+        wants = [O.stable_sort_order(x, 64, order) for x in segs]
+        A = to_dev(np.concatenate(segs))
+        B = torch.empty_like(A)
+        lsb_amd.sort_segmented(A, B, off, key_order=order, sync=True, lsd=False, graph=True)
+        got = from_dev(A)
+        for s in range(len(segs)):
+            lo, hi = int(off[s]), int(off[s + 1])
+            assert np.array_equal(got[lo:hi], wants[s]), (order, s)
+        del A, B
+    torch.cuda.empty_cache()
