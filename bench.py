@@ -4,16 +4,21 @@ Workload: 2^30 16-byte {uint64 key uniform over [0, 2^64), uint64 val =
 index} pairs (16 GiB in A + 16 GiB ping-pong B), keys from pcg64 over 256
 virtual ranks exactly like the reference keygen (mpi/mpi_lsbsort.cpp:
 676-704), sorted by the full 64-bit key, stable.  A step is one lsb_sort of
-that array: the input is restored from a pristine device copy before every
-step (untimed), then the sort is timed between two device synchronisations.
-Metric: Melem/s sorted = n / t_sort / 1e6 (mpi/mpi_lsbsort.cpp:741).
+that array in the library's automatic form (most significant digit first at
+this size and key width; --form picks one): the input is restored from a
+pristine device copy before every step (untimed), then the sort is timed
+between two device synchronisations.  Metric: Melem/s sorted = n / t_sort /
+1e6 (mpi/mpi_lsbsort.cpp:741).
 
-roofline: the dominant kernel is the onesweep scatter pass.  Its
-algorithmic bytes are 32 B per element per executed pass (16 B read + 16 B
-written); its duration is read from HIP events recorded on the sort's stream
-between launches (lsb_profile_read), averaged over the timed steps.  The
-whole sort is also reported against the model n * (8 + 32 * P_exec) bytes
-(one 8-byte key read for the all-digit histogram, 32 B per executed pass).
+roofline: the dominant kernel of the timed form, the launch with the largest
+average duration, read from HIP events recorded on the sort's stream between
+its launches (lsb_msd_profile_read / lsb_profile_read), averaged over the
+timed steps, against its algorithmic bytes: 32 B per element it moves (16 B
+read + 16 B written) for a scatter, a partition or the in-LDS finish, 8 B per
+element for a histogram (the key).  The whole sort is also reported against
+the sum of its launches' algorithmic bytes.  The LSD form (the reference's
+pass order) is timed on the same input beside it: its scatter pass's
+roofline and its throughput are under lsd_form.
 
 cpu_baseline: the reference's own mpi_lsbsort (oracle/_ref, built from its
 sources) under mpirun on this host's cores at a bounded sample size, before
@@ -136,47 +141,114 @@ def cpu_baseline(reps=5, full=False):
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_onesweep_rg_2p30_uniform_traffic.md")
+MSD_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_msd_2p30_uniform_traffic.md")
 
 
-# The sources the plain sort's kernels are compiled from, and the build file.
+# The sources each form's kernels are compiled from, and the build file.
 PLAIN_SORT_SOURCES = ["csrc/lsb_kernels.hip", "csrc/lsb_onesweep.hpp", "csrc/lsb_device.hpp",
                       "csrc/lsb_internal.hpp", "Makefile"]
+MSD_SOURCES = ["csrc/lsb_msd.hip", "csrc/lsb_onesweep.hpp", "csrc/lsb_device.hpp",
+               "csrc/lsb_internal.hpp", "Makefile"]
 
 
-def source_digest():
-    """SHA-256 over the sources of the plain sort's kernels and the build
-    file: a committed counter summary is used only for the code it was taken
-    on."""
+def source_digest(sources):
+    """SHA-256 over kernel sources and the build file: a committed counter
+    summary is used only for the code it was taken on."""
     import hashlib
     h = hashlib.sha256()
     pkg = os.path.join(ROOT, "distributed-lsb_amd")
-    for rel in PLAIN_SORT_SOURCES:
+    for rel in sources:
         h.update(rel.encode())
         h.update(open(os.path.join(pkg, rel), "rb").read())
     return h.hexdigest()[:16]
 
 
-def traffic_from_profile(log2n, d, tile):
-    """HBM bytes per scatter launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary, when it was taken on this workload, this layout and
+def traffic_from_profile(path, log2n, sources, kernel, config=None):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary, when it was taken on this workload and
     these exact sources; else None."""
     try:
-        text = open(TRAFFIC_PROFILE).read()
+        text = open(path).read()
     except OSError:
         return None
-    m = re.search(r"^traffic_bytes_per_launch: (\d+)$", text, re.M)
-    c = re.search(r"^traffic_config: log2n=(\d+) digit_bits=(\d+) tile=(\d+)$", text, re.M)
+    m = re.search(r"^traffic_bytes_per_launch\[" + re.escape(kernel) + r"\]: (\d+)$", text, re.M)
+    c = re.search(r"^traffic_config: (.*)$", text, re.M)
     h = re.search(r"^traffic_sources: ([0-9a-f]+)$", text, re.M)
-    if not m or not c or tuple(map(int, c.groups())) != (log2n, d, tile):
+    want = config or f"log2n={log2n}"
+    if not m or not c or c.group(1).strip() != want:
         return None
-    if not h or h.group(1) != source_digest():
-        log("traffic profile was taken on other sources: traffic reported as null")
+    if not h or h.group(1) != source_digest(sources):
+        log(f"{os.path.basename(path)} was taken on other sources: traffic reported as null")
         return None
     return int(m.group(1))
 
 
+def lsd_summary(n, wall, profs, d, tile, P, log2n):
+    """The LSD form's figures from its profiled steps: its dominant kernel is
+    the onesweep scatter pass, 32 B per element per executed pass (16 B read
+    + 16 B written); the whole sort moves n * (8 + 32 * passes executed)."""
+    ms_wall = sum(wall) / len(wall) * 1e3
+    p_exec = sum(profs[-1]["executed"])
+    pass_ms = [t for p in profs for t, e in zip(p["pass_ms"], p["executed"]) if e]
+    avg_pass_ms = sum(pass_ms) / len(pass_ms) if pass_ms else float("nan")
+    hist_ms = sum(p["histogram_ms"] for p in profs) / len(profs)
+    achieved = 32 * n / (avg_pass_ms * 1e-3) / 1e9
+    model_bytes = n * (8 + 32 * p_exec)
+    sort_gbs = model_bytes / (ms_wall * 1e-3) / 1e9
+    traffic = traffic_from_profile(TRAFFIC_PROFILE, log2n, PLAIN_SORT_SOURCES, "k_onesweep_rg",
+                                   f"log2n={log2n} digit_bits={d} tile={tile}")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": f"k_onesweep_rg scatter pass (d={d}, tile={tile}), 32 B/elem x 2^{log2n}",
+                "avg_launch_ms": round(avg_pass_ms, 4)}
+    block = {"melem_s": round(n / (ms_wall * 1e-3) / 1e6, 2), "ms_per_step": round(ms_wall, 4),
+             "steps": len(wall), "digit_bits": d, "tile": tile, "passes": P,
+             "passes_executed": p_exec, "histogram_ms": round(hist_ms, 4), "roofline": roofline,
+             "sort_roofline": {"model_bytes": model_bytes, "achieved": round(sort_gbs, 1),
+                               "frac": round(sort_gbs / HBM_PEAK_GBS, 4)},
+             "what": "lsb_sort forced to the LSD form (the reference's pass order), same input"}
+    return {"avg_pass_ms": avg_pass_ms, "model_bytes": model_bytes, "roofline": roofline,
+            "passes_executed": p_exec, "block": block}
+
+
+MSD_KERNELS = {"w": ("k_msd_whist", "k_msd_wplan", "k_msd_wscatter"),
+               "l": ("k_msd_hist", "k_msd_plan", "k_msd_scatter")}
+
+
+def msd_launches(profs):
+    """Every launch of the MSD form that moved elements, averaged over the
+    profiled steps, with its algorithmic bytes: a scatter or the finish 32 B
+    per element it moved (16 B read + 16 B written), a histogram 8 B per
+    element of its step (the key read; on uniform keys every range it counts
+    is partitioned), a plan none."""
+    K, S = profs[-1]["levels"], profs[-1]["wide_steps"]
+    steps = [("wide step 0", "w", 0)]
+    for k in range(K):
+        steps.append((f"level {k}", "l", k))
+        if k + 1 < S:
+            steps.append((f"wide step {k + 1}", "w", k + 1))
+    mean = lambda xs: sum(xs) / len(xs)
+    out = []
+    for label, kind, i in steps:
+        elems = profs[-1]["wide_elems" if kind == "w" else "level_elems"][i]
+        if not elems:
+            continue
+        for j, bpe in enumerate((8, 0, 32)):
+            ms = mean([p["wide_ms" if kind == "w" else "level_ms"][i][j] for p in profs])
+            b = bpe * elems
+            out.append({"kernel": MSD_KERNELS[kind][j], "step": label, "ms": round(ms, 4),
+                        "elems": elems, "bytes_per_elem": bpe, "bytes": b,
+                        "gbs": round(b / (ms * 1e-3) / 1e9, 1) if b and ms > 0 else None})
+    ms = mean([p["finish_ms"] for p in profs])
+    fe = profs[-1]["finish_elems"]
+    out.append({"kernel": "k_msd_finish", "step": "finish", "ms": round(ms, 4), "elems": fe,
+                "bytes_per_elem": 32, "bytes": 32 * fe,
+                "gbs": round(32 * fe / (ms * 1e-3) / 1e9, 1) if ms > 0 else None})
+    return out
+
+
 def _counter_rows(csv_dir):
-    """(kernel name, dispatch id, counter value) rows of every rocprofv3
+    """(kernel name, dispatch id) -> counter value of every rocprofv3
     counter_collection.csv under csv_dir, summed per dispatch."""
     import csv
     per = {}
@@ -191,49 +263,45 @@ def _counter_rows(csv_dir):
 
 
 def traffic_summary(fetch_dir, write_dir, log2n, d, tile, cmd_fetch, cmd_write):
-    """Writes the traffic profile from two rocprofv3 --pmc runs of
+    """Writes both traffic profiles from two rocprofv3 --pmc runs of
     `bench.py --traffic-probe` (FETCH_SIZE, then WRITE_SIZE, KiB per
-    dispatch).  Scale factors come from the probe's plain device copy of the
+    dispatch), which sorts the array once in the MSD form and twice in the
+    LSD form.  Scale factors come from the probe's plain device copy of the
     same 2^log2n-element array (16 B read + 16 B written per element)."""
     fetch, write = _counter_rows(fetch_dir), _counter_rows(write_dir)
     n = 1 << log2n
     kib = 16 * n / 1024.0
 
     def pick(per, pat):
-        return [v for (k, _), v in sorted(per.items(), key=lambda kv: kv[0][1]) if pat in k]
+        return [(i, v) for (k, i), v in sorted(per.items(), key=lambda kv: kv[0][1]) if pat in k]
 
     def copy_of(per):
-        vals = [v for v in pick(per, "copyBuffer") if v > 0.5 * kib]
+        vals = [v for _, v in pick(per, "copyBuffer") if v > 0.5 * kib]
         return vals[-1] if vals else None
 
     cf, cw = copy_of(fetch), copy_of(write)
     sf, sw = kib / cf, kib / cw
-    pf, pw = pick(fetch, "k_onesweep_rg"), pick(write, "k_onesweep_rg")
-    hf = pick(fetch, "k_histogram")
+    head = ["Commands (two passes, one counter each; repo root = /root/repo):", "",
+            "    " + cmd_fetch, "    " + cmd_write, "",
+            "Summary written by `python3 bench.py --traffic-summary <fetch dir> <write dir>`.", "",
+            f"Calibration on the plain copy (`copyBuffer`, 2^{log2n} x 16 B = {kib:.0f} KiB read "
+            f"and written; raw FETCH_SIZE {cf:.0f} KiB, WRITE_SIZE {cw:.0f} KiB): FETCH_SIZE scale "
+            f"{sf:.3f}, WRITE_SIZE scale {sw:.3f}.", ""]
+
+    # LSD form: the onesweep scatter pass, mean over its dispatches
+    pf, pw = [v for _, v in pick(fetch, "k_onesweep_rg")], [v for _, v in pick(write, "k_onesweep_rg")]
+    hf = [v for _, v in pick(fetch, "k_histogram")]
     rd = sf * sum(pf) / len(pf) * 1024
     wr = sw * sum(pw) / len(pw) * 1024
     alg = 16 * n
-    lines = [
-        f"# HBM traffic per launch, 2^{log2n} uniform 16-byte pairs, d = {d}, tile {tile}",
-        "",
-        "Commands (two passes, one counter each; repo root = /root/repo):",
-        "",
-        "    " + cmd_fetch,
-        "    " + cmd_write,
-        "",
-        "Summary written by `python3 bench.py --traffic-summary <fetch dir> <write dir>`.",
-        "",
-        "Raw counters, KiB per dispatch:",
-        "",
+    lines = [f"# HBM traffic per launch, LSD form, 2^{log2n} uniform 16-byte pairs, d = {d}, "
+             f"tile {tile}", ""] + head + [
+        "Raw counters, KiB per dispatch:", "",
         "| kernel | dispatches | FETCH_SIZE | WRITE_SIZE |",
         "|---|---|---|---|",
-        f"| plain copy (`copyBuffer`, 2^{log2n} x 16 B) | 1 | {cf:.0f} | {cw:.0f} |",
         f"| `k_histogram` | {len(hf)} | {' '.join(f'{v:.0f}' for v in hf)} | - |",
         f"| `k_onesweep_rg` | {len(pf)} | {' '.join(f'{v:.0f}' for v in pf)} | "
         f"{' '.join(f'{v:.0f}' for v in pw)} |",
-        "",
-        f"Calibration on the plain copy ({kib:.0f} KiB read and written): FETCH_SIZE scale "
-        f"{sf:.3f}, WRITE_SIZE scale {sw:.3f}.",
         "",
         "Scatter pass, mean over its dispatches, scaled:",
         "",
@@ -242,12 +310,43 @@ def traffic_summary(fetch_dir, write_dir, log2n, d, tile, cmd_fetch, cmd_write):
         f"- traffic per launch = {(rd + wr) / 1e9:.3f} GB = {(rd + wr) / n:.2f} B per element "
         "(algorithmic 32 B)",
         "",
-        f"traffic_bytes_per_launch: {int(round(rd + wr))}",
+        f"traffic_bytes_per_launch[k_onesweep_rg]: {int(round(rd + wr))}",
         f"traffic_config: log2n={log2n} digit_bits={d} tile={tile}",
-        f"traffic_sources: {source_digest()}",
+        f"traffic_sources: {source_digest(PLAIN_SORT_SOURCES)}",
         "",
     ]
     open(TRAFFIC_PROFILE, "w").write("\n".join(lines))
+    print("\n".join(lines))
+
+    # MSD form: per kernel, its largest dispatch (the one step that moves the
+    # array: level 0's scatter, wide step 1's, the finish)
+    rows, keys = [], []
+    for k, alg_r, alg_w in (("k_msd_hist", 8, 0), ("k_msd_scatter", 16, 16),
+                            ("k_msd_whist", 8, 0), ("k_msd_wscatter", 16, 16),
+                            ("k_msd_finish", 16, 16)):
+        fd = pick(fetch, k)
+        wd = dict(pick(write, k))
+        if not fd:
+            continue
+        i, v = max(fd, key=lambda x: x[1])
+        rb = sf * v * 1024
+        wb = sw * wd.get(i, 0.0) * 1024
+        ar, aw = alg_r * n, alg_w * n
+        rows.append(f"| `{k}` | {len(fd)} | {' '.join(f'{x:.0f}' for _, x in fd)} | "
+                    f"{rb / 1e9:.3f} ({100 * (rb / ar - 1):+.1f} %) | "
+                    + (f"{wb / 1e9:.3f} ({100 * (wb / aw - 1):+.1f} %)" if aw else f"{wb / 1e9:.3f}")
+                    + f" | {(rb + wb) / n:.2f} ({alg_r + alg_w}) |")
+        keys.append(f"traffic_bytes_per_launch[{k}]: {int(round(rb + wb))}")
+    lines = [f"# HBM traffic per launch, MSD form, 2^{log2n} uniform 16-byte pairs", ""] + head + [
+        "Per kernel, its largest dispatch in one sort (FETCH_SIZE KiB of every dispatch listed), "
+        "scaled; GB and the excess over the algorithmic bytes:", "",
+        "| kernel | dispatches | FETCH_SIZE per dispatch | read GB | write GB | B/elem (alg.) |",
+        "|---|---|---|---|---|---|"] + rows + [""] + keys + [
+        f"traffic_config: log2n={log2n}",
+        f"traffic_sources: {source_digest(MSD_SOURCES)}",
+        "",
+    ]
+    open(MSD_TRAFFIC_PROFILE, "w").write("\n".join(lines))
     print("\n".join(lines))
 
 
@@ -327,6 +426,8 @@ def main():
     ap.add_argument("--log2n", type=int, default=30)
     ap.add_argument("--digit-bits", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--form", choices=("auto", "lsd", "msd"), default="auto",
+                    help="sort form of the timed steps (auto: the library's choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--cpu-full", action="store_true",
@@ -389,36 +490,37 @@ def main():
     lsb_amd.generate(pristine, kind=lsb_amd.GEN_UNIFORM, ranks=256)
     A = torch.empty_like(pristine)
     B = torch.empty_like(pristine)
+    lsd = {"auto": None, "lsd": True, "msd": False}[args.form]
+    form = lsb_amd.sort_form(n, digit_bits=args.digit_bits, tile=args.tile, lsd=lsd)
     d, tile, P = lsb_amd.plan(n, digit_bits=args.digit_bits, tile=args.tile)
     torch.cuda.synchronize()
 
-    def step(profile):
-        A.copy_(pristine)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        lsb_amd.sort(A, B, digit_bits=args.digit_bits, tile=args.tile, profile=profile)
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
+    def run_form(lsd_flag, steps, warmup):
+        """Times `steps` sorts (after `warmup` untimed ones) in one form, each
+        profiled (HIP events between its launches), input restored untimed."""
+        read = lsb_amd.profile_read if lsd_flag else lsb_amd.msd_profile_read
 
-    for i in range(args.warmup):
-        step(False)
-    log(f"warmup done ({args.warmup}); timing {args.steps} steps ...")
-    wall, profs = [], []
-    for i in range(args.steps):
-        wall.append(step(True))
-        profs.append(lsb_amd.profile_read())
+        def step(profile):
+            A.copy_(pristine)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            lsb_amd.sort(A, B, digit_bits=args.digit_bits, tile=args.tile, profile=profile,
+                         lsd=lsd_flag)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0
+
+        for i in range(warmup):
+            step(False)
+        wall, profs = [], []
+        for i in range(steps):
+            wall.append(step(True))
+            profs.append(read())
+        return wall, profs
+
+    log(f"form {form}: warmup {args.warmup}, timing {args.steps} steps ...")
+    wall, profs = run_form(form == "lsd", args.steps, args.warmup)
     ms_wall = sum(wall) / len(wall) * 1e3
     ms_event = sum(p["total_ms"] for p in profs) / len(profs)
-
-    executed = profs[-1]["executed"]
-    p_exec = sum(executed)
-    pass_ms = [t for p in profs for t, e in zip(p["pass_ms"], p["executed"]) if e]
-    avg_pass_ms = sum(pass_ms) / len(pass_ms) if pass_ms else float("nan")
-    hist_ms = sum(p["histogram_ms"] for p in profs) / len(profs)
-    pass_bytes = 32 * n
-    achieved = pass_bytes / (avg_pass_ms * 1e-3) / 1e9
-    model_bytes = n * (8 + 32 * p_exec)
-    sort_gbs = model_bytes / (ms_wall * 1e-3) / 1e9
 
     verified = None
     if not args.no_verify:
@@ -429,6 +531,15 @@ def main():
         del keys
         if not verified:
             log(f"VERIFY FAILED: {bad} bad positions, first {first}")
+
+    # The LSD form (the reference's pass order) on the same input: its
+    # scatter pass is the kernel the LSD roofline is quoted on.
+    if form == "lsd":
+        lsd_wall, lsd_profs = wall, profs
+    else:
+        log("LSD form for comparison ...")
+        lsd_wall, lsd_profs = run_form(True, args.steps, 1)
+    lsd = lsd_summary(n, lsd_wall, lsd_profs, d, tile, P, args.log2n)
 
     # Context for the roofline: the HIP runtime's own device-to-device copy of
     # the same array (16 B read + 16 B written per element, as one scatter
@@ -445,14 +556,12 @@ def main():
     copy_avg = sum(copy_ms) / len(copy_ms)
     copy_gbs = 32 * n / (copy_avg * 1e-3) / 1e9
 
-    traffic = traffic_from_profile(args.log2n, d, tile)
-
     # The pass's access pattern without its sorting (lsb_debug_scatter_floor):
     # 8192-element tiles read in order, written as R runs into R regions
     # (R = 256: a uniform pass's layout; R = 1: a plain copy).  Mean of 3
     # after one warm-up, HIP events on the stream the probe runs on.
     floor = None
-    if n % 8192 == 0:
+    if n % 8192 == 0 and not args.traffic_probe:
         floor = {}
         for runs in (1, 16, 64, 256):
             ms = []
@@ -466,9 +575,34 @@ def main():
             m = sum(ms) / len(ms)
             floor[f"runs_{runs}"] = {"ms": round(m, 4),
                                      "gbs": round(32 * n / (m * 1e-3) / 1e9, 1)}
-        floor["pass_over_floor_256"] = round(avg_pass_ms / floor["runs_256"]["ms"], 4)
+        floor["lsd_pass_over_floor_256"] = round(lsd["avg_pass_ms"] / floor["runs_256"]["ms"], 4)
         floor["what"] = ("scatter pass memory traffic alone: 8192-element tiles read in order, "
                          "written as R runs of consecutive elements into R regions")
+
+    if form == "msd":
+        launches = msd_launches(profs)
+        dom = max(launches, key=lambda k: k["ms"])
+        model_bytes = sum(k["bytes"] for k in launches)
+        traffic = traffic_from_profile(MSD_TRAFFIC_PROFILE, args.log2n, MSD_SOURCES,
+                                       dom["kernel"])
+        roofline = {"bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "kernel": f"{dom['kernel']} ({dom['step']}), {dom['bytes_per_elem']} B/elem x "
+                              f"{dom['elems']} elements",
+                    "avg_launch_ms": dom["ms"]}
+        workload = (f"plain sort (MSD-first form), 2^{args.log2n} 16-byte pairs, uniform 64-bit "
+                    "keys, val = index, stable, HBM-resident")
+        cfg_extra = {"form": "msd", "levels": profs[-1]["levels"],
+                     "msd_launches": [{k: v for k, v in x.items() if k != "bytes"}
+                                      for x in launches]}
+    else:
+        model_bytes = lsd["model_bytes"]
+        roofline = dict(lsd["roofline"])
+        workload = (f"plain LSD sort, 2^{args.log2n} 16-byte pairs, uniform 64-bit keys, "
+                    "val = index, stable, HBM-resident")
+        cfg_extra = {"form": "lsd", "digit_bits": d, "tile": tile, "passes": P,
+                     "passes_executed": lsd["passes_executed"]}
+    sort_gbs = model_bytes / (ms_wall * 1e-3) / 1e9
 
     value = n / (ms_wall * 1e-3) / 1e6
     out = {
@@ -484,18 +618,14 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic (pcg64 keys over 256 virtual ranks, reference keygen)",
-        "config": {"workload": f"plain LSD sort, 2^{args.log2n} 16-byte pairs, uniform 64-bit "
-                               "keys, val = index, stable, HBM-resident",
-                   "n": n, "digit_bits": d, "tile": tile, "passes": P,
-                   "passes_executed": p_exec},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic,
-                     "kernel": f"onesweep scatter pass (d={d}, tile={tile}), 32 B/elem x 2^{args.log2n}",
-                     "avg_launch_ms": round(avg_pass_ms, 4)},
+        "config": dict({"workload": workload, "n": n}, **cfg_extra),
+        "roofline": roofline,
         "sort_roofline": {"model_bytes": model_bytes, "achieved": round(sort_gbs, 1),
                           "frac": round(sort_gbs / HBM_PEAK_GBS, 4),
-                          "ms_event": round(ms_event, 4), "histogram_ms": round(hist_ms, 4)},
+                          "ms_event": round(ms_event, 4),
+                          "what": "algorithmic bytes of every launch of the sort over its "
+                                  "wall time"},
+        "lsd_form": lsd["block"],
         "scatter_floor": floor,
         "copy_ceiling": {"achieved": round(copy_gbs, 1), "unit": "GB/s", "ms": round(copy_avg, 4),
                          "frac": round(copy_gbs / HBM_PEAK_GBS, 4),

@@ -365,6 +365,44 @@ struct ProfileState {
 };
 thread_local ProfileState g_prof;
 
+// Events of the last LSB_PROFILE call on this thread that ran the MSD-first
+// form, in launch order: [0] before the set-up, [1] after it, then one after
+// each histogram, plan and scatter launch of wide step 0, level 0, wide
+// step 1 and levels 1 .. levels-1, and one after the finish.
+struct MsdProfileState {
+  std::vector<hipEvent_t> ev;
+  uint32_t levels = 0;
+  const lsb::MsdLevel* lvl = nullptr;   // the call's level counters (workspace)
+  const lsb::MsdLevel* wlvl = nullptr;
+  const lsb::MsdRun* runs = nullptr;
+  const uint32_t* run_ctr = nullptr;
+  uint32_t cap_runs = 0;
+  bool valid = false;
+};
+thread_local MsdProfileState g_mprof;
+
+// Wide steps launched with `levels` levels: step 0, and step s after level
+// s-1 while a level s exists to take its large buckets.
+uint32_t msd_wide_steps(uint32_t levels) {
+  return std::min<uint32_t>((uint32_t)lsb::kMsdWideSteps, levels);
+}
+// Events of a profiled MSD sort: start, set-up, three per wide step and
+// level, finish.
+size_t msd_prof_event_count(uint32_t levels) {
+  return 2 + 3 * (size_t)(levels + msd_wide_steps(levels)) + 1;
+}
+
+// g_mprof.ev with at least `count` events.
+int msd_prof_events(size_t count, hipEvent_t** out) {
+  while (g_mprof.ev.size() < count) {
+    hipEvent_t e;
+    LSB_HIP(hipEventCreate(&e));
+    g_mprof.ev.push_back(e);
+  }
+  *out = g_mprof.ev.data();
+  return LSB_OK;
+}
+
 int prof_mark(bool on, size_t i, hipStream_t s) {
   if (!on) return LSB_OK;
   while (g_prof.ev.size() <= i) {
@@ -389,7 +427,7 @@ int enqueue_sort(lsb_elem* A, lsb_elem* B, uint32_t n, const Plan& p, char* ws,
   const size_t slice = (size_t)p.ntiles * p.nb;
 
   if (prof) {
-    g_prof.valid = false;
+    g_prof.valid = g_mprof.valid = false;
     if (p.passes > 16) return LSB_EINVAL;
   }
   // Region passes count each later digit for their successor, so their
@@ -512,17 +550,29 @@ struct SegLayout {
 };
 
 // The MSD-first form sorts the large segments unless the caller asks for the
-// LSD passes (LSB_SEG_LSD), names a digit width other than 8 or a tile, or
-// sets LSB_NO_SKIP (run every digit pass).  Unless asked for (LSB_SEG_MSD)
+// LSD passes (LSB_FORM_LSD), names a digit width other than 8 or a tile, or
+// sets LSB_NO_SKIP (run every digit pass).  Unless asked for (LSB_FORM_MSD)
 // it is used for keys wider than 32 bits only: at 32 bits and below the LSD
 // form's 4 passes move as many bytes as MSD's partitions and finish, run as
 // fast on uniform keys and skip the constant digits of skewed ones (Zipf
 // keys: 34 against 25 Gelem/s over a serving stream's jobs).
 bool wants_msd(const lsb_config& c) {
-  if ((c.flags & (LSB_SEG_LSD | LSB_NO_SKIP)) || !(c.digit_bits == 0 || c.digit_bits == 8) ||
+  if ((c.flags & (LSB_FORM_LSD | LSB_NO_SKIP)) || !(c.digit_bits == 0 || c.digit_bits == 8) ||
       c.tile != 0)
     return false;
-  return (c.flags & LSB_SEG_MSD) || c.key_bits == 0 || c.key_bits > 32;
+  return (c.flags & LSB_FORM_MSD) || c.key_bits == 0 || c.key_bits > 32;
+}
+
+// lsb_sort takes the MSD-first form (one segment of the batched sort) where
+// wants_msd would and, unless asked for, only with the digit width left
+// automatic and from LSB_MSD_MIN_N elements on.  There two 8-bit or 10-bit
+// partitions and the in-LDS finish move about 112 B per element against the
+// LSD form's 264 B (eight passes of 32 B and the 8 B histogram); below
+// 2^24 elements the LSD passes run fast enough that the partitions' extra
+// launches and finish do not pay.
+bool plain_msd(size_t n, const lsb_config& c) {
+  if (c.variant != 0 || !wants_msd(c)) return false;
+  return (c.flags & LSB_FORM_MSD) || (c.digit_bits == 0 && n >= LSB_MSD_MIN_N);
 }
 
 // Capacities of the MSD-first form for batches of at most cap_n elements.
@@ -532,7 +582,8 @@ bool wants_msd(const lsb_config& c) {
 // and at most cap_n / kMsdTile + that many tiles.  Runs: along one range's
 // buckets, two consecutive runs not separated by a large bucket hold more
 // than kMsdFinishCap elements together, so runs <= 2 cap_n / kMsdFinishCap
-// + (number of partitioned ranges + large buckets) over all levels.
+// + (number of partitioned ranges + large buckets) over all levels and wide
+// steps (a wide range has at most 2^kMsdWideBits large buckets).
 int make_msd_layout(size_t cap_n, SegLayout* L) {
   // Level 0 also holds one dead entry (and its tiles) per wide segment
   // besides the wide segments' large buckets: twice the bound.
@@ -541,7 +592,7 @@ int make_msd_layout(size_t cap_n, SegLayout* L) {
   const uint64_t wide = cap_n / (lsb::kMsdWideMin + 1) + 1;
   const uint64_t wtiles = cap_n / lsb::kMsdTile + wide;
   const uint64_t runs = 2 * (cap_n / lsb::kMsdFinishCap) + 2 * lsb::kMsdMaxLevels * ents +
-                        ((1u << lsb::kMsdWideBits) + 2) * wide + 64;
+                        lsb::kMsdWideSteps * ((1u << lsb::kMsdWideBits) + 2) * wide + 64;
   if (tiles > 0x7FFFFFFFull || runs > 0xFFFFFFFFull) return LSB_ERANGE;
   L->cap_entries = (uint32_t)ents;
   L->max_tiles = (uint32_t)tiles;
@@ -565,7 +616,7 @@ int make_msd_layout(size_t cap_n, SegLayout* L) {
   L->worr = off;    off += align_up(wide * 16);
   L->wtile = off;   off += align_up(wtiles * 4);
   L->wstatus = off; off += align_up(wtiles * wnb * 4);
-  L->wlvl = off;    off += align_up(2 * sizeof(lsb::MsdLevel));
+  L->wlvl = off;    off += align_up(lsb::kMsdWideSteps * sizeof(lsb::MsdLevel));
   L->total = off;
   return LSB_OK;
 }
@@ -686,7 +737,6 @@ int enqueue_segmented(const SegLayout& L, char* ws, uint32_t flags, hipStream_t 
   uint32_t* perm = reinterpret_cast<uint32_t*>(ws + L.perm);
   const uint32_t cus = device_cus();
   if (L.msd) {
-    LSB_HIP(lsb::launch_seg_small(hdr, small, L.key_bits, L.ko, 4 * cus, s));
     lsb::MsdArgs m;
     m.hdr = hdr;
     m.large = large;
@@ -714,20 +764,47 @@ int enqueue_segmented(const SegLayout& L, char* ws, uint32_t flags, hipStream_t 
     m.key_bits = L.key_bits;
     m.ko = L.ko;
     m.level = 0;
+    // levels 0 .. P: the last one only copies ranges left in B
+    const uint32_t levels = std::min<uint32_t>(L.passes + 1, (uint32_t)lsb::kMsdMaxLevels);
+    const uint32_t steps = msd_wide_steps(levels);
+    hipEvent_t* ev = nullptr;  // LSB_PROFILE: one event after each launch
+    if (flags & LSB_PROFILE) {
+      g_prof.valid = g_mprof.valid = false;
+      const int rc = msd_prof_events(msd_prof_event_count(levels), &ev);
+      if (rc) return rc;
+      LSB_HIP(hipEventRecord(ev[0], s));
+    }
+    LSB_HIP(lsb::launch_seg_small(hdr, small, L.key_bits, L.ko, 4 * cus, s));
     LSB_HIP(lsb::launch_msd_init(m, 4 * cus, s));
+    if (ev) LSB_HIP(hipEventRecord(ev[1], s));
+    size_t ei = 2;  // next event: three per wide step or level, in launch order
+    const auto next3 = [&]() {
+      hipEvent_t* e = ev ? ev + ei : nullptr;
+      ei += 3;
+      return e;
+    };
     m.wstep = 0;
-    LSB_HIP(lsb::launch_msd_wide(m, cus, s));
-    // levels 0 .. P: the last one only copies ranges left in B; after level
-    // 0 the wide step for its children of the wide size range
-    for (uint32_t k = 0; k <= L.passes && k < (uint32_t)lsb::kMsdMaxLevels; ++k) {
+    LSB_HIP(lsb::launch_msd_wide(m, cus, s, next3()));
+    // after level k the wide step k+1 for its children of the wide size range
+    for (uint32_t k = 0; k < levels; ++k) {
       m.level = k;
-      LSB_HIP(lsb::launch_msd_level(m, cus, s));
-      if (k == 0) {
-        m.wstep = 1;
-        LSB_HIP(lsb::launch_msd_wide(m, cus, s));
+      LSB_HIP(lsb::launch_msd_level(m, cus, s, next3()));
+      if (k + 1 < steps) {
+        m.wstep = k + 1;
+        LSB_HIP(lsb::launch_msd_wide(m, cus, s, next3()));
       }
     }
     LSB_HIP(lsb::launch_msd_finish(m, cus, s));
+    if (ev) {
+      LSB_HIP(hipEventRecord(ev[ei], s));
+      g_mprof.levels = levels;
+      g_mprof.lvl = m.lvl;
+      g_mprof.wlvl = m.wlvl;
+      g_mprof.runs = m.runs;
+      g_mprof.run_ctr = m.run_ctr;
+      g_mprof.cap_runs = m.cap_runs;
+      g_mprof.valid = true;
+    }
     return LSB_OK;
   }
   LSB_HIP(lsb::launch_seg_prepare(L.d, hdr, large, L.tile, L.key_bits, L.ko, L.passes, counts,
@@ -757,6 +834,15 @@ int enqueue_segmented(const SegLayout& L, char* ws, uint32_t flags, hipStream_t 
   LSB_HIP(lsb::launch_seg_copy(hdr, large, plan, tileseg, L.tile, 4 * cus, s));
   return LSB_OK;
 }
+
+// One segment of a batch: sorted in place in a, with b as its scratch.
+struct SegEntry {
+  lsb_elem* a;
+  lsb_elem* b;
+  uint64_t n;
+};
+
+int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_config& c);
 
 }  // namespace
 
@@ -812,7 +898,23 @@ size_t lsb_workspace_size(size_t n, const lsb_config* cfg) {
   if (n <= 1 || n > LSB_MAX_N) return 0;
   Plan p;
   if (make_plan(n, c, &p)) return 0;
+  if (plain_msd(n, c)) {
+    lsb_config m = c;
+    m.flags |= LSB_FORM_MSD;
+    SegLayout L;
+    if (make_seg_layout(n, 1, m, &L)) return 0;
+    return L.total;
+  }
   return p.total;
+}
+
+int lsb_sort_form(size_t n, const lsb_config* cfg) {
+  const lsb_config c = cfg ? *cfg : defaults();
+  if (n > LSB_MAX_N) return LSB_ERANGE;
+  Plan p;
+  const int rc = make_plan(n > 1 ? n : 2, c, &p);
+  if (rc) return rc;
+  return plain_msd(n, c) ? (int)LSB_FORM_MSD : (int)LSB_FORM_LSD;
 }
 
 int lsb_sort(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg) {
@@ -825,6 +927,13 @@ int lsb_sort(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg) {
   if (n <= 1) return LSB_OK;
   const size_t bytes = n * sizeof(lsb_elem);
   if ((rc = check_device_range(A, bytes)) || (rc = check_device_range(B, bytes))) return rc;
+  if ((c.flags & LSB_GRAPH) && (c.flags & LSB_PROFILE)) return LSB_EINVAL;
+  if (plain_msd(n, c)) {
+    // one segment of the batched sort's MSD-first form
+    lsb_config m = c;
+    m.flags |= LSB_FORM_MSD;
+    return sort_entries({SegEntry{A, B, n}}, n, m);
+  }
   hipStream_t s = (hipStream_t)c.stream;
   // Arrays that fit one workgroup are sorted in LDS by a single launch
   // (auto configuration only; an explicit digit width or tile, or
@@ -888,13 +997,6 @@ size_t lsb_segmented_workspace_size(size_t total_n, uint32_t nseg, const lsb_con
 }  // extern "C"
 
 namespace {
-
-// One segment of a batch: sorted in place in a, with b as its scratch.
-struct SegEntry {
-  lsb_elem* a;
-  lsb_elem* b;
-  uint64_t n;
-};
 
 // Sorts every entry on its own (shared body of lsb_sort_segmented and
 // lsb_sort_batch).  total_n = sum of the entries' sizes.
@@ -1379,6 +1481,51 @@ int lsb_profile_read(lsb_profile* out) {
     LSB_HIP(hipEventElapsedTime(&out->pass_ms[q], ev[2 + q], ev[3 + q]));
   LSB_HIP(hipEventElapsedTime(&out->copy_ms, ev[2 + P], ev[3 + P]));
   LSB_HIP(hipEventElapsedTime(&out->total_ms, ev[0], ev[3 + P]));
+  return LSB_OK;
+}
+
+static_assert(LSB_MSD_MAX_LEVELS == lsb::kMsdMaxLevels, "lsb_msd_profile level rows");
+static_assert(LSB_MSD_WIDE_STEPS == lsb::kMsdWideSteps, "lsb_msd_profile wide-step rows");
+
+int lsb_msd_profile_read(lsb_msd_profile* out) {
+  if (!out) return LSB_EINVAL;
+  memset(out, 0, sizeof *out);
+  if (!g_mprof.valid) return LSB_EINVAL;
+  const uint32_t K = g_mprof.levels, S = msd_wide_steps(K);
+  hipEvent_t* ev = g_mprof.ev.data();
+  const size_t last = msd_prof_event_count(K) - 1;
+  LSB_HIP(hipEventSynchronize(ev[last]));
+  lsb::MsdLevel lvl[lsb::kMsdMaxLevels], wlvl[lsb::kMsdWideSteps];
+  uint32_t ctr[4];
+  LSB_HIP(hipMemcpy(lvl, g_mprof.lvl, sizeof lvl, hipMemcpyDeviceToHost));
+  LSB_HIP(hipMemcpy(wlvl, g_mprof.wlvl, sizeof wlvl, hipMemcpyDeviceToHost));
+  LSB_HIP(hipMemcpy(ctr, g_mprof.run_ctr, sizeof ctr, hipMemcpyDeviceToHost));
+  const uint32_t nr = std::min(ctr[0], g_mprof.cap_runs);
+  std::vector<lsb::MsdRun> runs(nr);
+  if (nr)
+    LSB_HIP(hipMemcpy(runs.data(), g_mprof.runs, nr * sizeof(lsb::MsdRun),
+                      hipMemcpyDeviceToHost));
+  out->levels = K;
+  out->runs = nr;
+  for (const lsb::MsdRun& r : runs) out->finish_elems += r.n;
+  out->wide_steps = S;
+  for (uint32_t w = 0; w < S; ++w) out->wide_elems[w] = wlvl[w].moved;
+  for (uint32_t k = 0; k < K; ++k) out->level_elems[k] = lvl[k].moved;
+  const auto span = [&](size_t j, float* ms) {  // launch j: from event j-1 to event j
+    return hipEventElapsedTime(ms, ev[j - 1], ev[j]);
+  };
+  LSB_HIP(span(1, &out->setup_ms));
+  // the launch order of enqueue_segmented: wide step 0, then each level
+  // followed by its wide step
+  size_t j = 2;
+  for (int i = 0; i < 3; ++i) LSB_HIP(span(j++, &out->wide_ms[0][i]));
+  for (uint32_t k = 0; k < K; ++k) {
+    for (int i = 0; i < 3; ++i) LSB_HIP(span(j++, &out->level_ms[k][i]));
+    if (k + 1 < S)
+      for (int i = 0; i < 3; ++i) LSB_HIP(span(j++, &out->wide_ms[k + 1][i]));
+  }
+  LSB_HIP(span(last, &out->finish_ms));
+  LSB_HIP(hipEventElapsedTime(&out->total_ms, ev[0], ev[last]));
   return LSB_OK;
 }
 

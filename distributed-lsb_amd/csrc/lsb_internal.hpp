@@ -206,6 +206,8 @@ constexpr uint32_t kMsdWideBits = 10;
 constexpr uint32_t kMsdWideMin = 256 * kMsdFinishCap;
 constexpr uint32_t kMsdWideMax = 1024 * kMsdFinishCap;
 constexpr uint32_t kMsdWideKeyBits = 16;
+// Wide steps: step 0 before level 0, step s > 0 after level s - 1.
+constexpr int kMsdWideSteps = 3;
 
 // flags of an entry
 constexpr uint32_t kMsdInB = 1u;   // its elements are in the scratch buffer
@@ -231,7 +233,8 @@ struct MsdRun {
   uint32_t digits;
 };
 struct MsdLevel {
-  uint32_t entries, tiles, ticket, overflow;
+  uint32_t entries, tiles, ticket;
+  uint32_t moved;  // elements the level's scatter partitions or copies
 };
 struct MsdArgs {
   const SegHeader* hdr;
@@ -246,26 +249,33 @@ struct MsdArgs {
   uint32_t* run_ctr;       // [0] runs emitted, [1] finish ticket, [2] overflow
   uint32_t cap_entries, cap_tiles, cap_runs;
   // Wide partitions (kMsdWideBits-bit digit) of ranges of kMsdWideMin ..
-  // kMsdWideMax elements, in two steps: step 0 takes the segments the host
-  // marked and sends its large buckets to level 0; step 1 takes level 0's
-  // children of that size and sends its large buckets to level 1.  An entry's
-  // `dig` is the wide digit's lowest bit; kMsdInB says where its elements are.
+  // kMsdWideMax elements, in kMsdWideSteps steps: step 0 takes the segments
+  // the host marked and sends its large buckets to level 0; step s > 0 takes
+  // level s-1's children of that size and sends its large buckets to level s
+  // (a range's first partition is at level 0, or at level 1 after a level
+  // that found its top digit constant).  An entry's `dig` is the wide
+  // digit's lowest bit; kMsdInB says where its elements are.  The steps run
+  // one after another and share the wide arrays.
   MsdEntry* went;          // [wide] (tile_begin: first tile among wide tiles)
   uint32_t* wcounts;       // [wide][2^kMsdWideBits] digit counts, then bucket starts
   uint32_t* worr;          // [wide][4] OR and AND of its keys
   uint32_t* wtile;         // wide tile -> wide range
   uint32_t* wstatus;       // [wide tile][2^kMsdWideBits] look-back words
-  MsdLevel* wlvl;          // [2] entries, tiles, ticket of each step
-  uint32_t wstep;          // the step a wide launch runs (0 or 1)
+  MsdLevel* wlvl;          // [kMsdWideSteps] entries, tiles, ticket of each step
+  uint32_t wstep;          // the step a wide launch runs
   uint32_t cap_wide, cap_wide_tiles;
   uint32_t key_bits;
   KeyOrder ko;
   uint32_t level;
 };
-// One kernel each; every launch reads its work counts from the device.
+// Every launch reads its work counts from the device.  A wide step and a
+// level are three launches (histogram, plan, scatter); with `after` set, an
+// event is recorded after each of them into after[0..2].
 hipError_t launch_msd_init(const MsdArgs& a, uint32_t grid, hipStream_t s);
-hipError_t launch_msd_wide(const MsdArgs& a, uint32_t cus, hipStream_t s);
-hipError_t launch_msd_level(const MsdArgs& a, uint32_t cus, hipStream_t s);
+hipError_t launch_msd_wide(const MsdArgs& a, uint32_t cus, hipStream_t s,
+                           hipEvent_t* after = nullptr);
+hipError_t launch_msd_level(const MsdArgs& a, uint32_t cus, hipStream_t s,
+                            hipEvent_t* after = nullptr);
 hipError_t launch_msd_finish(const MsdArgs& a, uint32_t cus, hipStream_t s);
 
 // Workload generation and verification (lsb_workload.hip).

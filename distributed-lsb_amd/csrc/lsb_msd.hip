@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_msd_init(MsdArgs a) {
                   0u, 0u};
     a.wlvl[0] = w;
     MsdLevel z = {0u, 0u, 0u, 0u};
-    a.wlvl[1] = z;
+    for (int i = 1; i < kMsdWideSteps; ++i) a.wlvl[i] = z;
   }
   // A segment marked wide goes to the wide partition; its level-0 entry is
   // left empty and its level-0 tiles map to no entry.
@@ -326,7 +326,11 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
   __shared__ uint32_t s_misc[4];
   for (uint32_t e = blockIdx.x; e < nent; e += gridDim.x) {
     const MsdEntry E = a.ent[L][e];
-    if ((E.flags & kMsdCopy) || E.n == 0) continue;
+    if ((E.flags & kMsdCopy) || E.n == 0) {
+      // a copy (emitted by the level before) is moved by this level's scatter
+      if (t == 0 && E.n) atomicAdd(&a.lvl[k].moved, E.n);
+      continue;
+    }
     const uint32_t rem = umin(8u * (E.dig + 1u), a.key_bits);
     // bits in which the range's keys differ: OR of the keys ^ AND of the keys
     const uint4 oa = reinterpret_cast<const uint4*>(a.orr[L])[e];
@@ -335,7 +339,10 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
     const bool in_b = (E.flags & kMsdInB) != 0;
     if (x == 0) {
       // every key equal: done where it is, or copied to A by this level's scatter
-      if (in_b && t == 0) a.ent[L][e].flags = E.flags | kMsdCopy;
+      if (in_b && t == 0) {
+        a.ent[L][e].flags = E.flags | kMsdCopy;
+        atomicAdd(&a.lvl[k].moved, E.n);
+      }
       continue;
     }
     if (((x >> (8u * E.dig)) & 0xFFull) == 0) {
@@ -351,11 +358,11 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
     const bool out_b = !in_b;
     const uint32_t c = a.counts[L][(size_t)e * kNB + t];
     const bool large = c > C;
-    // At level 0 a bucket of the wide size range whose next varying digit
-    // leaves room for a wide digit goes to wide step 1 instead of level 1:
-    // its buckets then finish after one more partition.
-    const bool to_wide = k == 0 && large && c > kMsdWideMin && c <= kMsdWideMax && cd >= 1 &&
-                         a.key_bits >= kMsdWideKeyBits;
+    // At level k < kMsdWideSteps - 1 a bucket of the wide size range whose
+    // next varying digit leaves room for a wide digit goes to wide step k+1
+    // instead of level k+1: its buckets then finish after one more partition.
+    const bool to_wide = k + 1 < (uint32_t)kMsdWideSteps && large && c > kMsdWideMin &&
+                         c <= kMsdWideMax && cd >= 1 && a.key_bits >= kMsdWideKeyBits;
     const bool emit = large && !to_wide && (cd >= 0 || out_b);
     const uint32_t nt = (c + kMsdTile - 1) / kMsdTile;
     const uint32_t my_q = large ? C + 1 : c;
@@ -386,14 +393,15 @@ __global__ __launch_bounds__(256) void k_msd_plan(MsdArgs a) {
       s_misc[0] = be;
       s_misc[1] = bt;
       a.ent[L][e].flags = E.flags | kMsdMove;
+      atomicAdd(&a.lvl[k].moved, E.n);
     }
     any_small = __syncthreads_or(any_small);
     if (to_wide) {
       constexpr uint32_t WNB = 1u << kMsdWideBits;
       const uint32_t nt = (c + kMsdTile - 1) / kMsdTile;
       const uint64_t ca = E.a + 16ull * start, cb = E.b + 16ull * start;
-      const uint32_t id = atomicAdd(&a.wlvl[1].entries, 1u);
-      const uint32_t tb = atomicAdd(&a.wlvl[1].tiles, nt);
+      const uint32_t id = atomicAdd(&a.wlvl[k + 1].entries, 1u);
+      const uint32_t tb = atomicAdd(&a.wlvl[k + 1].tiles, nt);
       if (id >= a.cap_wide || (uint64_t)tb + nt > a.cap_wide_tiles) {
         atomicAdd(&a.run_ctr[2], 1u);  // cannot happen within the workspace's capacities
         if (id < a.cap_wide) {
@@ -894,6 +902,7 @@ __global__ __launch_bounds__(256) void k_msd_wplan(MsdArgs a) {
       s_misc[0] = tot ? atomicAdd(&a.lvl[kt].entries, tot & 4095u) : 0u;
       s_misc[1] = tot ? atomicAdd(&a.lvl[kt].tiles, tot >> 12) : 0u;
       a.went[w].flags = E.flags | kMsdMove;
+      atomicAdd(&a.wlvl[kt].moved, E.n);
     }
     any_small = __syncthreads_or(any_small);
     const uint32_t NL = kt & 1u;
@@ -1061,21 +1070,35 @@ hipError_t launch_msd_init(const MsdArgs& a, uint32_t grid, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_msd_wide(const MsdArgs& a, uint32_t cus, hipStream_t s) {
+namespace {
+hipError_t mark(hipEvent_t* after, int i, hipStream_t s) {
+  return after ? hipEventRecord(after[i], s) : hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_msd_wide(const MsdArgs& a, uint32_t cus, hipStream_t s, hipEvent_t* after) {
+  hipError_t e;
   hipLaunchKernelGGL(k_msd_whist, dim3(2 * cus), dim3(256), 0, s, a);
+  if ((e = mark(after, 0, s)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_msd_wplan, dim3(cus), dim3(256), 0, s, a);
+  if ((e = mark(after, 1, s)) != hipSuccess) return e;
   hipLaunchKernelGGL((k_msd_wscatter<kScatterBlock, kScatterItems>), dim3(cus),
                      dim3(kScatterBlock), 0, s, a);
+  if ((e = mark(after, 2, s)) != hipSuccess) return e;
   return hipGetLastError();
 }
 
-hipError_t launch_msd_level(const MsdArgs& a, uint32_t cus, hipStream_t s) {
+hipError_t launch_msd_level(const MsdArgs& a, uint32_t cus, hipStream_t s, hipEvent_t* after) {
   static const uint32_t scatter_occ =
       resident_blocks(k_msd_scatter<kScatterBlock, kScatterItems>, kScatterBlock);
+  hipError_t e;
   hipLaunchKernelGGL(k_msd_hist, dim3(2 * cus), dim3(256), 0, s, a);
+  if ((e = mark(after, 0, s)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_msd_plan, dim3(4 * cus), dim3(256), 0, s, a);
+  if ((e = mark(after, 1, s)) != hipSuccess) return e;
   hipLaunchKernelGGL((k_msd_scatter<kScatterBlock, kScatterItems>), dim3(scatter_occ * cus),
                      dim3(kScatterBlock), 0, s, a);
+  if ((e = mark(after, 2, s)) != hipSuccess) return e;
   return hipGetLastError();
 }
 

@@ -6,7 +6,8 @@
 // object is loaded there, and that stays until the process ends.  The first
 // call on a device looks the array's address up once (hipGetSymbolAddress,
 // with that device current) and keeps it; every call on that device then
-// passes that address to the same launch sequence lsb_sort enqueues, so the
+// passes that address to the launch sequence lsb_sort enqueues in its LSD
+// form (the MSD form stages a range table in pinned host memory), so the
 // results are identical.  The kernels clear the counters, tickets and status
 // words they use themselves, so what the last call left in the array does
 // not matter; two calls that overlap on one device share the array and race.
@@ -73,6 +74,9 @@ int lsb_sort_noalloc(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg) 
   // would make the scratch array pointless.
   if (c.flags & (LSB_GRAPH | LSB_PROFILE)) return LSB_EINVAL;
   if (c.workspace || c.workspace_bytes) return LSB_EINVAL;
+  // The MSD form stages its range table in pinned host memory: LSD only.
+  if ((c.flags & LSB_FORM_MSD) && !(c.flags & LSB_FORM_LSD)) return LSB_EINVAL;
+  c.flags |= LSB_FORM_LSD;
   if (n > LSB_NOALLOC_MAX_N) return LSB_ERANGE;
   int rc = lsb::check_sort_args(A, B, n);
   if (rc || n <= 1) return rc;

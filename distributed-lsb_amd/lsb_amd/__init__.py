@@ -22,8 +22,13 @@ LSB_OK, LSB_EINVAL, LSB_ENOMEM, LSB_EHIP, LSB_ERANGE, LSB_EBUSY, LSB_ENOTFOUND =
     0, -1, -2, -3, -4, -5, -6)
 LSB_SYNC, LSB_NO_SKIP, LSB_PROFILE, LSB_GRAPH = 1, 2, 4, 8
 LSB_KEYS_SIGNED, LSB_KEYS_FLOAT = 16, 32
-LSB_SEG_LSD = 64
-LSB_SEG_MSD = 128
+LSB_FORM_LSD = 64
+LSB_FORM_MSD = 128
+LSB_SEG_LSD = LSB_FORM_LSD
+LSB_SEG_MSD = LSB_FORM_MSD
+LSB_MSD_MIN_N = 1 << 24
+LSB_MSD_MAX_LEVELS = 9
+LSB_MSD_WIDE_STEPS = 3
 # key_order names accepted by the sort, verify and server calls
 KEY_ORDERS = {"unsigned": 0, "signed": LSB_KEYS_SIGNED, "float": LSB_KEYS_FLOAT}
 LSB_MAX_N = 0x7FFFFFFE
@@ -66,6 +71,18 @@ class Profile(ctypes.Structure):
                 ("pass_ms", ctypes.c_float * 16)]
 
 
+class MsdProfile(ctypes.Structure):
+    """struct lsb_msd_profile."""
+    _fields_ = [("levels", ctypes.c_uint32), ("wide_steps", ctypes.c_uint32),
+                ("runs", ctypes.c_uint32),
+                ("wide_elems", ctypes.c_uint32 * LSB_MSD_WIDE_STEPS),
+                ("level_elems", ctypes.c_uint32 * LSB_MSD_MAX_LEVELS),
+                ("finish_elems", ctypes.c_uint32), ("setup_ms", ctypes.c_float),
+                ("wide_ms", (ctypes.c_float * 3) * LSB_MSD_WIDE_STEPS),
+                ("level_ms", (ctypes.c_float * 3) * LSB_MSD_MAX_LEVELS),
+                ("finish_ms", ctypes.c_float), ("total_ms", ctypes.c_float)]
+
+
 class ServerConfig(ctypes.Structure):
     """struct lsb_server_config."""
     _fields_ = [("memory_budget", ctypes.c_size_t), ("max_pending", ctypes.c_uint32),
@@ -104,6 +121,7 @@ SIGNATURES = {
     "lsb_sort_noalloc": (_I, [_P, _P, _SZ, ctypes.POINTER(Config)]),
     "lsb_noalloc_scratch_bytes": (_SZ, []),
     "lsb_workspace_size": (_SZ, [_SZ, ctypes.POINTER(Config)]),
+    "lsb_sort_form": (_I, [_SZ, ctypes.POINTER(Config)]),
     "lsb_plan": (_I, [_SZ, ctypes.POINTER(Config), ctypes.POINTER(_U32),
                       ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
     "lsb_variants": (_I, [_P, _P, _P, _I]),
@@ -131,6 +149,7 @@ SIGNATURES = {
     "lsb_tune_load": (_I, [ctypes.c_char_p]),
     "lsb_tune_clear": (_I, []),
     "lsb_profile_read": (_I, [ctypes.POINTER(Profile)]),
+    "lsb_msd_profile_read": (_I, [ctypes.POINTER(MsdProfile)]),
     "lsb_debug_histogram": (_I, [_P, _SZ, ctypes.POINTER(Config), _P]),
     "lsb_generate": (_I, [_P, _SZ, ctypes.POINTER(GenSpec), _P]),
     "lsb_verify": (_I, [_P, _P, _SZ, _U64, _U32, _U32, ctypes.POINTER(_U64),
@@ -243,9 +262,19 @@ def plan(n, key_bits=64, digit_bits=0, tile=0, variant=0):
     return d.value, t.value, p.value
 
 
-def workspace_size(n, key_bits=64, digit_bits=0, tile=0, variant=0):
-    c = make_config(key_bits, digit_bits, tile, stream=0, variant=variant)
+def workspace_size(n, key_bits=64, digit_bits=0, tile=0, variant=0, lsd=None):
+    c = make_config(key_bits, digit_bits, tile, stream=0, flags=_form_flag(lsd),
+                    variant=variant)
     return lib().lsb_workspace_size(n, ctypes.byref(c))
+
+
+def sort_form(n, key_bits=64, digit_bits=0, tile=0, variant=0, lsd=None, no_skip=False):
+    """The form sort() takes for n elements: "lsd" or "msd"."""
+    c = make_config(key_bits, digit_bits, tile, stream=0,
+                    flags=_form_flag(lsd) | (LSB_NO_SKIP if no_skip else 0), variant=variant)
+    rc = lib().lsb_sort_form(n, ctypes.byref(c))
+    _check(rc if rc < 0 else LSB_OK, "lsb_sort_form")
+    return "msd" if rc == LSB_FORM_MSD else "lsd"
 
 
 def _key_flags(key_order):
@@ -256,13 +285,15 @@ def _key_flags(key_order):
 
 def sort(A, B, n=None, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,
          no_skip=False, workspace=None, workspace_bytes=0, profile=False, graph=False,
-         variant=0, key_order="unsigned"):
+         variant=0, key_order="unsigned", lsd=None):
     """mySort(A, B): stable sort of A[0:n] by key, B is scratch, result in A.
-    key_order: "unsigned" (default), "signed" or "float" keys."""
+    key_order: "unsigned" (default), "signed" or "float" keys.  lsd=True
+    runs the LSD passes, lsd=False most significant digit first, None the
+    library's choice (sort_form tells which)."""
     n = _nelem(A) if n is None else n
     flags = ((LSB_SYNC if sync else 0) | (LSB_NO_SKIP if no_skip else 0)
              | (LSB_PROFILE if profile else 0) | (LSB_GRAPH if graph else 0)
-             | _key_flags(key_order))
+             | _key_flags(key_order) | _form_flag(lsd))
     c = make_config(key_bits, digit_bits, tile, stream, flags, workspace, workspace_bytes,
                     variant)
     _check(lib().lsb_sort(_ptr(A), _ptr(B), n, ctypes.byref(c)), "lsb_sort")
@@ -285,10 +316,10 @@ def noalloc_scratch_bytes():
 
 
 def _form_flag(lsd):
-    """Batched form: None (the library's choice), True (LSD), False (MSD first)."""
+    """Sort form: None (the library's choice), True (LSD), False (MSD first)."""
     if lsd is None:
         return 0
-    return LSB_SEG_LSD if lsd else LSB_SEG_MSD
+    return LSB_FORM_LSD if lsd else LSB_FORM_MSD
 
 
 def sort_segmented(A, B, offsets, key_bits=64, digit_bits=0, tile=0, stream=None, sync=False,
@@ -375,6 +406,23 @@ def profile_read():
             "digit_bits": p.digit_bits, "tile": p.tile, "histogram_ms": p.histogram_ms,
             "scan_ms": p.scan_ms, "copy_ms": p.copy_ms, "total_ms": p.total_ms,
             "pass_ms": [p.pass_ms[q] for q in range(P)]}
+
+
+def msd_profile_read():
+    """Kernel timings (ms) and element counts of the last sort(profile=True)
+    on this thread that ran the MSD form.  Launch order: wide step 0, then
+    level k followed by wide step k+1 while k + 1 < wide_steps; each is
+    [histogram, plan, scatter]."""
+    p = MsdProfile()
+    _check(lib().lsb_msd_profile_read(ctypes.byref(p)), "lsb_msd_profile_read")
+    K, S = p.levels, p.wide_steps
+    return {"levels": K, "wide_steps": S, "runs": p.runs,
+            "wide_elems": [p.wide_elems[w] for w in range(S)],
+            "level_elems": [p.level_elems[k] for k in range(K)],
+            "finish_elems": p.finish_elems, "setup_ms": p.setup_ms,
+            "wide_ms": [list(p.wide_ms[w]) for w in range(S)],
+            "level_ms": [list(p.level_ms[k]) for k in range(K)],
+            "finish_ms": p.finish_ms, "total_ms": p.total_ms}
 
 
 def debug_histogram(A, counts, n=None, key_bits=64, digit_bits=0, stream=None, sync=True,

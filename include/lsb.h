@@ -63,15 +63,25 @@ typedef struct lsb_elem {
  * (chpl/arkouda-radix-sort.chpl:37-39). */
 #define LSB_KEYS_SIGNED 16u
 #define LSB_KEYS_FLOAT 32u
-/* Batched sorts (lsb_sort_segmented, lsb_sort_batch): the form for the
- * segments larger than one workgroup's capacity (same results either way;
- * see lsb_sort_segmented).  LSB_SEG_LSD: LSD passes over every digit, like
- * lsb_sort.  LSB_SEG_MSD: most significant digit first.  Neither: MSD first
- * for keys wider than 32 bits, LSD for the others (on MI355X the two run
- * level for uniform 32-bit keys and LSD is faster for skewed ones).
- * LSB_SEG_LSD takes precedence. */
-#define LSB_SEG_LSD 64u
-#define LSB_SEG_MSD 128u
+/* Form of the sort (same results either way).  LSB_FORM_LSD: the
+ * reference's order, one stable pass per digit from the least significant
+ * (mpi/mpi_lsbsort.cpp:591-597).  LSB_FORM_MSD: most significant digit
+ * first -- every range is partitioned on its highest varying digit until
+ * its buckets fit one workgroup, which sorts them in LDS by the digits left
+ * (see lsb_sort_segmented).  LSB_FORM_LSD takes precedence; an explicit
+ * digit width other than 8, tile or variant, or LSB_NO_SKIP, selects LSD.
+ * Neither flag: lsb_sort runs MSD for keys wider than 32 bits and
+ * n >= LSB_MSD_MIN_N with automatic digit width, tile and variant, and LSD
+ * otherwise; batched sorts (lsb_sort_segmented, lsb_sort_batch) run MSD for
+ * their segments above one workgroup's capacity when keys are wider than 32
+ * bits, and LSD otherwise (on MI355X the two forms run level for uniform
+ * 32-bit keys and LSD is faster for skewed ones).  lsb_sort_form tells
+ * which form lsb_sort takes. */
+#define LSB_FORM_LSD 64u
+#define LSB_FORM_MSD 128u
+#define LSB_SEG_LSD LSB_FORM_LSD /* names of the flags in the batched sorts */
+#define LSB_SEG_MSD LSB_FORM_MSD
+#define LSB_MSD_MIN_N ((size_t)1 << 24)
 
 typedef struct lsb_config {
   uint32_t key_bits;          /* sort by the low key_bits bits of key (1..64); 0 = 64.
@@ -87,7 +97,7 @@ typedef struct lsb_config {
                                  NULL = library-managed, cached per (device, stream) */
   size_t workspace_bytes;
   uint32_t flags;             /* LSB_SYNC | LSB_NO_SKIP | LSB_PROFILE | LSB_GRAPH |
-                                 LSB_KEYS_SIGNED | LSB_KEYS_FLOAT */
+                                 LSB_KEYS_SIGNED | LSB_KEYS_FLOAT | LSB_FORM_* */
   uint32_t reserved1;
 } lsb_config;
 
@@ -95,17 +105,25 @@ typedef struct lsb_config {
  * stream, library workspace, asynchronous). */
 void lsb_config_init(lsb_config* cfg);
 
-/* Stable LSD sort of A[0..n) by key, B[0..n) is scratch; result in A.
+/* Stable sort of A[0..n) by key, B[0..n) is scratch; result in A.
  * Replaces mySort (mpi/mpi_lsbsort.cpp:591-597; shmem/shmem_lsbsort.cpp:
- * 469-482).  n = 0 and n = 1 succeed without touching memory.  cfg may be
- * NULL.  Returns LSB_OK or a negative LSB_E* code. */
+ * 469-482), an LSD sort; the form (LSB_FORM_LSD / LSB_FORM_MSD) changes the
+ * order of the work, never the result.  n = 0 and n = 1 succeed without
+ * touching memory.  cfg may be NULL.  Returns LSB_OK or a negative LSB_E*
+ * code. */
 int lsb_sort(lsb_elem* A, lsb_elem* B, size_t n, const lsb_config* cfg);
+
+/* The form lsb_sort(n, cfg) takes: LSB_FORM_LSD or LSB_FORM_MSD, or a
+ * negative LSB_E* code for a configuration lsb_sort rejects. */
+int lsb_sort_form(size_t n, const lsb_config* cfg);
 
 /* lsb_sort with no device allocation at call time, for callers that cannot
  * allocate (code under hipGraph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  No counterpart in the reference, whose mySort
  * allocates its temporaries per pass (mpi/mpi_lsbsort.cpp:523-528, 550-555).
- * Same contract, launch sequence and results as lsb_sort, but the workspace
+ * Same contract and results as lsb_sort, always in the LSD form (the MSD
+ * form stages its range table in pinned host memory; LSB_FORM_MSD is
+ * LSB_EINVAL here), with the LSD form's launch sequence; the workspace
  * is a fixed device array of lsb_noalloc_scratch_bytes() bytes that every
  * GPU the library runs on holds until the process ends; the first call on a
  * device looks up that device's copy.  Accepts n <= LSB_NOALLOC_MAX_N
@@ -119,8 +137,9 @@ size_t lsb_noalloc_scratch_bytes(void);
 /* Bytes of device workspace lsb_sort(n, cfg) needs (0 for n <= 1). */
 size_t lsb_workspace_size(size_t n, const lsb_config* cfg);
 
-/* Digit width and tile that lsb_sort will use for (n, cfg); returns
- * LSB_OK or LSB_EINVAL. */
+/* Digit width, tile and passes of the LSD form for (n, cfg) (what
+ * lsb_sort uses when lsb_sort_form says LSB_FORM_LSD); returns LSB_OK or
+ * LSB_EINVAL. */
 int lsb_plan(size_t n, const lsb_config* cfg, uint32_t* digit_bits, uint32_t* tile,
              uint32_t* passes);
 
@@ -284,7 +303,10 @@ int lsb_server_reset_stats(lsb_server* srv);
 /* Kernel timings of the last lsb_sort call made with LSB_PROFILE on this
  * thread, from HIP events recorded on its stream between its launches.  No
  * counterpart in the reference, which times only the whole sort with
- * steady_clock between barriers (mpi/mpi_lsbsort.cpp:731-745). */
+ * steady_clock between barriers (mpi/mpi_lsbsort.cpp:731-745).
+ * lsb_profile_read covers a call that ran the LSD form, lsb_msd_profile_read
+ * one that ran the MSD form; each returns LSB_EINVAL after the other form.
+ * Both read the call's workspace: read before the next sort on the stream. */
 typedef struct lsb_profile {
   uint32_t passes;         /* digit passes planned */
   uint32_t executed_mask;  /* bit p set when pass p ran (not single-bucket) */
@@ -299,6 +321,32 @@ typedef struct lsb_profile {
 
 /* Waits for the profiled call's events and fills *out. */
 int lsb_profile_read(lsb_profile* out);
+
+/* The MSD form, kernel by kernel: the set-up, wide step 0 (ranges of
+ * 2^21..2^23 elements partitioned on a 10-bit digit), then level k (8-bit
+ * digit; the last level only copies) followed by wide step k+1 (level k's
+ * buckets of 2^21..2^23 elements) for k + 1 < wide_steps, and the in-LDS
+ * finish of every run.  A step is a histogram, a plan and a scatter launch.
+ * The element counts are what each scatter moved (partitioned or copied)
+ * and what the runs of the finish held: a scatter's and the finish's
+ * algorithmic traffic is 32 B per element (16 read, 16 written), a
+ * histogram's 8 B per element of its ranges. */
+#define LSB_MSD_MAX_LEVELS 9
+#define LSB_MSD_WIDE_STEPS 3
+typedef struct lsb_msd_profile {
+  uint32_t levels;                           /* levels launched */
+  uint32_t wide_steps;                       /* wide steps launched */
+  uint32_t runs;                             /* runs the finish sorted */
+  uint32_t wide_elems[LSB_MSD_WIDE_STEPS];   /* elements wide step s moved */
+  uint32_t level_elems[LSB_MSD_MAX_LEVELS];  /* elements level k moved */
+  uint32_t finish_elems;                     /* elements of the runs */
+  float setup_ms;                            /* small segments + level-0 set-up */
+  float wide_ms[LSB_MSD_WIDE_STEPS][3];      /* wide step s: histogram, plan, scatter */
+  float level_ms[LSB_MSD_MAX_LEVELS][3];     /* level k: histogram, plan, scatter */
+  float finish_ms;
+  float total_ms;                            /* first event to last */
+} lsb_msd_profile;
+int lsb_msd_profile_read(lsb_msd_profile* out);
 
 /* Debug entry for the all-digit histogram kernel alone: writes
  * counts[p * 2^d + b] (uint32, device memory, P = ceil(key_bits/d) rows) for

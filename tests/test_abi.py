@@ -67,7 +67,7 @@ def test_plan_and_workspace():
     assert lsb_amd.plan(1 << 24, key_bits=32, digit_bits=8)[2] == 4
     assert lsb_amd.plan(1 << 24, key_bits=1, digit_bits=8)[2] == 1
     assert lsb_amd.workspace_size(0) == 0 and lsb_amd.workspace_size(1) == 0
-    w1, w2 = lsb_amd.workspace_size(1 << 20), lsb_amd.workspace_size(1 << 24)
+    w1, w2 = lsb_amd.workspace_size(1 << 20), lsb_amd.workspace_size(1 << 24, lsd=True)
     assert 0 < w1 < w2
     # two look-back slices of tiles x 2^d words dominate
     d, tile, _ = lsb_amd.plan(1 << 24)
@@ -88,6 +88,36 @@ def test_plan_and_workspace():
         rc = lsb_amd.lib().lsb_plan(1 << 20, ctypes.byref(c), None, None, None)
         assert (rc == 0) == ok, (kb, flags, rc)
         assert (lsb_amd.lib().lsb_workspace_size(1 << 20, ctypes.byref(c)) > 0) == ok
+
+
+def test_sort_form_choice():
+    """lsb_sort takes the MSD form for keys wider than 32 bits from
+    LSB_MSD_MIN_N elements on, with the digit width, tile and variant left
+    automatic; the form flags, an explicit LSD configuration or LSB_NO_SKIP
+    override it; the workspace size follows the form."""
+    m = lsb_amd.LSB_MSD_MIN_N
+    f = lsb_amd.sort_form
+    assert f(m) == "msd" and f(1 << 30) == "msd" and f(m, key_bits=33) == "msd"
+    assert f(m - 1) == "lsd" and f(m, key_bits=32) == "lsd" and f(m, key_bits=20) == "lsd"
+    assert f(m, digit_bits=8) == "lsd" and f(m, digit_bits=11) == "lsd"
+    assert f(m, no_skip=True) == "lsd" and f(m, lsd=True) == "lsd"
+    vs = lsb_amd.variants()
+    assert f(m, variant=vs[0][0]) == "lsd"
+    # forced MSD: any size and width, unless the configuration is LSD-only
+    assert f(100, lsd=False) == "msd" and f(m, key_bits=16, lsd=False) == "msd"
+    assert f(m, digit_bits=8, lsd=False) == "msd"
+    assert f(m, digit_bits=11, lsd=False) == "lsd" and f(m, tile=4096, lsd=False) == "lsd"
+    assert f(m, no_skip=True, lsd=False) == "lsd"
+    c = lsb_amd.make_config(key_bits=65, stream=0)
+    assert lsb_amd.lib().lsb_sort_form(m, ctypes.byref(c)) == lsb_amd.LSB_EINVAL
+    c = lsb_amd.make_config(stream=0)
+    assert lsb_amd.lib().lsb_sort_form(lsb_amd.LSB_MAX_N + 1, ctypes.byref(c)) == \
+        lsb_amd.LSB_ERANGE
+    # the workspace follows the form: the MSD layout holds the level lists
+    w_msd, w_lsd = lsb_amd.workspace_size(m), lsb_amd.workspace_size(m, lsd=True)
+    assert w_msd > 0 and w_lsd > 0 and w_msd != w_lsd
+    assert w_msd == lsb_amd.workspace_size(m, lsd=False)
+    assert w_lsd == lsb_amd.workspace_size(m, digit_bits=8)
 
 
 def test_generate_and_verify_reject_bad_args():

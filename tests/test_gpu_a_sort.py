@@ -636,3 +636,144 @@ def test_scatter_floor_probe_layout(runs):
     for bad in (dict(runs=3), dict(runs=0), dict(runs=16384), dict(n=n - 1)):
         with pytest.raises(lsb_amd.LsbError):
             lsb_amd.debug_scatter_floor(src, dst, **bad)
+
+
+def _forms_equal(A_in, **kw):
+    """Sorts copies of A_in in the automatic form and in the LSD form and
+    returns (automatic form, the two results equal bit for bit)."""
+    n = A_in.shape[0]
+    form = lsb_amd.sort_form(n, key_bits=kw.get("key_bits", 64))
+    A1, A2 = A_in.clone(), A_in.clone()
+    B = torch.empty_like(A1)
+    lsb_amd.sort(A1, B, sync=True, **kw)
+    lsb_amd.sort(A2, B, sync=True, lsd=True, **kw)
+    same = torch.equal(A1, A2)
+    del A1, A2, B
+    return form, same
+
+
+@pytest.mark.parametrize("case", ["uniform", "zipf", "few", "high_constant", "kb40",
+                                  "signed", "float", "sorted"])
+def test_plain_msd_form_equals_lsd_form(case):
+    """From LSB_MSD_MIN_N elements on, lsb_sort sorts wider-than-32-bit keys
+    most significant digit first; its result is the LSD form's bit for bit
+    (which the oracle pins at smaller sizes), and both are the stable sort
+    (device verifier + torch's stable sort)."""
+    n = (1 << 24) + 12345
+    A = torch.empty((n, 2), dtype=torch.int64, device=DEV)
+    kw = {}
+    if case == "uniform":
+        lsb_amd.generate(A, kind=lsb_amd.GEN_UNIFORM, ranks=4)
+    elif case == "zipf":
+        lsb_amd.generate(A, kind=lsb_amd.GEN_ZIPF, ranks=4, zipf_m=1 << 16)
+    elif case == "few":
+        lsb_amd.generate(A, kind=lsb_amd.GEN_UNIFORM, ranks=4)
+        A[:, 0] = torch.bitwise_and(A[:, 0], 3) << 40
+    elif case == "high_constant":   # top 3 bytes one value: a pass-through level
+        lsb_amd.generate(A, kind=lsb_amd.GEN_UNIFORM, ranks=4)
+        A[:, 0] = torch.bitwise_or(torch.bitwise_and(A[:, 0], (1 << 40) - 1), 0x5A << 48)
+    elif case == "kb40":
+        lsb_amd.generate(A, kind=lsb_amd.GEN_UNIFORM, ranks=4)
+        kw["key_bits"] = 40
+    elif case == "signed":
+        lsb_amd.generate(A, kind=lsb_amd.GEN_UNIFORM, ranks=4)
+        A[:, 0] = A[:, 0] >> 20    # arithmetic: both signs, 44 significant bits
+        kw["key_order"] = "signed"
+    elif case == "float":
+        g = torch.Generator(device=DEV).manual_seed(7)
+        f = torch.randn(n, dtype=torch.float64, device=DEV, generator=g) * 1e6
+        f[: n // 16] = 0.0
+        A[:, 0] = f.view(torch.int64)
+        A[:, 1] = torch.arange(n, dtype=torch.int64, device=DEV)
+        kw["key_order"] = "float"
+    else:
+        lsb_amd.generate(A, kind=lsb_amd.GEN_SORTED)
+    torch.cuda.synchronize()
+    form, same = _forms_equal(A, **kw)
+    assert form == "msd" and same, (case, form)
+    if kw.get("key_order", "unsigned") == "unsigned" and "key_bits" not in kw:
+        keys = torch.empty(n, dtype=torch.int64, device=DEV)
+        lsb_amd.extract_keys(A, keys)
+        B = torch.empty_like(A)
+        lsb_amd.sort(A, B, sync=True)
+        del B
+        assert _device_check(A, keys, n) == (0, -1)
+        assert _torch_stable_check(A, keys)
+        del keys
+    del A
+    torch.cuda.empty_cache()
+
+
+def test_plain_msd_reference_digest():
+    """The reference's 4-rank 2^24 input sorted in the automatic (MSD) form
+    reproduces the sorted digest of the reference's own output."""
+    n = 1 << 24
+    assert lsb_amd.sort_form(n) == "msd"
+    A = torch.empty((n, 2), dtype=torch.int64, device=DEV)
+    lsb_amd.generate(A, kind=lsb_amd.GEN_UNIFORM, ranks=4)
+    B = torch.empty_like(A)
+    lsb_amd.sort(A, B, sync=True)
+    assert O.sha256(from_dev(A)) == \
+        "edbb62f7b024eaade41bc12278e7cb240d038889a9780f8913a336352530d0c9"
+
+
+@pytest.mark.parametrize("key_bits", [1, 16, 32, 33, 64])
+def test_plain_msd_forced_small_sizes(key_bits):
+    """lsd=False forces the MSD form at any size and key width: bit-exact
+    against the oracle's stable sort, sizes around the run and tile
+    capacity (8192) and the one-workgroup paths."""
+    for n in (2, 1000, 8192, 8193, 65537, 100003, (1 << 20) + 7):
+        a = O.gen_uniform(n, ranks=3, seed_base=n + key_bits)
+        if n > 8193:
+            a["key"][: n // 3] = a["key"][n // 3: 2 * (n // 3)]  # duplicates
+        assert lsb_amd.sort_form(n, key_bits=key_bits, lsd=False) == "msd"
+        got = gpu_sort(a, key_bits=key_bits, lsd=False)
+        assert np.array_equal(got, stable(a, key_bits)), (n, key_bits)
+
+
+def test_plain_msd_workspace_graph_and_profile():
+    """The MSD form with a caller workspace of workspace_size(n) bytes (and
+    too small a one refused), as a captured graph replayed on new data, and
+    profiled: lsb_msd_profile_read's element counts follow the input
+    (uniform 2^24 + 3: level 0 and level 1 each move every element, the
+    finish's runs hold them all) and lsb_profile_read refuses, as
+    lsb_msd_profile_read does after an LSD sort."""
+    n = (1 << 24) + 3
+    a = O.gen_uniform(n, ranks=4, seed_base=99)
+    want = stable(a)
+    A = to_dev(a)
+    B = torch.empty_like(A)
+    ws = torch.empty(lsb_amd.workspace_size(n), dtype=torch.uint8, device=DEV)
+    lsb_amd.sort(A, B, workspace=ws, sync=True)
+    assert np.array_equal(from_dev(A), want)
+    small = torch.empty(lsb_amd.workspace_size(n, lsd=True) // 2, dtype=torch.uint8, device=DEV)
+    with pytest.raises(lsb_amd.LsbError):
+        lsb_amd.sort(A, B, workspace=small)
+    del ws, small
+    # graph: two replays on fresh inputs, eager work and a sync in between
+    for seed in (5, 6, 7):
+        b = O.gen_uniform(n, ranks=2, seed_base=seed)
+        A.copy_(to_dev(b))
+        lsb_amd.sort(A, B, graph=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(A), stable(b)), seed
+    # profile
+    A.copy_(to_dev(a))
+    lsb_amd.sort(A, B, profile=True, sync=True)
+    p = lsb_amd.msd_profile_read()
+    assert np.array_equal(from_dev(A), want)
+    assert p["levels"] == 9 and p["level_elems"][0] == n and p["level_elems"][1] == n
+    assert p["wide_steps"] == 3 and p["wide_elems"] == [0, 0, 0]
+    assert p["finish_elems"] == n and p["runs"] > 0
+    assert all(x == 0 for x in p["level_elems"][2:])
+    parts = p["setup_ms"] + p["finish_ms"] + sum(map(sum, p["wide_ms"])) + \
+        sum(map(sum, p["level_ms"]))
+    assert p["total_ms"] > 0 and abs(parts - p["total_ms"]) < 0.05 * p["total_ms"] + 0.01
+    with pytest.raises(lsb_amd.LsbError):
+        lsb_amd.profile_read()
+    A.copy_(to_dev(a))
+    lsb_amd.sort(A, B, profile=True, sync=True, lsd=True)
+    assert lsb_amd.profile_read()["total_ms"] > 0
+    with pytest.raises(lsb_amd.LsbError):
+        lsb_amd.msd_profile_read()
+    assert np.array_equal(from_dev(A), want)

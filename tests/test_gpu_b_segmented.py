@@ -540,6 +540,49 @@ def test_automatic_form_for_narrow_keys(key_bits):
             assert np.array_equal(got[lo:hi], want[lo:hi]), (key_bits, lsd, s)
 
 
+def test_msd_wide_step_after_level_one():
+    """Segments whose top byte is one value: level 0 finds it constant and
+    level 1 partitions on the second byte.  In the first (9 M) that byte
+    takes three values: three buckets of about 3 M elements go through wide
+    step 2 (10-bit digit below the second byte) before they finish; in the
+    second (9.5 M) one value holds 60 % (a wide-step bucket of 5.7 M) and the
+    rest spread over the other 255 (level 2), in both key orders; the plain
+    sort of the first in the MSD form moves every element through level 1
+    and wide step 2."""
+    rng = np.random.default_rng(6)
+    a = O.gen_uniform(9_000_000, ranks=4, seed_base=43)
+    second = np.array([0x10, 0x80, 0xF0], dtype=np.uint64)[rng.integers(0, 3, len(a))]
+    a["key"] = (a["key"] & np.uint64((1 << 48) - 1)) | (second << np.uint64(48)) | \
+        np.uint64(0x5A << 56)
+    b = O.gen_uniform(9_500_000, ranks=2, seed_base=44)
+    heavy = rng.random(len(b)) < 0.6
+    b["key"] = (b["key"] & np.uint64((1 << 56) - 1)) | np.uint64(0xC3 << 56)
+    b["key"][heavy] = (b["key"][heavy] & np.uint64((1 << 48) - 1)) | np.uint64(0xC377 << 48)
+    segs = [a, b]
+    sizes = [len(x) for x in segs]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    for order in ("unsigned", "signed"):
+        wants = [O.stable_sort_order(x, 64, order) for x in segs]
+        A = to_dev(np.concatenate(segs))
+        B = torch.empty_like(A)
+        lsb_amd.sort_segmented(A, B, off, key_order=order, sync=True, lsd=False, graph=True)
+        got = from_dev(A)
+        for s in range(len(segs)):
+            lo, hi = int(off[s]), int(off[s + 1])
+            assert np.array_equal(got[lo:hi], wants[s]), (order, s)
+        del A, B
+    A = to_dev(a)
+    B = torch.empty_like(A)
+    lsb_amd.sort(A, B, sync=True, lsd=False, profile=True)
+    p = lsb_amd.msd_profile_read()
+    assert np.array_equal(from_dev(A), O.stable_sort_order(a, 64, "unsigned"))
+    n = len(a)
+    assert p["level_elems"][0] == 0 and p["level_elems"][1] == n, p
+    assert p["wide_elems"] == [0, 0, n] and p["finish_elems"] == n, p
+    del A, B
+    torch.cuda.empty_cache()
+
+
 def test_msd_wide_step_after_level_zero():
     """A segment whose top byte takes three values: level 0 leaves three
     buckets of about 3 M elements, which go through the second wide step

@@ -29,6 +29,8 @@ def test_noalloc_argument_errors_need_no_gpu():
     assert _noalloc_rc(a, b, 5, workspace=0x30000000, workspace_bytes=1 << 20) == \
         lsb_amd.LSB_EINVAL
     assert _noalloc_rc(a + 8, b, 5) == lsb_amd.LSB_EINVAL
+    # the MSD form stages its range table in pinned host memory: refused
+    assert _noalloc_rc(a, b, 5, flags=lsb_amd.LSB_FORM_MSD) == lsb_amd.LSB_EINVAL
     # an explicit variant whose workspace exceeds the scratch array is refused
     small_tile = [v for v, d, t, p in lsb_amd.variants() if d == 8 and t == 2048 and p == 0][0]
     assert _noalloc_rc(a, 1 << 40, lsb_amd.LSB_NOALLOC_MAX_N, variant=small_tile) == \
