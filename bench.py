@@ -811,13 +811,18 @@ def bench_serving(jobs, seed=77, lo_log2=12, span_log2=16, streams=0):
                     "latency": "p50/p99/max of completion - submission over every job of the "
                                "Poisson run (arrivals at 0.8 of the saturated capacity)"})
     # every job's result is the stable sort of its input
-    ok = True
+    bad_jobs = []
     keys = torch.empty(max(j["n"] for j in trace), dtype=torch.int64, device=dev)
-    for a, p, job in zip(bufs, pris, trace):
+    for j, (a, p, job) in enumerate(zip(bufs, pris, trace)):
         lsb_amd.extract_keys(p, keys, n=job["n"])
-        bad, _ = lsb_amd.verify(a, keys, n=job["n"], key_bits=job["key_bits"])
-        ok &= bad == 0
-    out["verified_bit_exact"] = ok
+        bad, first = lsb_amd.verify(a, keys, n=job["n"], key_bits=job["key_bits"])
+        if bad:
+            bad_jobs.append({"job": j, "n": job["n"], "key_bits": job["key_bits"],
+                             "zipf": job["zipf"], "bad": bad, "first": first})
+    out["verified_bit_exact"] = not bad_jobs
+    if bad_jobs:
+        out["bad_jobs"] = bad_jobs[:8]
+        log("SERVING VERIFY FAILED:", len(bad_jobs), "jobs, first:", bad_jobs[:8])
     del bufs, pris, keys
     return out
 

@@ -1176,8 +1176,11 @@ int sort_entries(const std::vector<SegEntry>& ent, uint64_t total_n, const lsb_c
     if (L.msd) {
       // the MSD lists stayed within their capacities (a proven bound; a
       // violation would mean wrong results, so it is reported)
+      // (on the caller's stream: a copy on the legacy stream would be an
+      // implicit dependency for any graph capture another thread has open)
       uint32_t overflow = 0;
-      LSB_HIP(hipMemcpy(&overflow, ws + L.mrunctr + 8, 4, hipMemcpyDeviceToHost));
+      LSB_HIP(hipMemcpyAsync(&overflow, ws + L.mrunctr + 8, 4, hipMemcpyDeviceToHost, s));
+      LSB_HIP(hipStreamSynchronize(s));
       if (overflow) return LSB_ENOMEM;
     }
   }

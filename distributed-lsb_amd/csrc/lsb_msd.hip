@@ -56,6 +56,15 @@ __device__ __forceinline__ int top_varying_digit(uint64_t x, int below) {
 
 __device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
+// The thread index through an empty asm: every use site computes its own
+// copy, so the finish kernel's loop over runs does not hoist the many values
+// derived from it (slot offsets, row limits) out of the loop and spill them.
+__device__ __forceinline__ uint32_t opaque_tid() {
+  uint32_t t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 // LDS passes for a run of n elements over nb non-empty buckets: the partition digit
 // (it keeps the buckets in order) and enough digits below it that a bucket of
 // the average size m = n / nb has few keys left tied for the fix-up: with
@@ -569,15 +578,10 @@ __global__ __launch_bounds__(BLOCK) void k_msd_scatter(MsdArgs a) {
 // A run is sorted in LDS by LSD passes over the highest few of its digits
 // (as many as its buckets' size needs: finish_passes).  Keys that still tie
 // on everything from the lowest sorted digit up then sit next to each other
-// in input order; rounds of odd-even transposition over the LDS copy sort
-// each such group by its remaining bits (neighbours swap only when both tie
-// there and the first key is strictly larger, so equal keys keep their
-// order) until a round changes nothing after a round of the other parity.
-// For uniform keys the groups are mostly pairs and a few rounds do.  Groups
-// of many distinct keys (still swapping after kFixRounds rounds) send the run
-// through LDS passes over every one of its digits instead; long groups of
-// equal keys never swap.
-constexpr int kFixRounds = 8;
+// in input order, and fix_groups sorts each such group that is out of order
+// by insertion.  A group of many distinct keys sends the run through LDS
+// passes over every one of its digits instead; groups of equal keys are in
+// order and cost nothing.
 
 // One stable LDS pass: ranks e[] by the digit at shift/mask and writes the
 // elements to sm.elems in digit order.  Returns false (nothing written) when
@@ -589,7 +593,7 @@ __device__ __forceinline__ bool lds_pass(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t
                                          uint32_t pass_id) {
   constexpr int W = BLOCK / kWave;
   static_assert(BLOCK >= kNB, "one bucket per thread");
-  const uint32_t t = threadIdx.x;
+  const uint32_t t = opaque_tid();
   const uint32_t lane = t & (kWave - 1);
   const uint32_t wave = t / kWave;
   uint32_t* wh = whist + wave * kNB;
@@ -631,8 +635,9 @@ __device__ __forceinline__ void lds_sort(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t
                                          elem_t (&e)[ITEMS], uint32_t lim, uint32_t n,
                                          uint32_t key_bits, uint32_t digits, uint32_t* one_flag,
                                          uint32_t& pass_id) {
-  const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint32_t wrun = (threadIdx.x / kWave) * (kWave * ITEMS);
+  const uint32_t tid = opaque_tid();
+  const uint32_t lane = tid & (kWave - 1);
+  const uint32_t wrun = (tid / kWave) * (kWave * ITEMS);
   bool in_lds = false;  // sm.elems holds the current order (else e[] does)
   for (uint32_t p = 0; p < 8; ++p) {
     if (!((digits >> p) & 1u)) continue;
@@ -666,8 +671,9 @@ constexpr int kRawBufferFlags = 0x00020000;  // dword 3 of a raw gfx9 buffer res
 // reaches the address's high word).
 template <int BLOCK, int ITEMS>
 __device__ __forceinline__ void run_load(elem_t (&e)[ITEMS], const elem_t* src, uint32_t n) {
-  const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint32_t wrun = (threadIdx.x / kWave) * (kWave * ITEMS);
+  const uint32_t tid = opaque_tid();
+  const uint32_t lane = tid & (kWave - 1);
+  const uint32_t wrun = (tid / kWave) * (kWave * ITEMS);
   const uint64_t sa = reinterpret_cast<uint64_t>(src);
   const uint64_t su =
       ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(sa >> 32)) << 32) |
@@ -679,92 +685,399 @@ __device__ __forceinline__ void run_load(elem_t (&e)[ITEMS], const elem_t* src, 
   const uint32_t off = (wrun + lane) * (uint32_t)sizeof(elem_t);
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
+    // the row's offset in the scalar operand: one address register for all rows
     const auto x = __builtin_amdgcn_raw_buffer_load_b128(
-        rs, (int)(off + (uint32_t)(i * kWave * (int)sizeof(elem_t))), 0, 0);
+        rs, (int)off, i * kWave * (int)sizeof(elem_t), 0);
     e[i].key = ((uint64_t)x[1] << 32) | x[0];
     e[i].val = ((uint64_t)x[3] << 32) | x[2];
   }
 }
 
-// Odd-even transposition rounds over sm.elems[0, n) within groups of keys
-// equal from bit `cut` up.  Returns false if the groups are still unsorted
-// after kFixRounds rounds.
-// flags: 2 * BLOCK / 64 words of LDS, one per wave and round parity, so a
-// round's OR needs one barrier.
+// Ties left by the LDS passes.  The passes sorted the run on its key bits
+// from `cut` up (within kbm), so elements that tie there sit next to each
+// other in input order.  Bit p of tmask: elements p and p + 1 tie from `cut`
+// up; bit p of dmask: they tie and p's key is the larger (out of order).  A
+// group (a maximal run of tmask bits and the element after it) holding an
+// out-of-order pair is sorted by insertion, by the one thread whose positions
+// [PER t, PER t + PER) hold the group's first element; only strictly larger
+// keys move, so equal keys keep their order.  With uniform keys the groups
+// are pairs and triples.  Returns false when a group that needs sorting is
+// longer than kFixMaxGroup (the caller then sorts the whole run on all its
+// digits; what was sorted here was sorted stably, so that stays correct).
+// flags: 2 * BLOCK / 64 words of LDS (the two block-wide ORs).
+constexpr uint32_t kFixMaxGroup = 16;
+
 template <int BLOCK, int ITEMS>
-__device__ __forceinline__ bool fix_ties(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t* flags,
-                                         uint32_t n, uint64_t kbm, uint32_t cut) {
-  constexpr int PAIRS = (BLOCK * ITEMS / 2 + BLOCK - 1) / BLOCK;  // pairs per thread
+__device__ __forceinline__ bool fix_groups(TileSmem<8, BLOCK, ITEMS>& sm, uint64_t* tmask,
+                                           uint64_t* dmask, uint32_t* flags, uint32_t n,
+                                           uint64_t kbm, uint32_t cut) {
   constexpr int W = BLOCK / kWave;
-  for (int r = 0; r < kFixRounds; ++r) {
-    int sw = 0;
-    // recomputed here rather than kept live across the kernel's loop (the
-    // compiler would otherwise hoist the pair indices and spill them)
-    uint32_t tid = threadIdx.x;
-    asm volatile("" : "+v"(tid));
+  constexpr uint32_t PER = (uint32_t)ITEMS;  // positions per thread: BLOCK * PER = the run cap
+  static_assert(PER < 32 && kWave % PER == 0, "a thread's positions lie in one mask word");
+  const uint32_t t = opaque_tid();
+  const uint32_t lane = t & (kWave - 1);
+  const uint32_t wave = t / kWave;
+  bool any = false;
 #pragma unroll
-    for (int q = 0; q < PAIRS; ++q) {
-      const uint32_t j = 2u * (tid + (uint32_t)(q * BLOCK)) + (uint32_t)(r & 1);
-      if (j + 1 < n) {
-        const uint64_t ka = sm.elems[j].key & kbm, kb = sm.elems[j + 1].key & kbm;
-        if ((ka >> cut) == (kb >> cut) && ka > kb) {
-          const elem_t x = sm.elems[j];
-          sm.elems[j] = sm.elems[j + 1];
-          sm.elems[j + 1] = x;
-          sw = 1;
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t p = (uint32_t)i * BLOCK + t;
+    bool tie = false, desc = false;
+    if (p + 1 < n) {
+      const uint64_t k0 = sm.elems[p].key & kbm, k1 = sm.elems[p + 1].key & kbm;
+      tie = (k0 >> cut) == (k1 >> cut);
+      desc = tie && k0 > k1;
+    }
+    const uint64_t bt = __ballot(tie), bd = __ballot(desc);
+    if (lane == 0) {
+      tmask[p / kWave] = bt;
+      dmask[p / kWave] = bd;
+    }
+    any |= bd != 0;
+  }
+  if (lane == 0) flags[wave] = any ? 1u : 0u;
+  lds_barrier();
+  uint32_t need = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) need |= flags[w];
+  if (!need) return true;
+
+  uint32_t too_long = 0;
+  const uint32_t p0 = t * PER;
+  if (p0 < n) {
+    const uint32_t bits = (uint32_t)(tmask[p0 / kWave] >> (p0 % kWave)) & ((1u << PER) - 1u);
+    const uint32_t prev = p0 ? (uint32_t)(tmask[(p0 - 1) / kWave] >> ((p0 - 1) % kWave)) & 1u : 0u;
+    uint32_t starts = bits & ~((bits << 1) | prev);
+    while (starts) {
+      const uint32_t s = p0 + (uint32_t)__builtin_ctz(starts);
+      starts &= starts - 1u;
+      // the group's last element: the first position from s on whose bit is
+      // clear (position n - 1 has a clear bit, so the walk stops inside the run)
+      uint32_t wi = s / kWave;
+      uint64_t x = ~tmask[wi] & (~0ull << (s % kWave));
+      while (!x) x = ~tmask[++wi];
+      const uint32_t e = wi * kWave + (uint32_t)__builtin_ctzll(x);
+      // an out-of-order pair among [s, e)?
+      bool bad = false;
+      for (uint32_t w2 = s / kWave; w2 <= (e - 1) / kWave && !bad; ++w2) {
+        uint64_t m = dmask[w2];
+        if (w2 == s / kWave) m &= ~0ull << (s % kWave);
+        if (w2 == e / kWave) m &= (1ull << (e % kWave)) - 1ull;
+        bad = m != 0;
+      }
+      if (!bad) continue;
+      if (e - s + 1u > kFixMaxGroup) {
+        too_long = 1;
+        continue;
+      }
+      for (uint32_t j = s + 1; j <= e; ++j) {
+        const elem_t v = sm.elems[j];
+        const uint64_t kv = v.key & kbm;
+        uint32_t i = j;
+        while (i > s && (sm.elems[i - 1].key & kbm) > kv) {
+          sm.elems[i] = sm.elems[i - 1];
+          --i;
         }
+        if (i != j) sm.elems[i] = v;
       }
     }
-    // a round without a swap after one of the other parity: every adjacent
-    // pair is in order
-    uint32_t* fl = flags + (r & 1) * W;
-    const bool any_w = __ballot(sw != 0) != 0;
-    if ((threadIdx.x & (kWave - 1)) == 0) fl[threadIdx.x / kWave] = any_w ? 1u : 0u;
-    lds_barrier();  // also orders this round's swaps before the next round's reads
-    uint32_t any = 0;
-#pragma unroll
-    for (int w = 0; w < W; ++w) any |= fl[w];
-    if (!any && r > 0) return true;
   }
-  return false;
+  const bool lw = __ballot(too_long != 0) != 0;
+  if (lane == 0) flags[W + wave] = lw ? 1u : 0u;
+  lds_barrier();  // also orders the groups' writes before the stores' reads
+  uint32_t fail = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) fail |= flags[W + w];
+  return fail == 0;
 }
 
-// Sorts one run (n <= BLOCK * ITEMS) from src into dst.  Thread 0 takes the
-// next run's ticket (into sm.scalar[1]) before the stores go out, so its
-// round trip does not wait behind them.
-template <int BLOCK, int ITEMS>
-__device__ __forceinline__ void finish_run(TileSmem<8, BLOCK, ITEMS>& sm, uint32_t* whist,
-                                           const elem_t* src, elem_t* dst, uint32_t n,
-                                           uint32_t key_bits, uint32_t digits,
-                                           const KeyOrder& ko, uint32_t* ticket,
-                                           uint32_t* flags, uint32_t* one_flag,
-                                           uint32_t& pass_id) {
-  const uint32_t t = threadIdx.x;
-  const uint32_t lane = t & (kWave - 1);
-  const uint32_t wrun = (t / kWave) * (kWave * ITEMS);
-  const uint32_t lim = n > wrun ? n - wrun : 0u;
-  const uint64_t kbm = low_bits(key_bits);
-  elem_t e[ITEMS];
-  run_load<BLOCK, ITEMS>(e, src, n);
-  keys_fwd<ITEMS>(e, ko);
-  uint32_t top = 0, ntop = 0;
-  const uint32_t want = digits >> 16;
-  digits &= 0xFFu;
-  for (int p = 7; p >= 0 && ntop < want; --p)
-    if ((digits >> p) & 1u) {
-      top |= 1u << p;
-      ++ntop;
-    }
-  lds_sort<BLOCK, ITEMS>(sm, whist, e, lim, n, key_bits, top, one_flag, pass_id);
-  if ((digits & ~top) &&
-      !fix_ties<BLOCK, ITEMS>(sm, flags, n, kbm, 8u * (uint32_t)__builtin_ctz(top))) {
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i)
-      if ((uint32_t)i * kWave + lane < lim) e[i] = sm.elems[wrun + (uint32_t)i * kWave + lane];
-    lds_sort<BLOCK, ITEMS>(sm, whist, e, lim, n, key_bits, digits, one_flag, pass_id);
+// A run's descriptor as the finish kernel keeps it: the run and its index.
+struct RunDesc {
+  uint64_t src, dst;
+  uint32_t n, digits, idx;
+};
+
+// The descriptor that ends a block's loop: no run has every digits bit set
+// (a run's word holds an 8-bit digit mask and its LDS pass count << 16).
+constexpr uint32_t kNoRun = 0xFFFFFFFFu;
+
+// Run `r`'s descriptor (thread 0), through the global address space so the
+// load counts against vmcnt only; the end marker past the last run.
+__device__ __forceinline__ RunDesc fetch_run(const MsdRun* runs, uint32_t r, uint32_t nr) {
+  RunDesc x = {0ull, 0ull, 0u, kNoRun, r};
+  if (r < nr) {
+    const g_u64_t* q = (const g_u64_t*)&runs[r];
+    x.src = q[0];
+    x.dst = q[1];
+    const uint64_t w = q[2];
+    x.n = (uint32_t)w;
+    x.digits = (uint32_t)(w >> 32);
   }
-  uint32_t nt = 0;
-  if (t == 0) nt = atomicAdd(ticket, 1u);
+  return x;
+}
+
+// A 64-bit value every lane holds the same copy of, in scalar registers.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+  return ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+         (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
+// Takes a ticket (thread 0): a returning atomic through the global address
+// space, so LDS waits do not wait for its round trip.
+__device__ __forceinline__ uint32_t take_ticket(uint32_t* ticket) {
+  typedef __attribute__((address_space(1))) uint32_t g_u32_t;
+  return __hip_atomic_fetch_add((g_u32_t*)ticket, 1u, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Per-block LDS of the finish kernel beside the tile buffer.
+template <int BLOCK>
+struct FinishSmem {
+  uint32_t whist[(BLOCK / kWave) * kNB];  // the waves' counter rows (LDS passes)
+  uint64_t tmask[kMsdFinishCap / kWave];  // fix_groups: tie bits
+  uint64_t dmask[kMsdFinishCap / kWave];  // fix_groups: out-of-order bits
+  uint64_t red[2 * (BLOCK / kWave)];      // proxy_sort: the waves' OR and AND of keys
+  uint32_t ord[257];                      // proxy_sort: top-window values present, then
+                                          // ordinals ([256]: rows past the run)
+  RunDesc run[3];                         // descriptors of this run, the next, the one after
+  uint32_t flags[2 * (BLOCK / kWave)];
+  uint32_t pf[2 * (BLOCK / kWave)];       // proxy_sort: large buckets, ties
+  uint32_t one;
+};
+
+// ------------------------------------------------------------ proxy_sort
+// The finish's main path.  Each element of the run gets a 32-bit proxy,
+// unique within the run: key bits below the counting digit (see below) over
+// its position in the run (kOriginBits bits).  Sorting the elements by
+// (counting digit, proxy) sorts the run by those key bits, ties in input
+// order, so no step has to be stable: elements are counted by their
+// counting digit with LDS atomics (in whatever order the atomics land), the
+// proxies are scattered by bucket, and each element's final slot is its
+// bucket's start plus the number of smaller proxies in its bucket (a few
+// compares: the buckets hold about one element each for spread keys).
+//   The key bits used: the run's varying part (from the highest bit that
+// differs within the run down), with its top 8 bits replaced by their
+// ordinal among the values that occur (a run of many small buckets of a
+// partition holds few values of its top digit; the ordinals pack them so
+// the counting buckets stay small); the counting digit is the top
+// kCountBits of that, the proxy holds the next kProxyKeyBits, and key bits
+// below those (`cut`) are left out.
+//   Returns kProxyEqual (all keys equal: the run is sorted as it is),
+// kProxyPasses (some counting bucket exceeds kMaxBucket elements, e.g.
+// heavy duplicates: the caller leaves the run to the LDS passes) or
+// kProxyDone: pos[] holds each element's slot, and fs.pf[W + w] says
+// whether wave w saw two elements tie on all the key bits used (they may
+// differ below `cut`: fix_groups sorts those groups) -- read after the
+// caller's next barrier.  The caller writes the elements to their slots
+// after that barrier.
+constexpr uint32_t kOriginBits = 13;                  // positions in a run
+constexpr uint32_t kCountBits = 13;                   // counting digit
+constexpr uint32_t kProxyKeyBits = 32 - kOriginBits;  // key bits a proxy holds below it
+constexpr uint32_t kMaxBucket = 16;
+static_assert(kMsdFinishCap == (1u << kOriginBits), "origins are the positions of a run");
+enum : uint32_t { kProxyDone = 0, kProxyPasses = 1, kProxyEqual = 2 };
+
+// OR (AND) of x over the 64 lanes, complete in lane 63: the DPP pattern of
+// wave_inclusive_scan, with lanes shifted in from outside a row reading the
+// identity (0 for OR, ~0 for AND).
+template <bool AND>
+__device__ __forceinline__ uint32_t wave_reduce_bits(uint32_t x) {
+  const int id = AND ? -1 : 0;
+  const auto op = [](uint32_t p, uint32_t q) { return AND ? (p & q) : (p | q); };
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)x, 0x111, 0xf, 0xf, false));  // row_shr:1
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)x, 0x112, 0xf, 0xf, false));  // row_shr:2
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)x, 0x114, 0xf, 0xf, false));  // row_shr:4
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)x, 0x118, 0xf, 0xf, false));  // row_shr:8
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return x;
+}
+
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ uint32_t proxy_sort(TileSmem<8, BLOCK, ITEMS>& sm,
+                                               FinishSmem<BLOCK>& fs,
+                                               const elem_t (&e)[ITEMS], uint32_t n,
+                                               uint32_t lim, uint64_t kbm, uint32_t& cut,
+                                               uint32_t& lowvary, uint32_t (&pos)[ITEMS]) {
+  constexpr int W = BLOCK / kWave;
+  constexpr uint32_t C = 1u << kCountBits;
+  constexpr uint32_t CPT = C / BLOCK;  // counters per thread in the scan
+  static_assert(BLOCK * ITEMS == (int)kMsdFinishCap && CPT * BLOCK == C && CPT % 2 == 0,
+                "one run per block, whole counters per thread");
+  // proxies [kMsdFinishCap] u32, then counters [C + 2] (C: the sentinel, C + 1:
+  // where slots past the run count, so no atomic needs a branch)
+  static_assert(4 * kMsdFinishCap + 4 * (C + 2) <= sizeof(sm.elems), "fits the tile buffer");
+  uint32_t* P = reinterpret_cast<uint32_t*>(sm.elems);
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(P + kMsdFinishCap);
+  const uint32_t t = opaque_tid();
+  const uint32_t lane = t & (kWave - 1);
+  const uint32_t wave = t / kWave;
+  const uint32_t wrun = wave * (kWave * ITEMS);
+  // bit i: row i of this lane holds an element of the run
+  uint32_t vm = 0;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) vm |= ((uint32_t)i * kWave + lane < lim ? 1u : 0u) << i;
+
+  // Counters and presence flags start clear (the caller's loop ends each
+  // run with a barrier, after which the tile buffer is free).
+  for (uint32_t i = t; i < C / 2 + 1; i += BLOCK) reinterpret_cast<uint2*>(cnt)[i] = make_uint2(0u, 0u);
+  if (t < 256) fs.ord[t] = 0u;
+
+  // The bits that vary within the run: OR xor AND of its keys.
+  uint64_t o = 0, a = ~0ull;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint64_t k = e[i].key & kbm;
+    const bool ok = (vm >> i) & 1u;
+    o |= ok ? k : 0ull;
+    a &= ok ? k : ~0ull;
+  }
+  const uint32_t olo = wave_reduce_bits<false>((uint32_t)o);
+  const uint32_t ohi = wave_reduce_bits<false>((uint32_t)(o >> 32));
+  const uint32_t alo = wave_reduce_bits<true>((uint32_t)a);
+  const uint32_t ahi = wave_reduce_bits<true>((uint32_t)(a >> 32));
+  if (lane == kWave - 1) {
+    fs.red[wave] = ((uint64_t)ohi << 32) | olo;
+    fs.red[W + wave] = ((uint64_t)ahi << 32) | alo;
+  }
+  lds_barrier();
+  uint64_t O = 0, A = ~0ull;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    O |= fs.red[w];
+    A &= fs.red[W + w];
+  }
+  const uint64_t vary = (O ^ A) & kbm;
+  if (vary == 0) return kProxyEqual;
+  const uint32_t hb = 63u - (uint32_t)__builtin_clzll(vary);
+  const uint32_t lb = (uint32_t)__builtin_ctzll(vary);
+  lowvary = lb;
+  const uint32_t w0 = hb >= 7u ? hb - 7u : 0u;  // the top window: bits [w0, w0 + 8)
+
+  // Ordinals of the top window's values that occur (rows past the run mark
+  // fs.ord[256], which the scan leaves out).
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    fs.ord[((vm >> i) & 1u) ? (uint32_t)((e[i].key & kbm) >> w0) & 0xFFu : 256u] = 1u;
+  block_exclusive_scan<256, BLOCK>(fs.ord, sm.wsum);  // opens and closes with a barrier
+  uint32_t np = 0;  // values present (>= 2: bit hb varies)
+#pragma unroll
+  for (int w = 0; w < W; ++w) np += sm.wsum[w];
+  const uint32_t ob = 32u - (uint32_t)__builtin_clz(np - 1u);  // bits of the largest ordinal
+  const uint32_t L = w0;                                      // key bits below the window
+  const uint32_t T = ob + L;                                  // bits of the varying part
+  // the counting digit: the top cb bits of the varying part; a proxy: the
+  // next pb - cb bits and the origin; key bits below `cut` are left out
+  const uint32_t cb = T < kCountBits ? T : kCountBits;
+  const uint32_t pb = T < kCountBits + kProxyKeyBits ? T : kCountBits + kProxyKeyBits;
+  cut = T - pb;  // < L: only key bits below the window are left out
+  const uint64_t lowm = L ? (1ull << L) - 1ull : 0ull;
+  const uint32_t pmask = (1u << (pb - cb)) - 1u;
+
+  // Counting: rows past the run count in cnt[C + 1], which nothing reads.
+  // Every LDS load in these loops is unconditional (rows past the run hold
+  // zero keys, whose lookups stay in bounds), so a row's load never sits
+  // behind a branch that waits for it alone.
+  uint32_t c[ITEMS], old[ITEMS], px[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint64_t k = e[i].key & kbm;
+    const uint64_t v = ((uint64_t)fs.ord[(uint32_t)(k >> w0) & 0xFFu] << L) | (k & lowm);
+    c[i] = ((vm >> i) & 1u) ? (uint32_t)(v >> (T - cb)) : C + 1u;
+    px[i] = (((uint32_t)(v >> cut) & pmask) << kOriginBits) | (wrun + (uint32_t)i * kWave + lane);
+  }
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) old[i] = atomicAdd(&cnt[c[i]], 1u);
+  lds_barrier();
+
+  // Bucket starts (CPT consecutive counters per thread); any bucket above
+  // kMaxBucket sends the run to the LDS passes.
+  uint32_t v[CPT];
+#pragma unroll
+  for (uint32_t k = 0; k < CPT; k += 2) {
+    const uint2 x = reinterpret_cast<const uint2*>(cnt)[(t * CPT + k) / 2];
+    v[k] = x.x;
+    v[k + 1] = x.y;
+  }
+  uint32_t sum = 0;
+  bool big = false;
+#pragma unroll
+  for (uint32_t k = 0; k < CPT; ++k) {
+    sum += v[k];
+    big |= v[k] > kMaxBucket;
+  }
+  const uint32_t inc = wave_inclusive_scan(sum);
+  if (lane == kWave - 1) sm.wsum[wave] = inc;
+  const bool wbig = __ballot(big) != 0;
+  if (lane == 0) fs.pf[wave] = wbig ? 1u : 0u;
+  lds_barrier();
+  uint32_t anybig = 0, run = inc - sum;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    anybig |= fs.pf[w];
+    run += (uint32_t)w < wave ? sm.wsum[w] : 0u;
+  }
+  if (anybig) return kProxyPasses;
+#pragma unroll
+  for (uint32_t k = 0; k < CPT; k += 2) {
+    reinterpret_cast<uint2*>(cnt)[(t * CPT + k) / 2] = make_uint2(run, run + v[k]);
+    run += v[k] + v[k + 1];
+  }
+  if (t == 0) cnt[C] = n;
+  lds_barrier();
+
+  // Proxies in bucket order (in whatever order within a bucket).  From
+  // here c[] holds each element's bucket start and old[] its bucket end
+  // (rows past the run: the empty bucket [0, 0)).
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const bool ok = (vm >> i) & 1u;
+    const uint32_t cc = ok ? c[i] : 0u;
+    const uint32_t b0 = cnt[cc], b1 = cnt[cc + 1u];
+    if (ok) P[b0 + old[i]] = px[i];
+    c[i] = ok ? b0 : 0u;
+    old[i] = ok ? b1 : 0u;
+  }
+  lds_barrier();
+  // Each element's final slot: its bucket's start plus the proxies of its
+  // bucket below its own; one wave-uniform loop over the buckets' slots (at
+  // most kMaxBucket), every row's load issued without a branch.  Two
+  // proxies with the same key bits (different origins) mean the window left
+  // out key bits that may differ.
+  bool tie = false;
+  uint32_t r[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) r[i] = c[i];
+#pragma nounroll
+  for (uint32_t m = 0;; ++m) {
+    uint32_t q[ITEMS];
+    // slot min(c + m, end - 1): inside the bucket while it lasts, and
+    // m < kMaxBucket for the empty bucket of rows past the run
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) q[i] = P[umin(c[i] + m, old[i] - 1u)];
+    bool more = false;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t j = c[i] + m;
+      const uint32_t act = j < old[i] ? 1u : 0u;
+      r[i] += act & (q[i] < px[i] ? 1u : 0u);
+      tie |= (act & ((q[i] >> kOriginBits) == (px[i] >> kOriginBits) ? 1u : 0u) &
+              (q[i] != px[i] ? 1u : 0u)) != 0u;
+      more |= j + 1u < old[i];
+    }
+    if (__ballot(more) == 0) break;
+  }
+  const bool wtie = __ballot(tie) != 0;
+  if (lane == 0) fs.pf[W + wave] = wtie ? 1u : 0u;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) pos[i] = r[i];
+  return kProxyDone;
+}
+
+// Stores sm.elems[0, n) (key order mapped back) to dst, coalesced.
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ void store_sorted(const TileSmem<8, BLOCK, ITEMS>& sm, elem_t* dst,
+                                             uint32_t n, const KeyOrder& ko) {
+  const uint32_t t = opaque_tid();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const uint32_t p = t + (uint32_t)i * BLOCK;
@@ -774,52 +1087,155 @@ __device__ __forceinline__ void finish_run(TileSmem<8, BLOCK, ITEMS>& sm, uint32
       gstore(dst + p, x);
     }
   }
-  if (t == 0) sm.scalar[1] = nt;
+}
+
+// LDS passes over the highest few of the run's digits (as many as its
+// buckets' size needs: finish_passes), then fix_groups; if a group is too
+// long for it, passes over every digit of the run.  e[] holds the run in
+// input order on entry; on return sm.elems[0, n) holds it sorted.
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ void passes_sort(TileSmem<8, BLOCK, ITEMS>& sm, FinishSmem<BLOCK>& fs,
+                                            elem_t (&e)[ITEMS], uint32_t n, uint32_t lim,
+                                            uint32_t key_bits, uint32_t digits,
+                                            uint32_t& pass_id) {
+  const uint32_t t = opaque_tid();
+  const uint32_t lane = t & (kWave - 1);
+  const uint32_t wrun = (t / kWave) * (kWave * ITEMS);
+  const uint32_t want = digits >> 16;
+  digits &= 0xFFu;
+  uint32_t top = 0, ntop = 0;
+  for (int p = 7; p >= 0 && ntop < want; --p)
+    if ((digits >> p) & 1u) {
+      top |= 1u << p;
+      ++ntop;
+    }
+  lds_sort<BLOCK, ITEMS>(sm, fs.whist, e, lim, n, key_bits, top, &fs.one, pass_id);
+  if ((digits & ~top) &&
+      !fix_groups<BLOCK, ITEMS>(sm, fs.tmask, fs.dmask, fs.flags, n, low_bits(key_bits),
+                                8u * (uint32_t)__builtin_ctz(top))) {
+    // every slot is read (those past the run hold stale elements that the
+    // passes ignore)
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) e[i] = sm.elems[wrun + (uint32_t)i * kWave + lane];
+    lds_sort<BLOCK, ITEMS>(sm, fs.whist, e, lim, n, key_bits, digits, &fs.one, pass_id);
+  }
+}
+
+// The run in sm.elems (its equal keys in input order) sorted again by LDS
+// passes over all of `digits`: for the rare runs whose tied groups are too
+// long for fix_groups.  Out of line, so its registers do not weigh on the
+// common path.
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ void resort_all(TileSmem<8, BLOCK, ITEMS>& sm, FinishSmem<BLOCK>& fs,
+                                           uint32_t n, uint32_t lim, uint32_t key_bits,
+                                           uint32_t digits, uint32_t& pass_id) {
+  const uint32_t t = opaque_tid();
+  const uint32_t lane = t & (kWave - 1);
+  const uint32_t wrun = (t / kWave) * (kWave * ITEMS);
+  // every slot is read (those past the run hold stale elements that the
+  // passes ignore)
+  elem_t e[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) e[i] = sm.elems[wrun + (uint32_t)i * kWave + lane];
+  lds_sort<BLOCK, ITEMS>(sm, fs.whist, e, lim, n, key_bits, digits, &fs.one, pass_id);
+}
+
+// Sorts one run (n <= BLOCK * ITEMS) from src into dst: by proxy_sort (and
+// fix_groups where ties remain), or by passes_sort for the runs proxy_sort
+// leaves (a counting bucket of more than kMaxBucket elements: heavy
+// duplicates, few varying bits) and those whose tied groups are too long for
+// fix_groups.  Thread 0 stores the descriptor of the block's run after next
+// (ticket `nt`, taken while the previous run was stored) into fs.run[nxt]
+// right after this run's loads go out: waiting for it waits for them, which
+// the run needs at once anyway.
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ void finish_run(TileSmem<8, BLOCK, ITEMS>& sm, FinishSmem<BLOCK>& fs,
+                                           const elem_t* src, elem_t* dst, uint32_t n,
+                                           uint32_t key_bits, uint32_t digits,
+                                           const KeyOrder& ko, const MsdRun* runs, uint32_t nt,
+                                           uint32_t nr, uint32_t nxt, uint32_t& pass_id) {
+  constexpr int W = BLOCK / kWave;
+  const uint32_t t = opaque_tid();
+  const uint32_t lane = t & (kWave - 1);
+  const uint32_t wrun = (t / kWave) * (kWave * ITEMS);
+  const uint32_t lim = n > wrun ? n - wrun : 0u;
+  const uint64_t kbm = low_bits(key_bits);
+  elem_t e[ITEMS];
+  run_load<BLOCK, ITEMS>(e, src, n);
+  if (t == 0) fs.run[nxt] = fetch_run(runs, nt, nr);
+  keys_fwd<ITEMS>(e, ko);
+  uint32_t cut = 0, lb = 0, pos[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) pos[i] = wrun + (uint32_t)i * kWave + lane;  // input order
+  const uint32_t how =
+      n > 1u ? proxy_sort<BLOCK, ITEMS>(sm, fs, e, n, lim, kbm, cut, lb, pos) : kProxyEqual;
+  lds_barrier();  // the proxies and counters are dead: the buffer takes the elements
+  if (how == kProxyPasses) {
+    passes_sort<BLOCK, ITEMS>(sm, fs, e, n, lim, key_bits, digits, pass_id);
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i)
+      if ((uint32_t)i * kWave + lane < lim) sm.elems[pos[i]] = e[i];
+    bool ties = false;
+    if (how == kProxyDone && cut > lb) {
+      uint32_t any = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) any |= fs.pf[W + w];
+      ties = any != 0;
+    }
+    lds_barrier();
+    if (ties && !fix_groups<BLOCK, ITEMS>(sm, fs.tmask, fs.dmask, fs.flags, n, kbm, cut))
+      resort_all<BLOCK, ITEMS>(sm, fs, n, lim, key_bits, digits & 0xFFu, pass_id);
+  }
+  store_sorted<BLOCK, ITEMS>(sm, dst, n, ko);
 }
 
 // Sorts every run inside one workgroup (runs handed out by a ticket, so
-// uneven runs balance across the grid).  A run's loads were consumed by its
-// first LDS pass before anything is stored, so a run sorted in place
-// (src == dst) needs no further wait; runs are disjoint.
+// uneven runs balance across the grid).  A run's loads were consumed before
+// anything is stored, so a run sorted in place (src == dst) needs no further
+// wait; runs are disjoint.  The descriptors of a block's current, next and
+// after-next runs rotate through fs.run[0..2]; thread 0 takes each ticket
+// two runs ahead, after the stores of a run have gone out.
 template <int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_msd_finish(MsdArgs a) {
   static_assert(BLOCK * ITEMS >= (int)kMsdFinishCap, "a run fits one workgroup");
   __shared__ TileSmem<8, BLOCK, ITEMS> sm;
-  __shared__ uint32_t s_whist[(BLOCK / kWave) * kNB];
-  __shared__ uint32_t s_flags[2 * (BLOCK / kWave)];
-  __shared__ uint32_t s_one;
+  __shared__ FinishSmem<BLOCK> fs;
   const uint32_t nr = umin(a.run_ctr[0], a.cap_runs);
   uint32_t* ticket = &a.run_ctr[1];
   uint32_t pass_id = 0;  // LDS passes of this block so far
+  uint32_t nt = 0;       // thread 0: the ticket of the run after next
   if (threadIdx.x == 0) {
-    s_one = 0u;
-    sm.scalar[1] = atomicAdd(ticket, 1u);
+    fs.one = 0u;
+    fs.run[0] = fetch_run(a.runs, take_ticket(ticket), nr);
+    fs.run[1] = fetch_run(a.runs, take_ticket(ticket), nr);
+    nt = take_ticket(ticket);
   }
   lds_barrier();
-  while (true) {
-    const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane(sm.scalar[1]);
-    lds_barrier();  // every thread has the ticket before thread 0 replaces it
-    if (r >= nr) return;
-    const MsdRun R = a.runs[r];
-    const elem_t* src = reinterpret_cast<const elem_t*>(R.src);
-    elem_t* dst = reinterpret_cast<elem_t*>(R.dst);
-    if ((R.digits & 0xFFu) == 0) {
+  for (uint32_t cur = 0;; cur = cur == 2u ? 0u : cur + 1u) {
+    const uint32_t digits = (uint32_t)__builtin_amdgcn_readfirstlane(fs.run[cur].digits);
+    if (digits == kNoRun) return;
+    const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane(fs.run[cur].n);
+    const elem_t* src = reinterpret_cast<const elem_t*>(uniform_u64(fs.run[cur].src));
+    elem_t* dst = reinterpret_cast<elem_t*>(uniform_u64(fs.run[cur].dst));
+    const uint32_t after = cur == 0u ? 2u : cur - 1u;  // slot of the run after next
+    if ((digits & 0xFFu) == 0) {
       // a copy
       elem_t e[ITEMS];
-      run_load<BLOCK, ITEMS>(e, src, R.n);
-      uint32_t nt = 0;
-      if (threadIdx.x == 0) nt = atomicAdd(ticket, 1u);
-      const uint32_t lane = threadIdx.x & (kWave - 1);
-      const uint32_t wrun = (threadIdx.x / kWave) * (kWave * ITEMS);
+      run_load<BLOCK, ITEMS>(e, src, n);
+      if (threadIdx.x == 0) fs.run[after] = fetch_run(a.runs, nt, nr);
+      const uint32_t tid = opaque_tid();
+      const uint32_t lane = tid & (kWave - 1);
+      const uint32_t wrun = (tid / kWave) * (kWave * ITEMS);
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i)
-        if (wrun + (uint32_t)i * kWave + lane < R.n) gstore(dst + wrun + (uint32_t)i * kWave + lane, e[i]);
-      if (threadIdx.x == 0) sm.scalar[1] = nt;
+        if (wrun + (uint32_t)i * kWave + lane < n) gstore(dst + wrun + (uint32_t)i * kWave + lane, e[i]);
     } else {
-      finish_run<BLOCK, ITEMS>(sm, s_whist, src, dst, R.n, a.key_bits, R.digits, a.ko, ticket,
-                               s_flags, &s_one, pass_id);
+      finish_run<BLOCK, ITEMS>(sm, fs, src, dst, n, a.key_bits, digits, a.ko, a.runs, nt, nr,
+                               after, pass_id);
     }
-    lds_barrier();  // the next ticket is in sm.scalar[1]; LDS is free again
+    if (threadIdx.x == 0) nt = take_ticket(ticket);
+    lds_barrier();  // fs.run[after] is written; LDS is free again
   }
 }
 
