@@ -832,12 +832,13 @@ __device__ __forceinline__ uint32_t take_ticket(uint32_t* ticket) {
 // Per-block LDS of the finish kernel beside the tile buffer.
 template <int BLOCK>
 struct FinishSmem {
-  uint32_t whist[(BLOCK / kWave) * kNB];  // the waves' counter rows (LDS passes)
+  uint32_t whist[(BLOCK / kWave) * kNB];  // the waves' counter rows (LDS passes; proxy_sort:
+                                          // the waves' ordinal tables)
   uint64_t tmask[kMsdFinishCap / kWave];  // fix_groups: tie bits
   uint64_t dmask[kMsdFinishCap / kWave];  // fix_groups: out-of-order bits
   uint64_t red[2 * (BLOCK / kWave)];      // proxy_sort: the waves' OR and AND of keys
-  uint32_t ord[257];                      // proxy_sort: top-window values present, then
-                                          // ordinals ([256]: rows past the run)
+  uint32_t ord[257];                      // proxy_sort: top-window values present
+                                          // ([256]: rows past the run)
   RunDesc run[3];                         // descriptors of this run, the next, the one after
   uint32_t flags[2 * (BLOCK / kWave)];
   uint32_t pf[2 * (BLOCK / kWave)];       // proxy_sort: large buckets, ties
@@ -954,14 +955,27 @@ __device__ __forceinline__ uint32_t proxy_sort(TileSmem<8, BLOCK, ITEMS>& sm,
   const uint32_t w0 = hb >= 7u ? hb - 7u : 0u;  // the top window: bits [w0, w0 + 8)
 
   // Ordinals of the top window's values that occur (rows past the run mark
-  // fs.ord[256], which the scan leaves out).
+  // fs.ord[256], which nothing reads).  Every wave turns the flags into its
+  // own table of ordinals (in its counter row, unused on this path), so no
+  // further barrier is needed: lane l covers values 4l .. 4l + 3.
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
     fs.ord[((vm >> i) & 1u) ? (uint32_t)((e[i].key & kbm) >> w0) & 0xFFu : 256u] = 1u;
-  block_exclusive_scan<256, BLOCK>(fs.ord, sm.wsum);  // opens and closes with a barrier
-  uint32_t np = 0;  // values present (>= 2: bit hb varies)
-#pragma unroll
-  for (int w = 0; w < W; ++w) np += sm.wsum[w];
+  lds_barrier();
+  uint32_t* const tab = fs.whist + wave * kNB;
+  uint32_t np;  // values present (>= 2: bit hb varies)
+  {
+    const uint32_t f0 = fs.ord[4u * lane], f1 = fs.ord[4u * lane + 1u];
+    const uint32_t f2 = fs.ord[4u * lane + 2u], f3 = fs.ord[4u * lane + 3u];
+    const uint32_t s4 = f0 + f1 + f2 + f3;
+    const uint32_t inc = wave_inclusive_scan(s4);
+    const uint32_t ex = inc - s4;
+    tab[4u * lane] = ex;
+    tab[4u * lane + 1u] = ex + f0;
+    tab[4u * lane + 2u] = ex + f0 + f1;
+    tab[4u * lane + 3u] = ex + f0 + f1 + f2;
+    np = (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
+  }
   const uint32_t ob = 32u - (uint32_t)__builtin_clz(np - 1u);  // bits of the largest ordinal
   const uint32_t L = w0;                                      // key bits below the window
   const uint32_t T = ob + L;                                  // bits of the varying part
@@ -981,7 +995,7 @@ __device__ __forceinline__ uint32_t proxy_sort(TileSmem<8, BLOCK, ITEMS>& sm,
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const uint64_t k = e[i].key & kbm;
-    const uint64_t v = ((uint64_t)fs.ord[(uint32_t)(k >> w0) & 0xFFu] << L) | (k & lowm);
+    const uint64_t v = ((uint64_t)tab[(uint32_t)(k >> w0) & 0xFFu] << L) | (k & lowm);
     c[i] = ((vm >> i) & 1u) ? (uint32_t)(v >> (T - cb)) : C + 1u;
     px[i] = (((uint32_t)(v >> cut) & pmask) << kOriginBits) | (wrun + (uint32_t)i * kWave + lane);
   }
