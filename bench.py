@@ -350,6 +350,53 @@ def traffic_summary(fetch_dir, write_dir, log2n, d, tile, cmd_fetch, cmd_write):
     print("\n".join(lines))
 
 
+def serving_classes(jobs=500, seed=77, span_log2=12, reps=3):
+    """The serving mix split by class (key width x uniform/Zipf): each class's
+    jobs sorted as one batch with lsb_sort_batch (graph-replayed, as the
+    server launches them), once in the LSD segmented form and once most
+    significant digit first; median of `reps` after one warm-up; the two
+    forms' outputs compared bit for bit.  Prints a markdown table."""
+    import torch
+    import lsb_amd
+    from lsb_amd import workloads as W
+
+    trace = W.serving_trace(jobs, seed, lo_log2=12, span_log2=span_log2)
+    dev = torch.device("cuda:0")
+    classes = {}
+    for j, job in enumerate(trace):
+        classes.setdefault((job["key_bits"], job["zipf"]), []).append((j, job))
+    print("| key bits | keys | jobs | elements | LSD form ms | Gelem/s | MSD form ms | Gelem/s "
+          "| same output |")
+    print("|---|---|---|---|---|---|---|---|---|")
+    for (kb, z), jl in sorted(classes.items()):
+        As, Bs, pris, ns = [], [], [], []
+        for j, job in jl:
+            a = torch.empty((job["n"], 2), dtype=torch.int64, device=dev)
+            lsb_amd.generate(a, kind=lsb_amd.GEN_ZIPF if z else lsb_amd.GEN_UNIFORM, ranks=4,
+                             key_bits=kb, seed_base=7000 + 16 * j, val_base=0)
+            As.append(a)
+            pris.append(a.clone())
+            ns.append(job["n"])
+            Bs.append(torch.empty_like(a) if job["n"] > 8192 else None)
+        total = sum(ns)
+        row, outs = [], {}
+        for lsd in (True, False):
+            ts = []
+            for _ in range(reps + 1):
+                for a, p in zip(As, pris):
+                    a.copy_(p)
+                ts.append(_timed(lambda: lsb_amd.sort_batch(As, Bs, ns, key_bits=kb, graph=True,
+                                                            lsd=lsd)))
+            t = sorted(ts[1:])[len(ts[1:]) // 2]
+            row += [f"{t * 1e3:.3f}", f"{total / t / 1e9:.2f}"]
+            outs[lsd] = [a.clone() for a in As]
+        same = all(torch.equal(x, y) for x, y in zip(outs[True], outs[False]))
+        print(f"| {kb} | {'Zipf(1.1)' if z else 'uniform'} | {len(jl)} | {total} | "
+              + " | ".join(row) + f" | {'yes' if same else 'NO'} |", flush=True)
+        del As, Bs, pris, outs
+        torch.cuda.empty_cache()
+
+
 def variant_table(log2ns=(24, 26, 28, 30), reps=3):
     """Every compiled plain-sort variant timed (median of `reps` after one
     warm-up, HIP events around the whole lsb_sort on its stream) on uniform
@@ -446,6 +493,10 @@ def main():
     ap.add_argument("--serving-streams", type=int, default=0,
                     help="device streams of the server (0: its default, 8); 1 runs the "
                          "coalesced launches one after another, for per-kernel profiles")
+    ap.add_argument("--serving-classes", action="store_true",
+                    help="only time each class of the 2^12..2^24 serving mix (key width x "
+                         "uniform/Zipf) as one batch in both batched forms and print a "
+                         "markdown table")
     ap.add_argument("--variant-table", action="store_true",
                     help="only time every compiled plain-sort variant at 2^24..2^30 for 32- "
                          "and 64-bit keys and print a markdown table")
@@ -470,6 +521,9 @@ def main():
 
     if args.variant_table:
         variant_table()
+        return
+    if args.serving_classes:
+        serving_classes(args.serving_jobs)
         return
 
     base = None
@@ -645,10 +699,11 @@ def main():
         out["serving"] = bench_serving(args.serving_jobs, span_log2=12,
                                        streams=args.serving_streams)
         torch.cuda.empty_cache()
-        log("serving stream, sizes 2^12..2^28 ...")
-        out["serving_wide"] = bench_serving(args.serving_wide_jobs, span_log2=16,
-                                            streams=args.serving_streams)
-        torch.cuda.empty_cache()
+        if args.serving_wide_jobs > 0:
+            log("serving stream, sizes 2^12..2^28 ...")
+            out["serving_wide"] = bench_serving(args.serving_wide_jobs, span_log2=16,
+                                                streams=args.serving_streams)
+            torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
 
 

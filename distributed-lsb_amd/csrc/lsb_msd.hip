@@ -668,7 +668,9 @@ constexpr int kRawBufferFlags = 0x00020000;  // dword 3 of a raw gfx9 buffer res
 // run, so slots past its end read zero with no branch and the loads go out
 // back to back.  The resource lives in scalar registers (readfirstlane
 // returns int: each word goes through uint32_t so that no sign extension
-// reaches the address's high word).
+// reaches the address's high word).  Each load keeps its whole offset in
+// the vector operand: the range check covers that one (and the
+// instruction's offset), not the scalar offset.
 template <int BLOCK, int ITEMS>
 __device__ __forceinline__ void run_load(elem_t (&e)[ITEMS], const elem_t* src, uint32_t n) {
   const uint32_t tid = opaque_tid();
@@ -685,9 +687,8 @@ __device__ __forceinline__ void run_load(elem_t (&e)[ITEMS], const elem_t* src, 
   const uint32_t off = (wrun + lane) * (uint32_t)sizeof(elem_t);
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    // the row's offset in the scalar operand: one address register for all rows
     const auto x = __builtin_amdgcn_raw_buffer_load_b128(
-        rs, (int)off, i * kWave * (int)sizeof(elem_t), 0);
+        rs, (int)(off + (uint32_t)(i * kWave) * (uint32_t)sizeof(elem_t)), 0, 0);
     e[i].key = ((uint64_t)x[1] << 32) | x[0];
     e[i].val = ((uint64_t)x[3] << 32) | x[2];
   }
