@@ -545,7 +545,7 @@ struct SegLayout {
   size_t hdr, meta, counts, plan, tickets, status, tileseg, perm, total;
   // MSD-first form
   uint32_t cap_entries, cap_runs, cap_wide, cap_wide_tiles;
-  size_t ment[2], mcounts[2], morr[2], mtile[2], mstatus, mlvl, mruns, mrunctr;
+  size_t ment[2], mcounts[2], morr[2], mtile[2], mstatus, mlvl, mruns, mrunctr, mrg;
   size_t went, wcounts, worr, wtile, wstatus, wlvl;
 };
 
@@ -610,6 +610,7 @@ int make_msd_layout(size_t cap_n, SegLayout* L) {
   L->mlvl = off;    off += align_up(lsb::kMsdMaxLevels * sizeof(lsb::MsdLevel));
   L->mruns = off;   off += align_up(runs * sizeof(lsb::MsdRun));
   L->mrunctr = off; off += align_up(16);
+  L->mrg = off;     off += align_up((lsb::kRegions * 256 + lsb::kRegions) * 4);
   const size_t wnb = (size_t)1 << lsb::kMsdWideBits;
   L->went = off;    off += align_up(wide * sizeof(lsb::MsdEntry));
   L->wcounts = off; off += align_up(wide * wnb * 4);
@@ -750,6 +751,7 @@ int enqueue_segmented(const SegLayout& L, char* ws, uint32_t flags, hipStream_t 
     m.lvl = reinterpret_cast<lsb::MsdLevel*>(ws + L.mlvl);
     m.runs = reinterpret_cast<lsb::MsdRun*>(ws + L.mruns);
     m.run_ctr = reinterpret_cast<uint32_t*>(ws + L.mrunctr);
+    m.rg = reinterpret_cast<uint32_t*>(ws + L.mrg);
     m.cap_entries = L.cap_entries;
     m.cap_tiles = L.max_tiles;
     m.cap_runs = L.cap_runs;

@@ -213,6 +213,11 @@ constexpr int kMsdWideSteps = 3;
 constexpr uint32_t kMsdInB = 1u;   // its elements are in the scratch buffer
 constexpr uint32_t kMsdCopy = 2u;  // all keys equal and in B: copy to A at this level
 constexpr uint32_t kMsdMove = 4u;  // partitioned on `dig` at this level (set by the plan)
+// A level 0 of one range of more than kMsdRegionMin elements (a plain sort's
+// array) is partitioned in the region form: kRegions look-back chains side
+// by side, each over a contiguous slice of the range's tiles, with bucket
+// bases from per-region counts the level's histogram takes.
+constexpr uint32_t kMsdRegionMin = kMsdWideMax;
 
 struct MsdEntry {
   uint64_t a;           // the range in the final buffer
@@ -247,6 +252,8 @@ struct MsdArgs {
   MsdLevel* lvl;           // [kMsdMaxLevels]
   MsdRun* runs;
   uint32_t* run_ctr;       // [0] runs emitted, [1] finish ticket, [2] overflow
+  uint32_t* rg;            // region form of level 0: [kRegions][256] counts, then
+                           // [kRegions] tickets
   uint32_t cap_entries, cap_tiles, cap_runs;
   // Wide partitions (kMsdWideBits-bit digit) of ranges of kMsdWideMin ..
   // kMsdWideMax elements, in kMsdWideSteps steps: step 0 takes the segments

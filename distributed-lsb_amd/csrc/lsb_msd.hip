@@ -145,6 +145,7 @@ __global__ __launch_bounds__(256) void k_msd_init(MsdArgs a) {
       a.lvl[t] = z;
     }
     if (t < 4) a.run_ctr[t] = 0u;
+    for (uint32_t i = t; i < (uint32_t)(kRegions * kNB + kRegions); i += 256) a.rg[i] = 0u;
   }
   if (blockIdx.x == 0 && t == 0) {
     MsdLevel w = {umin(a.hdr->n_wide, a.cap_wide), umin(a.hdr->wide_tiles, a.cap_wide_tiles),
@@ -182,6 +183,23 @@ __global__ __launch_bounds__(256) void k_msd_init(MsdArgs a) {
   }
 }
 
+// ------------------------------------------------------------ region form
+// Tiles per region when a range of n elements is cut into kRegions slices of
+// whole tiles (the last ones may hold fewer or none).
+__device__ __forceinline__ uint32_t msd_region_tiles(uint32_t n) {
+  const uint32_t nt = (n + kMsdTile - 1) / kMsdTile;
+  return (nt + kRegions - 1) / kRegions;
+}
+
+// Whether level k is partitioned in the region form: level 0 holding one
+// range of more than kMsdRegionMin elements (a plain sort's array; a wide
+// segment's emptied entry holds none).  Histogram, scatter and region
+// scatter decide it alike, from the device's own level table.
+__device__ __forceinline__ bool msd_region_level(const MsdArgs& a, uint32_t k) {
+  return k == 0 && umin(a.lvl[0].entries, a.cap_entries) == 1u &&
+         a.ent[0][0].n > kMsdRegionMin;
+}
+
 // ----------------------------------------------------------- k_msd_hist
 // Each block owns a contiguous range of the level's tiles, counts each
 // range's partition digit in LDS and accumulates the OR and the AND of its
@@ -192,13 +210,16 @@ __global__ __launch_bounds__(256) void k_msd_init(MsdArgs a) {
 // loads wait on no other load.
 // NB = 256: each range's own 8-bit digit E.dig; NB = 2^kMsdWideBits (a
 // wide partition): kMsdWideBits bits from bit E.dig up.
+// With `reg` (a level 0 partitioned in the region form: one range) the
+// counts are also added per region of the range's tiles into reg[r][NB].
 template <int NB>
 __device__ __forceinline__ void msd_hist(const MsdEntry* __restrict__ ent,
                                          uint32_t* __restrict__ counts,
                                          uint32_t* __restrict__ orr,
                                          const uint32_t* __restrict__ tilemap,
                                          uint32_t* __restrict__ status, uint32_t tiles,
-                                         uint32_t nent, uint32_t key_bits, const KeyOrder& ko) {
+                                         uint32_t nent, uint32_t key_bits, const KeyOrder& ko,
+                                         uint32_t* __restrict__ reg = nullptr) {
   constexpr int U = 8;
   constexpr int PER = NB / 256;  // counters per thread
   const uint32_t t = threadIdx.x;
@@ -217,10 +238,11 @@ __device__ __forceinline__ void msd_hist(const MsdEntry* __restrict__ ent,
   for (int i = 0; i < PER; ++i) h[t + 256 * i] = 0u;
   if (t < 4) s_or[t] = t < 2 ? 0u : ~0u;
   const uint64_t kbm = low_bits(key_bits);
-  uint32_t cur = 0xFFFFFFFFu;  // range of the counts in h
+  uint32_t cur = 0xFFFFFFFFu;     // range of the counts in h
+  uint32_t cur_rg = 0xFFFFFFFFu;  // and their region (region form)
   uint64_t acc_or = 0, acc_and = ~0ull;
   MsdEntry E = {0ull, 0ull, 0u, 0xFFFFFFFFu, 0u, kMsdCopy};
-  auto flush = [&](uint32_t e) {
+  auto flush = [&](uint32_t e, uint32_t rg) {
     acc_or &= kbm;
     acc_and |= ~kbm;
     if ((uint32_t)acc_or) atomicOr(&s_or[0], (uint32_t)acc_or);
@@ -233,6 +255,7 @@ __device__ __forceinline__ void msd_hist(const MsdEntry* __restrict__ ent,
     for (int i = 0; i < PER; ++i) {
       const uint32_t c = h[t + 256 * i];
       if (c && e < nent) atomicAdd(&counts[(size_t)e * NB + t + 256 * i], c);
+      if (c && reg && rg < (uint32_t)kRegions) atomicAdd(&reg[rg * NB + t + 256 * i], c);
       h[t + 256 * i] = 0u;
     }
     if (t < 4) {
@@ -253,12 +276,21 @@ __device__ __forceinline__ void msd_hist(const MsdEntry* __restrict__ ent,
     for (uint32_t tl = c0; tl < c1; ++tl) {
       const uint32_t e = s_tm[tl - c0];
       if (e != cur) {
-        if (cur != 0xFFFFFFFFu) flush(cur);
+        if (cur != 0xFFFFFFFFu) flush(cur, cur_rg);
         cur = e;
+        cur_rg = 0xFFFFFFFFu;
         if (e < nent) E = ent[e];
       }
       if (e >= nent || (E.flags & kMsdCopy)) continue;
       if (tl < E.tile_begin || (uint64_t)(tl - E.tile_begin) * kMsdTile >= E.n) continue;
+      if (reg) {
+        const uint32_t rt = msd_region_tiles(E.n);
+        const uint32_t rg = (tl - E.tile_begin) / rt;
+        if (rg != cur_rg) {
+          if (cur_rg != 0xFFFFFFFFu) flush(cur, cur_rg);
+          cur_rg = rg;
+        }
+      }
       const elem_t* src = reinterpret_cast<const elem_t*>((E.flags & kMsdInB) ? E.b : E.a);
       const uint32_t shift = NB == kNB ? E.dig * 8u : E.dig;
       const uint32_t mask = NB == kNB ? digit_mask(E.dig, key_bits) : (uint32_t)NB - 1u;
@@ -297,14 +329,14 @@ __device__ __forceinline__ void msd_hist(const MsdEntry* __restrict__ ent,
       }
     }
   }
-  if (cur != 0xFFFFFFFFu) flush(cur);
+  if (cur != 0xFFFFFFFFu) flush(cur, cur_rg);
 }
 
 __global__ __launch_bounds__(256) void k_msd_hist(MsdArgs a) {
   const uint32_t k = a.level, L = k & 1u;
   msd_hist<kNB>(a.ent[L], a.counts[L], a.orr[L], a.tilemap[L], a.status,
                 umin(a.lvl[k].tiles, a.cap_tiles), umin(a.lvl[k].entries, a.cap_entries),
-                a.key_bits, a.ko);
+                a.key_bits, a.ko, msd_region_level(a, k) ? a.rg : nullptr);
 }
 
 // A wide step's histogram: each range's kMsdWideBits-bit digit.
@@ -536,6 +568,7 @@ __global__ __launch_bounds__(BLOCK) void k_msd_scatter(MsdArgs a) {
   constexpr bool kOwn = BLOCK >= 512;
   __shared__ uint32_t s_whist[kOwn ? (BLOCK / kWave) * kNB : 1];
   const uint32_t k = a.level, L = k & 1u;
+  if (msd_region_level(a, k)) return;  // k_msd_scatter_rg partitions it
   const uint32_t tiles = umin(a.lvl[k].tiles, a.cap_tiles);
   const uint32_t nent = umin(a.lvl[k].entries, a.cap_entries);
   uint32_t* ticket = &a.lvl[k].ticket;
@@ -571,6 +604,61 @@ __global__ __launch_bounds__(BLOCK) void k_msd_scatter(MsdArgs a) {
     }
     if (!took && threadIdx.x == 0) sm.scalar[3] = atomicAdd(ticket, 1u);
     __syncthreads();
+  }
+}
+
+// ------------------------------------------------------ k_msd_scatter_rg
+// The region form of a level-0 range (msd_region_level): the range's tiles
+// are cut into kRegions slices; block b serves slice b % kRegions with its
+// own ticket counter and look-back chain (the slice's first tile publishes
+// an inclusive word), so kRegions chains advance side by side instead of
+// one, as in the LSD passes' region form.  A slice's bucket bases are the
+// range's bucket starts (the plan's) plus the bucket's counts in earlier
+// slices (the histogram's per-region counts).  The output is the same
+// stable partition k_msd_scatter writes.
+template <int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(3))) void k_msd_scatter_rg(MsdArgs a) {
+  static_assert(BLOCK * ITEMS == (int)kMsdTile, "one onesweep tile per level tile");
+  constexpr int W = BLOCK / kWave;
+  __shared__ TileSmem<8, BLOCK, ITEMS> sm;
+  __shared__ uint32_t s_rbase[kNB];
+  __shared__ uint32_t s_whist[W * kNB];
+  if (!msd_region_level(a, 0)) return;
+  const MsdEntry E = a.ent[0][0];
+  if (!(E.flags & kMsdMove)) return;
+  const uint32_t region = blockIdx.x % (uint32_t)kRegions;
+  const uint32_t nt = (E.n + kMsdTile - 1) / kMsdTile;
+  const uint32_t rt = msd_region_tiles(E.n);
+  const uint32_t t_begin = region * rt;
+  if (t_begin >= nt) return;
+  const uint32_t t_count = umin(rt, nt - t_begin);
+  const uint32_t* starts = a.counts[0];  // entry 0: bucket starts after the plan
+  for (uint32_t b = threadIdx.x; b < (uint32_t)kNB; b += BLOCK) {
+    uint32_t before = 0;
+    for (uint32_t r = 0; r < region; ++r) before += a.rg[r * kNB + b];
+    s_rbase[b] = starts[b] + before;
+  }
+  for (uint32_t i = threadIdx.x; i < (uint32_t)(W * kNB); i += BLOCK) s_whist[i] = 0u;
+  uint32_t* ticket = a.rg + kRegions * kNB + region;
+  if (threadIdx.x == 0) sm.scalar[0] = atomicAdd(ticket, 1u);
+  lds_barrier();
+  const bool in_b = (E.flags & kMsdInB) != 0;
+  const elem_t* src = reinterpret_cast<const elem_t*>(in_b ? E.b : E.a);
+  elem_t* dst = reinterpret_cast<elem_t*>(in_b ? E.a : E.b);
+  const uint32_t shift = E.dig * 8u, mask = digit_mask(E.dig, a.key_bits);
+  uint32_t* st = a.status + (size_t)E.tile_begin * kNB;  // cleared by the histogram
+  while (true) {
+    const uint32_t local = sm.scalar[0];
+    if (local >= t_count) break;
+    const uint32_t tile = t_begin + local;
+    elem_t e[ITEMS];
+    tile_load<BLOCK, ITEMS>(e, src, E.n, tile);
+    keys_fwd<ITEMS>(e, a.ko);
+    onesweep_tile_region<8, BLOCK, ITEMS>(sm, e, dst, E.n, tile, local == 0, st, nullptr,
+                                          s_rbase, shift, mask, ticket, nullptr, 0u, 0u,
+                                          rt * kMsdTile, a.ko, s_whist, false);
+    lds_barrier();
   }
 }
 
@@ -1529,6 +1617,13 @@ hipError_t launch_msd_level(const MsdArgs& a, uint32_t cus, hipStream_t s, hipEv
   if ((e = mark(after, 1, s)) != hipSuccess) return e;
   hipLaunchKernelGGL((k_msd_scatter<kScatterBlock, kScatterItems>), dim3(scatter_occ * cus),
                      dim3(kScatterBlock), 0, s, a);
+  if (a.level == 0) {
+    // the region form of a level 0 of one large range (no-op otherwise)
+    static const uint32_t rg_occ =
+        resident_blocks(k_msd_scatter_rg<kScatterBlock, kScatterItems>, kScatterBlock);
+    hipLaunchKernelGGL((k_msd_scatter_rg<kScatterBlock, kScatterItems>), dim3(rg_occ * cus),
+                       dim3(kScatterBlock), 0, s, a);
+  }
   if ((e = mark(after, 2, s)) != hipSuccess) return e;
   return hipGetLastError();
 }

@@ -456,7 +456,9 @@ __device__ __forceinline__ void onesweep_tile_region(
     whist = reinterpret_cast<uint32_t*>(sm.elems);
     for (uint32_t i = t; i < (uint32_t)(W * NB); i += BLOCK) whist[i] = 0u;
   }
-  for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) st_next[(size_t)tile * NB + b] = 0u;
+  // (null st_next: no later pass reads this chain's other slice)
+  if (st_next)
+    for (uint32_t b = t; b < (uint32_t)NB; b += BLOCK) st_next[(size_t)tile * NB + b] = 0u;
   const uint32_t tile_base = tile * (uint32_t)T;
   const uint32_t tile_n = n - tile_base < (uint32_t)T ? n - tile_base : (uint32_t)T;
   const uint32_t wrun = wave * (kWave * ITEMS);
