@@ -212,7 +212,7 @@ def lsd_summary(n, wall, profs, d, tile, P, log2n):
 
 
 MSD_KERNELS = {"w": ("k_msd_whist", "k_msd_wplan", "k_msd_wscatter"),
-               "l": ("k_msd_hist", "k_msd_plan", "k_msd_scatter")}
+               "l": ("k_msd_hist", "k_msd_plan", "k_msd_scatter (or _rg)")}
 
 
 def msd_launches(profs):
@@ -272,8 +272,12 @@ def traffic_summary(fetch_dir, write_dir, log2n, d, tile, cmd_fetch, cmd_write):
     n = 1 << log2n
     kib = 16 * n / 1024.0
 
+    def base(name):  # "void lsb::k_msd_scatter<1024, 8>(...)" -> "k_msd_scatter"
+        return re.split(r"[<(]", name)[0].split("::")[-1].split(" ")[-1]
+
     def pick(per, pat):
-        return [(i, v) for (k, i), v in sorted(per.items(), key=lambda kv: kv[0][1]) if pat in k]
+        return [(i, v) for (k, i), v in sorted(per.items(), key=lambda kv: kv[0][1])
+                if (pat in k if pat == "copyBuffer" else base(k) == pat)]
 
     def copy_of(per):
         vals = [v for _, v in pick(per, "copyBuffer") if v > 0.5 * kib]
@@ -319,9 +323,11 @@ def traffic_summary(fetch_dir, write_dir, log2n, d, tile, cmd_fetch, cmd_write):
     print("\n".join(lines))
 
     # MSD form: per kernel, its largest dispatch (the one step that moves the
-    # array: level 0's scatter, wide step 1's, the finish)
+    # array: level 0's partition, in the region form at this size; wide step
+    # 1's; the finish)
     rows, keys = [], []
-    for k, alg_r, alg_w in (("k_msd_hist", 8, 0), ("k_msd_scatter", 16, 16),
+    for k, alg_r, alg_w in (("k_msd_hist", 8, 0), ("k_msd_scatter_rg", 16, 16),
+                            ("k_msd_scatter", 16, 16),
                             ("k_msd_whist", 8, 0), ("k_msd_wscatter", 16, 16),
                             ("k_msd_finish", 16, 16)):
         fd = pick(fetch, k)
@@ -329,6 +335,8 @@ def traffic_summary(fetch_dir, write_dir, log2n, d, tile, cmd_fetch, cmd_write):
         if not fd:
             continue
         i, v = max(fd, key=lambda x: x[1])
+        if v < 0.01 * kib:  # no dispatch of this kernel moved the array
+            continue
         rb = sf * v * 1024
         wb = sw * wd.get(i, 0.0) * 1024
         ar, aw = alg_r * n, alg_w * n
