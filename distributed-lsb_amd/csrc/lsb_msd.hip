@@ -942,8 +942,9 @@ struct FinishSmem {
 // order, so no step has to be stable: elements are counted by their
 // counting digit with LDS atomics (in whatever order the atomics land), the
 // proxies are scattered by bucket, and each element's final slot is its
-// bucket's start plus the number of smaller proxies in its bucket (a few
-// compares: the buckets hold about one element each for spread keys).
+// bucket's start plus the number of smaller proxies in its bucket (none or
+// one compare for buckets of one or two elements, the common case for
+// spread keys; larger buckets through a per-wave list).
 //   The key bits used: the run's varying part (from the highest bit that
 // differs within the run down), with its top 8 bits replaced by their
 // ordinal among the values that occur (a run of many small buckets of a
@@ -1129,50 +1130,69 @@ __device__ __forceinline__ uint32_t proxy_sort(TileSmem<8, BLOCK, ITEMS>& sm,
   lds_barrier();
 
   // Proxies in bucket order (in whatever order within a bucket).  From
-  // here c[] holds each element's bucket start and old[] its bucket end
-  // (rows past the run: the empty bucket [0, 0)).
+  // here c[] packs each element's bucket start (13 bits), its bucket's size
+  // (5 bits: at most kMaxBucket) and its arrival index in the bucket (4
+  // bits); rows past the run hold the empty bucket at 0.
+  static_assert(kMaxBucket <= 16u && kMsdFinishCap <= (1u << 13), "the packing below");
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const bool ok = (vm >> i) & 1u;
     const uint32_t cc = ok ? c[i] : 0u;
     const uint32_t b0 = cnt[cc], b1 = cnt[cc + 1u];
     if (ok) P[b0 + old[i]] = px[i];
-    c[i] = ok ? b0 : 0u;
-    old[i] = ok ? b1 : 0u;
+    c[i] = ok ? b0 | ((b1 - b0) << 13) | (old[i] << 18) : 0u;
   }
   lds_barrier();
   // Each element's final slot: its bucket's start plus the proxies of its
-  // bucket below its own; one wave-uniform loop over the buckets' slots (at
-  // most kMaxBucket), every row's load issued without a branch.  Two
-  // proxies with the same key bits (different origins) mean the window left
-  // out key bits that may differ.
+  // bucket below its own.  A bucket of one element needs nothing and one of
+  // two a single compare (most elements: the buckets hold about 0.7
+  // elements on average); the elements of larger buckets go to a list of
+  // the wave's own in LDS that its lanes then work through, so the wave
+  // loops over those buckets only instead of over its largest bucket for
+  // every row.  One wave's LDS operations complete in order, so the list
+  // and its results need no barrier.  Two proxies with the same key bits
+  // (different origins) mean the window left out key bits that may differ.
+  static_assert(4 * (kMsdFinishCap + C + 4) + 4 * W * kWave * ITEMS <= sizeof(sm.elems),
+                "the lists fit the tile buffer beside the proxies and counters");
+  uint32_t* lst = P + kMsdFinishCap + (C + 4u) + wave * (uint32_t)(kWave * ITEMS);
   bool tie = false;
-  uint32_t r[ITEMS];
+  uint32_t nl = 0;  // entries of the wave's list (the same in every lane)
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) r[i] = c[i];
-#pragma nounroll
-  for (uint32_t m = 0;; ++m) {
-    uint32_t q[ITEMS];
-    // slot min(c + m, end - 1): inside the bucket while it lasts, and
-    // m < kMaxBucket for the empty bucket of rows past the run
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) q[i] = P[umin(c[i] + m, old[i] - 1u)];
-    bool more = false;
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t j = c[i] + m;
-      const uint32_t act = j < old[i] ? 1u : 0u;
-      r[i] += act & (q[i] < px[i] ? 1u : 0u);
-      tie |= (act & ((q[i] >> kOriginBits) == (px[i] >> kOriginBits) ? 1u : 0u) &
-              (q[i] != px[i] ? 1u : 0u)) != 0u;
-      more |= j + 1u < old[i];
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t b0 = c[i] & 0x1FFFu, sz = (c[i] >> 13) & 31u, a = c[i] >> 18;
+    const bool two = sz == 2u;
+    const uint32_t q = P[b0 + (two ? 1u - a : 0u)];  // the other element of a pair
+    pos[i] = b0 + (two && q < px[i] ? 1u : 0u);
+    tie |= two && (q >> kOriginBits) == (px[i] >> kOriginBits);
+    const bool big = sz > 2u;
+    const uint64_t bm = __ballot(big);
+    if (big) {
+      const uint32_t k = nl + count_below(bm);
+      lst[k] = c[i] | (((uint32_t)i * kWave + lane) << 22);
+      pos[i] = k;  // the list entry, replaced by its result below
     }
-    if (__ballot(more) == 0) break;
+    nl += (uint32_t)__popcll(bm);
   }
+  for (uint32_t k0 = 0; k0 < nl; k0 += kWave) {
+    const uint32_t k = k0 + lane;
+    if (k < nl) {
+      const uint32_t v = lst[k];
+      const uint32_t b0 = v & 0x1FFFu, sz = (v >> 13) & 31u, a = (v >> 18) & 15u;
+      const uint32_t me = P[b0 + a];
+      uint32_t rk = 0;
+      for (uint32_t m = 0; m < sz; ++m) {
+        const uint32_t q = P[b0 + m];
+        rk += q < me ? 1u : 0u;
+        tie |= m != a && (q >> kOriginBits) == (me >> kOriginBits);
+      }
+      lst[k] = b0 + rk;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if (((c[i] >> 13) & 31u) > 2u) pos[i] = lst[pos[i]];
   const bool wtie = __ballot(tie) != 0;
   if (lane == 0) fs.pf[W + wave] = wtie ? 1u : 0u;
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) pos[i] = r[i];
   return kProxyDone;
 }
 
