@@ -1044,13 +1044,18 @@ __device__ __forceinline__ uint32_t proxy_sort(TileSmem<8, BLOCK, ITEMS>& sm,
   lowvary = lb;
   const uint32_t w0 = hb >= 7u ? hb - 7u : 0u;  // the top window: bits [w0, w0 + 8)
 
-  // Ordinals of the top window's values that occur (rows past the run mark
-  // fs.ord[256], which nothing reads).  Every wave turns the flags into its
+  // Ordinals of the top window's values that occur (lanes past the run in
+  // its last row mark fs.ord[256], which nothing reads).  Every wave turns the flags into its
   // own table of ordinals (in its counter row, unused on this path), so no
   // further barrier is needed: lane l covers values 4l .. 4l + 3.
+  // Rows wholly past the run are skipped by a branch on the wave-uniform
+  // limit: their 64 lanes would all hit one LDS word, which the LDS takes
+  // one lane at a time.
+  const uint32_t ulim = (uint32_t)__builtin_amdgcn_readfirstlane(lim);
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
-    fs.ord[((vm >> i) & 1u) ? (uint32_t)((e[i].key & kbm) >> w0) & 0xFFu : 256u] = 1u;
+    if ((uint32_t)i * kWave < ulim)
+      fs.ord[((vm >> i) & 1u) ? (uint32_t)((e[i].key & kbm) >> w0) & 0xFFu : 256u] = 1u;
   lds_barrier();
   uint32_t* const tab = fs.whist + wave * kNB;
   uint32_t np;  // values present (>= 2: bit hb varies)
@@ -1077,10 +1082,11 @@ __device__ __forceinline__ uint32_t proxy_sort(TileSmem<8, BLOCK, ITEMS>& sm,
   const uint64_t lowm = L ? (1ull << L) - 1ull : 0ull;
   const uint32_t pmask = (1u << (pb - cb)) - 1u;
 
-  // Counting: rows past the run count in cnt[C + 1], which nothing reads.
-  // Every LDS load in these loops is unconditional (rows past the run hold
-  // zero keys, whose lookups stay in bounds), so a row's load never sits
-  // behind a branch that waits for it alone.
+  // Counting: lanes past the run in its last row count in cnt[C + 1], which
+  // nothing reads; rows wholly past it skip the atomic (the branch is
+  // wave-uniform).  Every LDS load in these loops is unconditional (rows past
+  // the run hold zero keys, whose lookups stay in bounds), so a row's load
+  // never sits behind a branch that waits for it alone.
   uint32_t c[ITEMS], old[ITEMS], px[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
@@ -1090,7 +1096,10 @@ __device__ __forceinline__ uint32_t proxy_sort(TileSmem<8, BLOCK, ITEMS>& sm,
     px[i] = (((uint32_t)(v >> cut) & pmask) << kOriginBits) | (wrun + (uint32_t)i * kWave + lane);
   }
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) old[i] = atomicAdd(&cnt[c[i]], 1u);
+  for (int i = 0; i < ITEMS; ++i) {
+    old[i] = 0u;
+    if ((uint32_t)i * kWave < ulim) old[i] = atomicAdd(&cnt[c[i]], 1u);
+  }
   lds_barrier();
 
   // Bucket starts (CPT consecutive counters per thread); any bucket above
